@@ -1,0 +1,237 @@
+"""Benchmark: exact-mode (double-double) output Msamples/s of the overlap-save resampler on one MI355X.
+
+Workload: stereo 96000 Hz -> 44100 Hz, 180 s (17,280,000 frames per channel) of float64 uniform noise in
+[-0.5, 0.5] (channel c from numpy default_rng(c + 1)), Kaiser beta 40, half-length 0.2 s, LPC edges, the
+reference's automatic block (nb = 630, 214 blocks). One step = the engine over every (channel, block) task, from
+the padded input resident in HBM to the output in HBM (the reference's `process_seconds` region).
+
+Prints one JSON line. Beside the GPU figure it times the reference's own CPU engine (oracle/_ref, built from the
+reference's sources) on the same workload on this host, and reports the roofline of the dominant kernel.
+"""
+import argparse
+import json
+import os
+import re
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+RATE_IN, RATE_OUT, SECONDS, CH = 96000, 44100, 180, 2
+FP64_PEAK_TFLOPS = 78.6   # MI355X published FP64 peak
+HBM_PEAK_GBS = 8000.0
+
+
+# ---- algorithmic FP64 flop counts of the reference's plain-dd / FP64 algorithm --------------------------------
+def _radices(n):
+    rs, t = [], n
+    while t % 4 == 0:
+        rs.append(4); t //= 4
+    while t % 2 == 0:
+        rs.append(2); t //= 2
+    for f in (3, 5, 7):
+        while t % f == 0:
+            rs.append(f); t //= f
+    assert t == 1
+    return rs
+
+
+def _costs(dd, accurate):
+    """(add, mul) flop costs: dd mul 10 (FMA = 2), dd add 11 (one-TwoSum) or 20 (accurate); FP64 1 and 1."""
+    if not dd:
+        return 1, 1
+    return (20 if accurate else 11), 10
+
+
+def sfft_flops(n, dd, accurate):
+    add, mul = _costs(dd, accurate)
+    cmul = 4 * mul + 2 * add
+    total, rem, s = 0, n, 1
+    for r in _radices(n):
+        m = rem // r
+        nb, tw = m * s, (m - 1) * s   # butterflies, twiddled butterflies (p != 0)
+        if r == 4:
+            total += nb * 16 * add + tw * 3 * cmul
+        elif r == 2:
+            total += nb * 4 * add + tw * cmul
+        else:
+            h = (r - 1) // 2
+            total += nb * ((6 * h + 4 * h * h + 4 * h) * add + 4 * h * h * mul) + tw * (r - 1) * cmul
+        rem //= r
+        s *= r
+    return total
+
+
+def cfft_pass_flops(n, dd):
+    """Flops of the (column, row) passes of the reference's complex FFT of length n (four-step above 2048)."""
+    if n > 2048:
+        n1 = max(d for d in range(1, int(n ** 0.5) + 1) if n % d == 0)
+        n2 = n // n1
+        if n1 >= 8:
+            add, mul = _costs(dd, False)
+            col = n2 * sfft_flops(n1, dd, False) + (n1 - 1) * n2 * (4 * mul + 2 * add)
+            row = n1 * sfft_flops(n2, dd, False)
+            return col, row
+    return 0, sfft_flops(n, dd, True)
+
+
+def block_flops(Nb, Mb, dd, up):
+    """Per (channel, block) task: forward col/row, split+multiply+fold, inverse col/row."""
+    add, mul = _costs(dd, True)
+    fc, fr = cfft_pass_flops(Nb // 2, dd)
+    ic, ir = cfft_pass_flops(Mb // 2, dd)
+    K = Mb // 2 + 1
+    fold = 0 if up else max(0, min(Nb // 2, Mb - 1) - K + 1)
+    mid = (Nb // 4) * (12 * add + 12 * mul) + (Mb // 4) * (12 * add + 8 * mul) + K * 2 * mul + fold * (2 * mul + 2 * add)
+    return {"fwd_col": fc, "fwd_row": fr, "mid": mid, "inv_col": ic, "inv_row": ir + Mb}
+
+
+def cpu_baseline(x, threads, workdir):
+    """The reference's engine (oracle/_ref/kbsrc, compiled from its sources) on the same input, on this host:
+    its `-v` line reports the resample time without design and file I/O (app/main.cpp:144-145)."""
+    ref = os.path.join(ROOT, "oracle", "_ref", "kbsrc")
+    if not os.path.exists(ref):
+        return None
+    from tests.helpers import wav_f64
+    src, dst = os.path.join(workdir, "c2_in.wav"), os.path.join(workdir, "c2_out.wav")
+    with open(src, "wb") as f:
+        f.write(wav_f64(x, CH, RATE_IN))
+    times = []
+    for _ in range(3):
+        r = subprocess.run([ref, src, dst, "--rate", str(RATE_OUT), "-t", str(threads), "-v"], capture_output=True,
+                           text=True, timeout=300)
+        m = re.search(r"design ([0-9.]+)s\s+resample ([0-9.]+)s", r.stderr)
+        if r.returncode or not m:
+            return None
+        times.append((float(m.group(2)), float(m.group(1))))
+        print(f"cpu reference run: resample {times[-1][0]:.3f} s (threads {threads})", file=sys.stderr, flush=True)
+    times.sort()
+    eng, design = times[len(times) // 2]
+    return eng, design
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--mode", default="exact", choices=["exact", "fast"])
+    ap.add_argument("--cpu-threads", type=int, default=0, help="reference CPU engine threads (0: min(16, cores))")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}): the path shards by block ranges with no "
+              "exchange, see DESIGN.md; run with --gpus 1", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    import kbsrc_amd
+    from tests.helpers import c2_input
+
+    frames = RATE_IN * SECONDS
+    x = c2_input(frames)
+    t0 = time.perf_counter()
+    plan = kbsrc_amd.Plan(frames, CH, RATE_IN, RATE_OUT, args.mode)
+    t_design = time.perf_counter() - t0
+    info = plan.info
+    dev = torch.device("cuda:0")
+    xx = torch.empty(CH * info["tot"], dtype=torch.float64, device=dev)
+    out = torch.empty(CH * info["nout"], dtype=torch.float64, device=dev)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    t0 = time.perf_counter()
+    plan.stage(x, xx.data_ptr(), stream)
+    torch.cuda.synchronize()
+    t_stage = time.perf_counter() - t0
+    print(f"plan: nb={info['nb']} Nb={info['Nb']} Mb={info['Mb']} tasks={info['tasks']} "
+          f"design {t_design:.3f}s stage {t_stage:.3f}s", file=sys.stderr, flush=True)
+
+    for _ in range(args.warmup):
+        plan.execute(xx.data_ptr(), out.data_ptr(), stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        plan.execute(xx.data_ptr(), out.data_ptr(), stream)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    nout_total = CH * info["nout"]
+    value = nout_total / (ms / 1e3) / 1e6
+    print(f"engine {ms:.3f} ms/step -> {value:.1f} output Msamples/s", file=sys.stderr, flush=True)
+
+    # per-kernel HIP-event times on the launch stream, averaged over a few timed executes
+    reps = 3
+    acc = None
+    for _ in range(reps):
+        t = plan.execute_timed(xx.data_ptr(), out.data_ptr(), stream)
+        acc = t if acc is None else {k: acc[k] + t[k] for k in acc}
+    kt = {k: v / reps for k, v in acc.items()}
+    tasks = info["tasks"]
+    fl = block_flops(info["Nb"], info["Mb"], args.mode == "exact", bool(info["up"]))
+    passes = {}
+    for k in kt:
+        tf = fl[k] * tasks / (kt[k] / 1e3) / 1e12 if kt[k] > 0 else 0.0
+        passes[k] = {"ms": round(kt[k], 4), "gflop": round(fl[k] * tasks / 1e9, 3), "tflops": round(tf, 2)}
+    dom = max(kt, key=kt.get)
+    achieved = fl[dom] * tasks / (kt[dom] / 1e3) / 1e12
+
+    # end to end (excluding file I/O): output copied back to the host
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    plan.execute(xx.data_ptr(), out.data_ptr(), stream)
+    host = out.cpu()
+    e2e_ms = (time.perf_counter() - t0) * 1e3
+
+    cpu = None
+    if not args.no_cpu and args.mode == "exact":
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        with tempfile.TemporaryDirectory(prefix="kbsrc-bench-") as td:
+            r = cpu_baseline(x, threads, td)
+        if r:
+            eng, design = r
+            cpu = {"value": round(nout_total / eng / 1e6, 2), "unit": "output Msamples/s", "cores": threads,
+                   "kind": "reference",
+                   "sample": f"full workload (2 x {frames} frames), reference engine resample time {eng:.3f} s "
+                             f"(median of 3, design {design:.3f} s excluded), {threads} threads"}
+
+    algo_gflop = sum(fl.values()) * tasks / 1e9
+    line = {
+        "metric": "exact-mode output Msamples/s (96k->44.1k)" if args.mode == "exact"
+        else "fast-mode output Msamples/s (96k->44.1k)",
+        "value": round(value, 2),
+        "unit": "output Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "stereo 96000->44100 Hz, 180 s float64 uniform noise, beta 40, len 0.2 s, LPC on, "
+                               f"{args.mode} mode", "frames_per_channel": frames, "channels": CH,
+                   "nb": info["nb"], "tasks": tasks, "parallelism": "single GPU"},
+        "roofline": {"bound": "fp64", "kernel": dom, "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None},
+        "cpu_baseline": cpu,
+        "passes": passes,
+        "algorithmic_gflop_per_step": round(algo_gflop, 2),
+        "step_tflops": round(algo_gflop / ms, 3),
+        "design_s": round(t_design, 3),
+        "stage_s": round(t_stage, 3),
+        "e2e_engine_plus_d2h_ms": round(e2e_ms, 3),
+    }
+    if cpu:
+        line["speedup_vs_cpu_engine"] = round(value / cpu["value"], 1)
+    print(json.dumps(line), flush=True)
+    del host
+    plan.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

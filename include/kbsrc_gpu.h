@@ -1,0 +1,91 @@
+/* KBSRC on MI355X: the C-ABI boundary of the GPU resampler (libkbsrc_gpu.so).
+ *
+ * Plain C types only; every call returns a status and never throws. The C++ API in include/kbsrc/kbsrc.hpp is
+ * a thin wrapper over these entry points. Each declaration names the reference interface it replaces.
+ */
+#ifndef KBSRC_GPU_H
+#define KBSRC_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes. EINVAL carries the reference's std::invalid_argument messages (src/resampler.cpp:94-155). */
+#define KBSRC_OK 0
+#define KBSRC_EINVAL 1   /* std::invalid_argument in the reference */
+#define KBSRC_ERUNTIME 2 /* std::runtime_error: device errors, unsupported plans */
+#define KBSRC_ENOMEM 3   /* std::bad_alloc: host or device memory exhausted */
+
+/* Precision values, as kbsrc::Precision (include/kbsrc/kbsrc.hpp:8-12). LongDouble is rejected. */
+#define KBSRC_FAST 0
+#define KBSRC_EXACT 1
+#define KBSRC_LONG_DOUBLE 2
+
+/* kbsrc::Options (include/kbsrc/kbsrc.hpp:14-21) */
+typedef struct kbsrc_options {
+    int precision;  /* KBSRC_EXACT (default) or KBSRC_FAST */
+    double length;  /* one-sided kernel length, seconds (default 0.2) */
+    double beta;    /* Kaiser beta (default 40) */
+    int threads;    /* host threads for the kernel design, 0 = automatic */
+    int block;      /* block size factor nb, 0 = automatic */
+    int lpc_edges;  /* Burg LPC edge extension, default 1 */
+} kbsrc_options;
+
+/* kbsrc::Stats (include/kbsrc/kbsrc.hpp:23-31) */
+typedef struct kbsrc_stats {
+    long long p, q, taps;
+    int block, threads, general_fold;
+    double design_seconds, process_seconds, passband, stopband;
+    const char* isa;  /* "gfx950" */
+} kbsrc_stats;
+
+/* Fills the reference's defaults (include/kbsrc/kbsrc.hpp:14-21). */
+void kbsrc_default_options(kbsrc_options* opt);
+
+/* kbsrc::version (include/kbsrc/kbsrc.hpp:45) */
+const char* kbsrc_version(void);
+
+/* Message of the last failed call on this thread. */
+const char* kbsrc_last_error(void);
+
+/* Output frames per channel, ceil(frames * P / Q) (include/kbsrc/kbsrc.hpp:33-34, src/resampler.cpp:100-105). */
+int kbsrc_output_frames(long long frames, int rate_in, int rate_out, long long* nout);
+
+/* kbsrc::resample (include/kbsrc/kbsrc.hpp:41-43; src/dispatch.cpp:70-81): interleaved host input (frames x
+ * channels) -> planar host output out[c * nout + i], nout = kbsrc_output_frames(). Exact runs the host-side dd
+ * work in round-to-nearest and restores the caller's rounding mode. stats may be NULL. */
+int kbsrc_resample(const double* interleaved, long long frames, int channels, int rate_in, int rate_out,
+                   const kbsrc_options* opt, double* out, kbsrc_stats* stats);
+
+/* ---- The engine seam: the reference's resample_impl split at run_engine (src/resampler.cpp:92-220,
+ * run_engine :34-88). A plan owns the kernel spectrum, twiddle tables and work buffers on the current device. */
+typedef struct kbsrc_plan kbsrc_plan;
+
+/* Plans the blocks, designs the kernel (taps on the host, spectrum on the GPU) and builds the FFT tables
+ * (src/resampler.cpp:92-167,187-206). */
+int kbsrc_plan_create(long long frames, int channels, int rate_in, int rate_out, const kbsrc_options* opt,
+                      kbsrc_plan** plan);
+
+/* Plan integers, in this order: P, Q, nout, nb, marg, Nb, Mb, hop, nblk, o, tot, base, pre, E, L, Lk, c2, gen,
+ * up, equal, tasks, batch (22 values). */
+#define KBSRC_PLAN_FIELDS 22
+int kbsrc_plan_query(const kbsrc_plan* plan, long long* values);
+
+/* Staging (src/resampler.cpp:168-186): host interleaved input -> device padded planar buffer d_padded of
+ * channels x tot doubles, with the Burg edge extensions. stream: a hipStream_t, or NULL for the default. */
+int kbsrc_plan_stage(kbsrc_plan* plan, const double* interleaved, double* d_padded, void* stream);
+
+/* The engine (src/resampler.cpp:34-88): every (channel, block) task, device to device. d_out: channels x nout
+ * doubles, planar. Asynchronous on stream. */
+int kbsrc_plan_execute(kbsrc_plan* plan, const double* d_padded, double* d_out, void* stream);
+
+/* Same, synchronous, with HIP-event times in milliseconds of the five kernel passes summed over the launch:
+ * ms[0] forward column, ms[1] forward row, ms[2] split/multiply/fold, ms[3] inverse column, ms[4] inverse row. */
+int kbsrc_plan_execute_timed(kbsrc_plan* plan, const double* d_padded, double* d_out, void* stream, double* ms);
+
+void kbsrc_plan_destroy(kbsrc_plan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,171 @@
+"""Python binding of the MI355X resampler (libkbsrc_gpu.so, C ABI in include/kbsrc_gpu.h).
+
+Mirrors the reference's library call ``kbsrc::resample`` (include/kbsrc/kbsrc.hpp:41-43): interleaved float64
+frames in, planar float64 channels out, with the same options (precision, length, beta, block, lpc_edges) and the
+same errors (``ValueError`` where the reference throws ``std::invalid_argument``, with its message). The engine
+seam (plan -> stage -> execute on device buffers) is exposed through :class:`Plan` for callers that keep data in
+HBM. There is no CPU fallback: importing fails loudly when the built library is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["resample", "output_frames", "Plan", "Stats", "version", "library_path", "FAST", "EXACT",
+           "KbsrcError", "PLAN_FIELDS"]
+
+FAST, EXACT, LONG_DOUBLE = 0, 1, 2
+OK, EINVAL, ERUNTIME, ENOMEM = 0, 1, 2, 3
+PLAN_FIELDS = ["P", "Q", "nout", "nb", "marg", "Nb", "Mb", "hop", "nblk", "o", "tot", "base", "pre", "E", "L", "Lk",
+               "c2", "gen", "up", "equal", "tasks", "batch"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def library_path() -> str:
+    return os.path.join(_HERE, "libkbsrc_gpu.so")
+
+
+class KbsrcError(RuntimeError):
+    """A device or internal error (std::runtime_error in the reference)."""
+
+
+class _Options(ctypes.Structure):
+    _fields_ = [("precision", ctypes.c_int), ("length", ctypes.c_double), ("beta", ctypes.c_double),
+                ("threads", ctypes.c_int), ("block", ctypes.c_int), ("lpc_edges", ctypes.c_int)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("p", ctypes.c_longlong), ("q", ctypes.c_longlong), ("taps", ctypes.c_longlong),
+                ("block", ctypes.c_int), ("threads", ctypes.c_int), ("general_fold", ctypes.c_int),
+                ("design_seconds", ctypes.c_double), ("process_seconds", ctypes.c_double),
+                ("passband", ctypes.c_double), ("stopband", ctypes.c_double), ("isa", ctypes.c_char_p)]
+
+
+def _load():
+    path = library_path()
+    if not os.path.exists(path):
+        raise ImportError(f"kbsrc_amd: {path} is not built (run __graft_entry__.build() or make -C kbsrc_amd)")
+    lib = ctypes.CDLL(path)
+    P, LL, I, D = ctypes.c_void_p, ctypes.c_longlong, ctypes.c_int, ctypes.c_double
+    sig = {
+        "kbsrc_default_options": (None, [ctypes.POINTER(_Options)]),
+        "kbsrc_version": (ctypes.c_char_p, []),
+        "kbsrc_last_error": (ctypes.c_char_p, []),
+        "kbsrc_output_frames": (I, [LL, I, I, ctypes.POINTER(LL)]),
+        "kbsrc_resample": (I, [P, LL, I, I, I, ctypes.POINTER(_Options), P, ctypes.POINTER(Stats)]),
+        "kbsrc_plan_create": (I, [LL, I, I, I, ctypes.POINTER(_Options), ctypes.POINTER(P)]),
+        "kbsrc_plan_query": (I, [P, ctypes.POINTER(LL)]),
+        "kbsrc_plan_stage": (I, [P, P, P, P]),
+        "kbsrc_plan_execute": (I, [P, P, P, P]),
+        "kbsrc_plan_execute_timed": (I, [P, P, P, P, ctypes.POINTER(D)]),
+        "kbsrc_plan_destroy": (None, [P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+_lib = _load()
+
+
+def _check(rc: int):
+    if rc == OK:
+        return
+    msg = (_lib.kbsrc_last_error() or b"").decode()
+    if rc == EINVAL:
+        raise ValueError(msg)
+    if rc == ENOMEM:
+        raise MemoryError(msg)
+    raise KbsrcError(msg)
+
+
+def _options(precision, length, beta, block, lpc_edges, threads=0) -> _Options:
+    o = _Options()
+    _lib.kbsrc_default_options(ctypes.byref(o))
+    o.precision = {"exact": EXACT, "fast": FAST, "ld": LONG_DOUBLE}.get(precision, precision) \
+        if isinstance(precision, str) else int(precision)
+    o.length, o.beta, o.block, o.lpc_edges, o.threads = float(length), float(beta), int(block), int(bool(lpc_edges)), \
+        int(threads)
+    return o
+
+
+def version() -> str:
+    return _lib.kbsrc_version().decode()
+
+
+def output_frames(frames: int, rate_in: int, rate_out: int) -> int:
+    n = ctypes.c_longlong()
+    _check(_lib.kbsrc_output_frames(frames, rate_in, rate_out, ctypes.byref(n)))
+    return n.value
+
+
+def resample(x, rate_in: int, rate_out: int, precision="exact", length=0.2, beta=40.0, block=0, lpc_edges=True,
+             threads=0, return_stats=False):
+    """Resample ``x`` (frames x channels, or 1-D mono) float64 from rate_in to rate_out on the GPU.
+
+    Returns a (channels, nout) float64 array (planar, as the reference's vector of channels)."""
+    a = np.asarray(x, dtype=np.float64)
+    if a.ndim == 1:
+        a = a[:, None]
+    a = np.ascontiguousarray(a)
+    frames, ch = a.shape
+    if ch < 1:
+        raise ValueError("bad input buffer")
+    nout = output_frames(frames, rate_in, rate_out) if rate_in >= 1 and rate_out >= 1 else 0
+    out = np.empty((ch, max(nout, 0)), dtype=np.float64)
+    st = Stats()
+    o = _options(precision, length, beta, block, lpc_edges, threads)
+    _check(_lib.kbsrc_resample(a.ctypes.data if frames else None, frames, ch, rate_in, rate_out, ctypes.byref(o),
+                               out.ctypes.data if out.size else None, ctypes.byref(st)))
+    return (out, st) if return_stats else out
+
+
+class Plan:
+    """The engine seam: plan/design once, then stage and execute on device buffers (integer device pointers,
+    e.g. ``torch.Tensor.data_ptr()``), on a HIP stream handle (0 = default stream)."""
+
+    def __init__(self, frames, channels, rate_in, rate_out, precision="exact", length=0.2, beta=40.0, block=0,
+                 lpc_edges=True, threads=0):
+        self._h = ctypes.c_void_p()
+        o = _options(precision, length, beta, block, lpc_edges, threads)
+        _check(_lib.kbsrc_plan_create(frames, channels, rate_in, rate_out, ctypes.byref(o), ctypes.byref(self._h)))
+        v = (ctypes.c_longlong * len(PLAN_FIELDS))()
+        _check(_lib.kbsrc_plan_query(self._h, v))
+        self.info = dict(zip(PLAN_FIELDS, list(v)))
+        self.channels = channels
+
+    def stage(self, interleaved, d_padded: int, stream: int = 0):
+        a = np.ascontiguousarray(np.asarray(interleaved, dtype=np.float64))
+        _check(_lib.kbsrc_plan_stage(self._h, a.ctypes.data, ctypes.c_void_p(d_padded), ctypes.c_void_p(stream)))
+
+    def execute(self, d_padded: int, d_out: int, stream: int = 0):
+        _check(_lib.kbsrc_plan_execute(self._h, ctypes.c_void_p(d_padded), ctypes.c_void_p(d_out),
+                                       ctypes.c_void_p(stream)))
+
+    def execute_timed(self, d_padded: int, d_out: int, stream: int = 0):
+        ms = (ctypes.c_double * 5)()
+        _check(_lib.kbsrc_plan_execute_timed(self._h, ctypes.c_void_p(d_padded), ctypes.c_void_p(d_out),
+                                             ctypes.c_void_p(stream), ms))
+        return dict(zip(["fwd_col", "fwd_row", "mid", "inv_col", "inv_row"], list(ms)))
+
+    def close(self):
+        if self._h:
+            _lib.kbsrc_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -1,0 +1,382 @@
+// gfx950 kernels of the overlap-save engine.
+//
+// The arithmetic of every kernel is the reference's, operation for operation (see dd.hpp), so the output is
+// bit-identical; what is MI355X-specific is how the work is laid out:
+//   - sub-FFT passes: a workgroup stages G sub-FFTs in LDS as split planes (re.hi, re.lo, im.hi, im.lo), runs
+//     all Stockham stages there, and reads/writes global memory once, with the lane mapping chosen so that
+//     consecutive lanes touch consecutive addresses;
+//   - global complex dd elements are 32 bytes {re.hi, im.hi, re.lo, im.lo}: a 16-byte "hi" complex followed by
+//     a 16-byte "lo" complex, two dwordx4 accesses per element; FP64 elements are the 16-byte hi half alone;
+//   - every (channel, block) task of a batch is one launch dimension, so a launch has thousands of workgroups.
+#include "kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace kbsrc::gpu {
+
+namespace {
+
+template <class A> using Val = typename A::T;
+template <class A> using Cx = cx<typename A::T>;
+
+// global complex element i of a buffer in the policy's element format
+template <class A> __device__ __forceinline__ Cx<A> gload(const double* base, long long i) {
+    if constexpr (A::kDD) {
+        const double2* p = reinterpret_cast<const double2*>(base) + 2 * i;
+        const double2 h = p[0], l = p[1];
+        return {dd{h.x, l.x}, dd{h.y, l.y}};
+    } else {
+        const double2 v = reinterpret_cast<const double2*>(base)[i];
+        return {v.x, v.y};
+    }
+}
+template <class A> __device__ __forceinline__ void gstore(double* base, long long i, Cx<A> v) {
+    if constexpr (A::kDD) {
+        double2* p = reinterpret_cast<double2*>(base) + 2 * i;
+        p[0] = make_double2(v.re.hi, v.im.hi);
+        p[1] = make_double2(v.re.lo, v.im.lo);
+    } else {
+        reinterpret_cast<double2*>(base)[i] = make_double2(v.re, v.im);
+    }
+}
+template <class A> __device__ __forceinline__ Val<A> gload_real(const double* base, long long i) {
+    if constexpr (A::kDD) {
+        const double2 v = reinterpret_cast<const double2*>(base)[i];
+        return dd{v.x, v.y};
+    } else {
+        return base[i];
+    }
+}
+
+// LDS split planes
+template <class A> struct Planes {
+    double* b;
+    int cap;
+    __device__ __forceinline__ Cx<A> get(int i) const {
+        if constexpr (A::kDD) return {dd{b[i], b[cap + i]}, dd{b[2 * cap + i], b[3 * cap + i]}};
+        else return {b[i], b[cap + i]};
+    }
+    __device__ __forceinline__ void put(int i, Cx<A> v) const {
+        if constexpr (A::kDD) {
+            b[i] = v.re.hi; b[cap + i] = v.re.lo; b[2 * cap + i] = v.im.hi; b[3 * cap + i] = v.im.lo;
+        } else {
+            b[i] = v.re; b[cap + i] = v.im;
+        }
+    }
+};
+template <class A> constexpr int kPlanes = A::kDD ? 4 : 2;
+
+template <class A> __device__ __forceinline__ Cx<A> renorm(Cx<A> v) {
+    if constexpr (A::kDD) return {quick2(v.re.hi, v.re.lo), quick2(v.im.hi, v.im.lo)};
+    else return v;
+}
+template <class A> __device__ __forceinline__ double to_f64(Val<A> v) {
+    if constexpr (A::kDD) return v.hi + v.lo;
+    else return v;
+}
+
+// Odd-radix butterfly by symmetric sums (src/dsp/fft.hpp:62-84)
+template <class A, int R>
+__device__ __forceinline__ void bfly_odd(const Planes<A>& cur, const Planes<A>& nxt, int Gs, int g, int sbase,
+                                         int sm, int dbase, int s, bool twid, const double* tw,
+                                         const double* oddc) {
+    constexpr int H = (R - 1) / 2;
+    constexpr int ri = R == 3 ? 0 : R == 5 ? 1 : 2;
+    const Cx<A> a0 = cur.get(sbase * Gs + g);
+    Cx<A> S[H], D[H];
+#pragma unroll
+    for (int t = 1; t <= H; ++t) {
+        const Cx<A> x = cur.get((sbase + t * sm) * Gs + g), y = cur.get((sbase + (R - t) * sm) * Gs + g);
+        S[t - 1] = c_add<A>(x, y);
+        D[t - 1] = c_sub<A>(x, y);
+    }
+    Cx<A> sum = a0;
+#pragma unroll
+    for (int t = 0; t < H; ++t) sum = c_add<A>(sum, S[t]);
+    nxt.put(dbase * Gs + g, sum);
+#pragma unroll
+    for (int u = 1; u <= H; ++u) {
+        Cx<A> ac = a0, bc = {A::lift(0.0), A::lift(0.0)};
+#pragma unroll
+        for (int t = 1; t <= H; ++t) {
+            const Cx<A> cs = gload<A>(oddc, ri * 7 + (t * u) % R);   // (cos, sin) of 2 pi m / R
+            ac = {A::add(ac.re, A::mul(S[t - 1].re, cs.re)), A::add(ac.im, A::mul(S[t - 1].im, cs.re))};
+            bc = {A::add(bc.re, A::mul(D[t - 1].re, cs.im)), A::add(bc.im, A::mul(D[t - 1].im, cs.im))};
+        }
+        Cx<A> xu = {A::add(ac.re, bc.im), A::sub(ac.im, bc.re)};
+        Cx<A> xr = {A::sub(ac.re, bc.im), A::add(ac.im, bc.re)};
+        if (twid) {
+            xu = c_mul<A>(xu, gload<A>(tw, u - 1));
+            xr = c_mul<A>(xr, gload<A>(tw, R - u - 1));
+        }
+        nxt.put((dbase + u * s) * Gs + g, xu);
+        nxt.put((dbase + (R - u) * s) * Gs + g, xr);
+    }
+}
+
+template <class A, int LD, int ST>
+__global__ __launch_bounds__(256) void k_subfft(PassArgs a) {
+    extern __shared__ double lds[];
+    const int tid = threadIdx.x, nthr = blockDim.x;
+    const int task = blockIdx.x / a.ngrp, grp = blockIdx.x % a.ngrp;
+    const int Gs = a.G, j0 = grp * Gs;
+    const int G = min(Gs, a.nsub - j0);
+    const int n = a.sd.n;
+    const int cap = n * Gs;
+    Planes<A> cur{lds, cap}, nxt{lds + kPlanes<A> * cap, cap};
+    const long long gtask = a.task0 + task;
+    const long long ch = gtask / a.nblk, blk = gtask - ch * a.nblk;
+
+    // ---- load G sub-FFTs into LDS
+    {
+        const bool gfast = a.ldg == 1 && a.lde != 1;
+        for (int w = tid; w < G * n; w += nthr) {
+            const int g = gfast ? w % G : w / n, e = gfast ? w / G : w % n;
+            const long long i = e * a.lde + (long long)(j0 + g) * a.ldg;
+            Cx<A> v;
+            if constexpr (LD == LD_REAL_F64) {
+                const double* s = a.src + ch * a.in_cstride + blk * a.in_bstride;
+                v = {A::lift(s[2 * i]), A::lift(s[2 * i + 1])};
+            } else if constexpr (LD == LD_REAL_DD) {
+                const double* s = a.src + task * a.src_tstride;
+                v = {gload_real<A>(s, 2 * i), gload_real<A>(s, 2 * i + 1)};
+            } else {
+                v = gload<A>(a.src + task * a.src_tstride, i);
+            }
+            cur.put(e * Gs + g, v);
+        }
+    }
+    __syncthreads();
+
+    // ---- Stockham stages, ping-pong between the two LDS halves (src/dsp/fft.hpp:115-150)
+    for (int si = 0; si < a.sd.nst; ++si) {
+        const StageDesc st = a.sd.st[si];
+        const int r = st.r, s = st.s, sm = n / r;
+        const int nbf = sm;   // butterflies per sub-FFT
+        for (int w = tid; w < G * nbf; w += nthr) {
+            const int g = w % G, b = w / G;
+            const int p = b / s, q = b - p * s;
+            const int dbase = s * r * p + q;
+            const bool twid = p != 0;
+            const double* tw = a.tw + (long long)(st.tw + p * (r - 1)) * (A::kDD ? 4 : 2);
+            if (r == 4) {
+                const Cx<A> a0 = cur.get(b * Gs + g), a1 = cur.get((b + sm) * Gs + g);
+                const Cx<A> a2 = cur.get((b + 2 * sm) * Gs + g), a3 = cur.get((b + 3 * sm) * Gs + g);
+                const Cx<A> t0 = c_add<A>(a0, a2), t1 = c_sub<A>(a0, a2), t2 = c_add<A>(a1, a3);
+                const Cx<A> t3 = c_mul_mi<A>(c_sub<A>(a1, a3));
+                nxt.put(dbase * Gs + g, c_add<A>(t0, t2));
+                Cx<A> o1 = c_add<A>(t1, t3), o2 = c_sub<A>(t0, t2), o3 = c_sub<A>(t1, t3);
+                if (twid) {
+                    o1 = c_mul<A>(o1, gload<A>(tw, 0));
+                    o2 = c_mul<A>(o2, gload<A>(tw, 1));
+                    o3 = c_mul<A>(o3, gload<A>(tw, 2));
+                }
+                nxt.put((dbase + s) * Gs + g, o1);
+                nxt.put((dbase + 2 * s) * Gs + g, o2);
+                nxt.put((dbase + 3 * s) * Gs + g, o3);
+            } else if (r == 2) {
+                const Cx<A> a0 = cur.get(b * Gs + g), a1 = cur.get((b + sm) * Gs + g);
+                nxt.put(dbase * Gs + g, c_add<A>(a0, a1));
+                Cx<A> d = c_sub<A>(a0, a1);
+                if (twid) d = c_mul<A>(d, gload<A>(tw, 0));
+                nxt.put((dbase + s) * Gs + g, d);
+            } else if (r == 3) {
+                bfly_odd<A, 3>(cur, nxt, Gs, g, b, sm, dbase, s, twid, tw, a.oddc);
+            } else if (r == 5) {
+                bfly_odd<A, 5>(cur, nxt, Gs, g, b, sm, dbase, s, twid, tw, a.oddc);
+            } else {
+                bfly_odd<A, 7>(cur, nxt, Gs, g, b, sm, dbase, s, twid, tw, a.oddc);
+            }
+        }
+        __syncthreads();
+        const Planes<A> t = cur;
+        cur = nxt;
+        nxt = t;
+    }
+
+    // ---- store: four-step twiddle, renormalisation, then either a complex buffer or the output samples
+    {
+        const bool gfast = a.stg == 1 && a.ste != 1;
+        for (int w = tid; w < G * n; w += nthr) {
+            const int g = gfast ? w % G : w / n, e = gfast ? w / G : w % n;
+            const long long jg = j0 + g;
+            const long long i = e * a.ste + jg * a.stg;
+            Cx<A> v = cur.get(e * Gs + g);
+            if (a.twmode && e != 0 && (a.twmode == 1 || jg != 0)) v = c_mul<A>(v, gload<A>(a.tw4, i));
+            if (a.renorm) v = renorm<A>(v);
+            if constexpr (ST == ST_CPLX) {
+                gstore<A>(a.dst + task * a.dst_tstride, i, v);
+            } else {
+                // r[2i] = re, r[2i+1] = -im (src/dsp/fft.hpp:460); keep the block's hop (src/resampler.cpp:80-84)
+                double* o = a.out + ch * a.nout;
+                const long long x0 = 2 * i - a.margP;
+                if (x0 >= 0 && x0 < a.hopP) {
+                    const long long d = blk * a.hopP + x0 - a.o;
+                    if (d >= 0 && d < a.nout) o[d] = to_f64<A>(v.re);
+                }
+                const long long x1 = x0 + 1;
+                if (x1 >= 0 && x1 < a.hopP) {
+                    const long long d = blk * a.hopP + x1 - a.o;
+                    if (d >= 0 && d < a.nout) o[d] = to_f64<A>(A::neg(v.im));
+                }
+            }
+        }
+    }
+}
+
+// ---- forward split X[i] from the complex FFT output z (src/dsp/fft.hpp:365-407)
+template <class A> __device__ __forceinline__ Cx<A> fwd_split(const double* z, const double* w, int h, int i) {
+    if (i == 0 || i == h) {
+        const Cx<A> z0 = gload<A>(z, 0);
+        return {i == 0 ? A::add(z0.re, z0.im) : A::sub(z0.re, z0.im), A::lift(0.0)};
+    }
+    const Val<A> half = A::lift(0.5);
+    if constexpr (!A::kDD) {   // FP64: every bin from its own pair (src/dsp/fft.hpp:366-373)
+        const Cx<A> Az = gload<A>(z, i), Bz = c_conj<A>(gload<A>(z, h - i));
+        const Cx<A> E = c_add<A>(Az, Bz), O = c_mul_mi<A>(c_sub<A>(Az, Bz));
+        const Cx<A> v = c_add<A>(E, c_mul<A>(O, gload<A>(w, i)));
+        return {A::mul(v.re, half), A::mul(v.im, half)};
+    } else {                   // dd: bins above h/2 from their conjugate partner's E and O (src/dsp/fft.hpp:397-407)
+        const bool lower = i <= h / 2;
+        const int k = lower ? i : h - i, j = h - k;
+        const Cx<A> Az = gload<A>(z, k), Bz = c_conj<A>(gload<A>(z, j));
+        const Cx<A> E = c_add<A>(Az, Bz), O = c_mul_mi<A>(c_sub<A>(Az, Bz));
+        const Cx<A> v = lower ? c_add<A>(E, c_mul<A>(O, gload<A>(w, k)))
+                              : c_add<A>(c_conj<A>(E), c_mul<A>(c_conj<A>(O), gload<A>(w, j)));
+        return {A::mul(v.re, half), A::mul(v.im, half)};
+    }
+}
+
+template <class A> __device__ __forceinline__ Val<A> load_H(const double* H, int k) { return gload_real<A>(H, k); }
+
+// spectrum bin idx in [0, hinv] after the kernel multiply and the bin map (src/resampler.cpp:64-78)
+template <class A> __device__ __forceinline__ Cx<A> mid_bin(const MidArgs& a, const double* z, int idx) {
+    if (!a.up) {
+        if (idx == a.hinv) {   // output Nyquist bin: twice the real part (src/resampler.cpp:71)
+            const Cx<A> x = fwd_split<A>(z, a.wf, a.h, idx);
+            return {A::mul(A::mul(A::lift(2.0), x.re), load_H<A>(a.H, idx)), A::lift(0.0)};
+        }
+        const Cx<A> x = fwd_split<A>(z, a.wf, a.h, idx);
+        const Val<A> hk = load_H<A>(a.H, idx);
+        Cx<A> y = {A::mul(x.re, hk), A::mul(x.im, hk)};
+        const int j = a.Mb - idx;   // bins above the output Nyquist fold onto Mb - j
+        if (j >= a.K && j <= a.jmax) {
+            const Cx<A> xj = fwd_split<A>(z, a.wf, a.h, j);
+            const Val<A> hj = load_H<A>(a.H, j);
+            const Cx<A> v = {A::mul(xj.re, hj), A::mul(xj.im, hj)};
+            y = c_add<A>(y, c_conj<A>(v));
+        }
+        return y;
+    }
+    const int kk = idx % a.Nb;   // upsampling: periodic extension of the input spectrum
+    const Cx<A> v = kk <= a.Nb / 2 ? fwd_split<A>(z, a.wf, a.h, kk) : c_conj<A>(fwd_split<A>(z, a.wf, a.h, a.Nb - kk));
+    const Val<A> hk = load_H<A>(a.H, idx);
+    Cx<A> y = {A::mul(v.re, hk), A::mul(v.im, hk)};
+    if (idx == a.hinv) y.im = A::lift(0.0);
+    return y;
+}
+
+// inverse pre-split (src/dsp/fft.hpp:412-458): v-form for bin k with partner j = hinv - k
+template <class A> __device__ __forceinline__ Cx<A> pre_v(Cx<A> Y, Cx<A> Yp, Cx<A> w) {
+    const Cx<A> B = c_conj<A>(Yp);
+    const Cx<A> E = c_add<A>(Y, B), D = c_mul<A>(c_sub<A>(Y, B), c_conj<A>(w));
+    return c_conj<A>(Cx<A>{A::sub(E.re, D.im), A::add(E.im, D.re)});
+}
+
+template <class A> __global__ __launch_bounds__(256) void k_mid(MidArgs a) {
+    const int task = blockIdx.y;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k > a.hinv / 2) return;
+    const double* z = a.z + task * a.z_tstride;
+    double* zi = a.zi + task * a.zi_tstride;
+    const int j = a.hinv - k;
+    const Cx<A> Yk = mid_bin<A>(a, z, k), Yj = mid_bin<A>(a, z, j);
+    gstore<A>(zi, k, pre_v<A>(Yk, Yj, gload<A>(a.wi, k)));
+    if (k && j != k) {
+        if constexpr (!A::kDD) {
+            gstore<A>(zi, j, pre_v<A>(Yj, Yk, gload<A>(a.wi, j)));
+        } else {   // dd: the partner from the same E and split (src/dsp/fft.hpp:452-456)
+            const Cx<A> B = c_conj<A>(Yj);
+            const Cx<A> E = c_add<A>(Yk, B), sp = c_sub<A>(Yk, B);
+            const Cx<A> Ej = c_conj<A>(E);
+            const Cx<A> Dj = c_mul<A>(Cx<A>{A::neg(sp.re), sp.im}, c_conj<A>(gload<A>(a.wi, j)));
+            gstore<A>(zi, j, c_conj<A>(Cx<A>{A::sub(Ej.re, Dj.im), A::add(Ej.im, Dj.re)}));
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void k_stage(StageArgs a) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= a.tot) return;
+    const long long pos = i - a.off;   // sample index in the signal
+    double v = 0.0;
+    if (pos >= 0 && pos < a.nin) v = a.in[pos * a.ch + c];
+    else if (pos >= a.nin && pos < a.nin + a.ext) v = a.tail[c * a.ext + (pos - a.nin)];
+    else if (pos < 0 && pos >= -a.ext) v = a.head[c * a.ext + (-1 - pos)];
+    a.xx[c * a.tot + i] = v;
+}
+
+__global__ __launch_bounds__(256) void k_spec_split(SpecArgs a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > a.h) return;
+    const Cx<DDA> x = fwd_split<DDA>(a.z, a.w, a.h, i);
+    const dd v = dddiv1(x.re, a.scale);
+    a.H[2 * i] = v.hi;
+    a.H[2 * i + 1] = v.lo;
+}
+
+template <class A, int LD, int ST> hipError_t launch_one(const PassArgs& a, int threads, hipStream_t s) {
+    const size_t lds = (size_t)2 * kPlanes<A> * a.sd.n * a.G * sizeof(double);
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_subfft<A, LD, ST>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((k_subfft<A, LD, ST>), dim3(a.ngrp * a.ntask), dim3(threads), lds, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+size_t pass_lds_bytes(int policy, int n, int G) {
+    return (size_t)2 * (policy == POL_F64 ? 2 : 4) * n * G * sizeof(double);
+}
+
+hipError_t launch_pass(int policy, int ld, int st, const PassArgs& a, int threads, hipStream_t s) {
+#define KB_CASE(P, POL, L, S) \
+    if (policy == P && ld == L && st == S) return launch_one<POL, L, S>(a, threads, s);
+    KB_CASE(POL_DDS, DDS, LD_REAL_F64, ST_CPLX)
+    KB_CASE(POL_DDS, DDS, LD_CPLX, ST_CPLX)
+    KB_CASE(POL_DDS, DDS, LD_CPLX, ST_OUT)
+    KB_CASE(POL_DDA, DDA, LD_REAL_F64, ST_CPLX)
+    KB_CASE(POL_DDA, DDA, LD_CPLX, ST_CPLX)
+    KB_CASE(POL_DDA, DDA, LD_CPLX, ST_OUT)
+    KB_CASE(POL_DDA, DDA, LD_REAL_DD, ST_CPLX)
+    KB_CASE(POL_F64, F64, LD_REAL_F64, ST_CPLX)
+    KB_CASE(POL_F64, F64, LD_CPLX, ST_CPLX)
+    KB_CASE(POL_F64, F64, LD_CPLX, ST_OUT)
+#undef KB_CASE
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_mid(int policy, const MidArgs& a, hipStream_t s) {
+    const dim3 grid((a.hinv / 2 + 1 + 255) / 256, a.ntask);
+    if (policy == POL_F64) hipLaunchKernelGGL(k_mid<F64>, grid, dim3(256), 0, s, a);
+    else hipLaunchKernelGGL(k_mid<DDA>, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_stage(const StageArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_stage, dim3((unsigned)((a.tot + 255) / 256), a.ch), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_spec_split(const SpecArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL(k_spec_split, dim3((a.h + 1 + 255) / 256), dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace kbsrc::gpu

@@ -1,0 +1,88 @@
+// gfx950 kernels of the overlap-save engine (declarations of their argument blocks and launchers).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "dd.hpp"
+
+namespace kbsrc::gpu {
+
+constexpr int kMaxStages = 24;
+
+// One Stockham stage of a sub-FFT: radix r, m = (remaining length) / r, s = product of earlier radices, and the
+// offset (in table elements) of its twiddles w[p*(r-1) + u-1] = cis(-2 pi s p u / n).
+struct StageDesc {
+    int r, m, s, tw;
+};
+struct SubDesc {
+    int n = 0, nst = 0;
+    StageDesc st[kMaxStages];
+};
+
+enum LoadKind { LD_REAL_F64 = 0, LD_REAL_DD = 1, LD_CPLX = 2 };
+enum StoreKind { ST_CPLX = 0, ST_OUT = 1 };
+enum Policy { POL_F64 = 0, POL_DDA = 1, POL_DDS = 2 };
+
+// A batch of equal sub-FFTs, G of them per workgroup, staged through LDS. Element e of sub-FFT j (within a
+// task) is read at complex index e*lde + j*ldg and written at e*ste + j*stg.
+struct PassArgs {
+    SubDesc sd;
+    const double* tw = nullptr;     // stage twiddles of this sub-FFT length
+    const double* oddc = nullptr;   // radix 3/5/7 constants: element (ri*7 + k) = cis(2 pi k / R)
+    int G = 1, nsub = 1, ngrp = 1, ntask = 1;
+    long long task0 = 0;            // global (channel, block) task number of local task 0
+    // load side
+    const double* src = nullptr;
+    long long src_tstride = 0;      // doubles between tasks (LD_CPLX, LD_REAL_DD)
+    long long lde = 1, ldg = 1;
+    long long nblk = 1, in_cstride = 0, in_bstride = 0;   // LD_REAL_F64: task t reads in + c*cstride + b*bstride
+    // store side
+    double* dst = nullptr;
+    long long dst_tstride = 0;
+    long long ste = 1, stg = 1;
+    const double* tw4 = nullptr;    // four-step twiddles, indexed like the store address
+    int twmode = 0;                 // 0: none, 1: every e != 0, 2: e != 0 and sub-FFT index != 0
+    int renorm = 0;                 // QuickTwoSum each stored dd component
+    // ST_OUT: unpack r[2i] = re, r[2i+1] = -im, keep [margP, margP + hopP), write out[c*nout + b*hopP + x - o]
+    double* out = nullptr;
+    long long nout = 0, o = 0, margP = 0, hopP = 0;
+};
+
+// Spectrum x H, bin map and inverse pre-split for one batch of tasks (src/fft.hpp:361-409,411-458;
+// src/resampler.cpp:53-78): z (forward complex FFT output, h per task) -> zi (inverse complex FFT input, hi).
+struct MidArgs {
+    const double* z = nullptr;
+    long long z_tstride = 0;
+    double* zi = nullptr;
+    long long zi_tstride = 0;
+    const double* H = nullptr;      // real kernel spectrum
+    const double* wf = nullptr;     // forward split twiddles w[0..h]
+    const double* wi = nullptr;     // inverse split twiddles w[0..hi]
+    int h = 0, hinv = 0, Nb = 0, Mb = 0, K = 0, jmax = 0, up = 0, ntask = 0;
+};
+
+// Zero-padded planar staging of interleaved input with the edge extensions (src/resampler.cpp:168-186).
+struct StageArgs {
+    const double* in = nullptr;     // interleaved frames x ch
+    const double* head = nullptr;   // ch x ext: sample -1-i
+    const double* tail = nullptr;   // ch x ext: sample nin+i
+    double* xx = nullptr;           // ch x tot
+    long long nin = 0, tot = 0, off = 0, ext = 0;
+    int ch = 0;
+};
+
+// Forward real-FFT split of the kernel spectrum and division by the scale (src/design.cpp:46-48).
+struct SpecArgs {
+    const double* z = nullptr;
+    const double* w = nullptr;
+    double* H = nullptr;            // (hi, lo) pairs, h+1 of them
+    int h = 0;
+    double scale = 1;
+};
+
+size_t pass_lds_bytes(int policy, int n, int G);
+hipError_t launch_pass(int policy, int ld, int st, const PassArgs& a, int threads, hipStream_t s);
+hipError_t launch_mid(int policy, const MidArgs& a, hipStream_t s);
+hipError_t launch_stage(const StageArgs& a, hipStream_t s);
+hipError_t launch_spec_split(const SpecArgs& a, hipStream_t s);
+
+}  // namespace kbsrc::gpu

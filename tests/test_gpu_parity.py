@@ -1,0 +1,148 @@
+"""GPU engine parity: bit-identical to the reference's pins and to the oracle (CPU restatement) on the same
+inputs, through the C ABI. Needs an MI355X."""
+import os
+import struct
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import helpers as H
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu():
+    import kbsrc_amd
+    return kbsrc_amd
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("name,rate,mode,want", H.CLI_PINS)
+def test_reference_cli_pins(name, rate, mode, want):
+    x = H.primary97() if name == "primary" else H.subnormal97()
+    say(f"pin {name} 48000->{rate} {mode}")
+    y = gpu().resample(x.reshape(-1, 2), 48000, rate, mode)
+    assert H.wav_sha_planar(y, rate) == want
+
+
+def test_frame1_and_subnormal_planar_pins():
+    y = gpu().resample(np.array([H.FRAME1]), 8000, 16000, "exact", length=0.01)
+    assert [struct.pack(">d", v).hex() for v in y.reshape(-1)] == H.FRAME1_OUT
+    y = gpu().resample(H.subnormal97().reshape(-1, 2), 48000, 96000, "exact", length=0.1)
+    assert H.sha(y.astype("<f8").tobytes()) == H.SUBNORMAL_96K_PLANAR
+
+
+@pytest.mark.parametrize("mode,lpc", [("exact", True), ("fast", True), ("exact", False)])
+def test_c1_pins(mode, lpc):
+    say(f"c1 48000->44100 {mode} lpc={lpc}")
+    y, st = gpu().resample(H.c1_input(), 48000, 44100, mode, lpc_edges=lpc, return_stats=True)
+    assert st.block == 250 and st.p == 147 and st.q == 160 and st.isa == b"gfx950"
+    assert H.wav_sha_planar(y, 44100) == H.C1_PINS[(mode, lpc)]
+
+
+CASES = [
+    # rin, rout, ch, frames, mode, length, block, lpc
+    (48000, 96000, 2, 3000, "exact", 0.2, 0, True),
+    (48000, 44100, 1, 20000, "fast", 0.2, 0, True),
+    (96000, 44100, 2, 30000, "exact", 0.05, 0, True),
+    (96000, 44100, 2, 30000, "fast", 0.05, 0, True),
+    (44100, 48000, 2, 9000, "exact", 0.1, 0, False),
+    (44100, 96000, 1, 5000, "fast", 0.2, 0, True),
+    (8000, 44100, 1, 4000, "exact", 0.1, 0, True),
+    (44100, 8000, 3, 12000, "exact", 0.1, 0, True),
+    (192000, 44100, 1, 40000, "exact", 0.02, 0, True),
+    (22050, 16000, 2, 7000, "fast", 0.1, 0, True),
+    (48000, 96000, 2, 64, "exact", 0.1, 9605, True),
+    (48000, 96000, 2, 64, "fast", 0.1, 20000, True),
+    (48000, 44100, 1, 10, "exact", 0.2, 0, True),
+    (48000, 44100, 1, 1, "exact", 0.2, 0, True),
+    (48000, 44100, 1, 0, "exact", 0.2, 0, True),
+    (48000, 44100, 6, 2000, "fast", 0.01, 0, True),
+    (16000, 48000, 1, 3000, "exact", 0.01, 0, True),
+    (48000, 16000, 2, 1000, "exact", 0.002, 0, True),
+    (44100, 88200, 1, 20000, "exact", 0.05, 2000, True),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c[0]}-{c[1]}-{c[2]}ch-{c[3]}-{c[4]}-L{c[5]}-b{c[6]}")
+def test_gpu_equals_oracle(case):
+    rin, rout, ch, frames, mode, length, block, lpc = case
+    say(f"case {case}")
+    x = np.random.default_rng(frames + rin + rout).uniform(-0.5, 0.5, frames * ch)
+    want = H.oracle_resample(x, ch, rin, rout, mode, length=length, block=block, lpc=lpc)
+    got = gpu().resample(x.reshape(frames, ch), rin, rout, mode, length=length, block=block, lpc_edges=lpc)
+    assert got.shape == want.shape
+    if got.tobytes() != want.tobytes():
+        bad = np.flatnonzero(got.reshape(-1) != want.reshape(-1))
+        pytest.fail(f"{len(bad)} of {got.size} samples differ, first at {bad[:5]}: "
+                    f"{got.reshape(-1)[bad[:3]]} vs {want.reshape(-1)[bad[:3]]}")
+
+
+def test_equal_rates_copy_bit_exact():
+    x = np.random.default_rng(5).standard_normal((4410, 2)) * 0.1
+    y = gpu().resample(x, 44100, 44100)
+    assert np.array_equal(y, x.T)
+
+
+def test_gpu_cli_pins(tmp_path):
+    """The kbsrc command line over the GPU engine writes the pinned WAV bytes (src/io/wav.cpp layout)."""
+    src = tmp_path / "primary.wav"
+    src.write_bytes(H.wav_f64(H.primary97(), 2, 48000))
+    for name, rate, mode, want in H.CLI_PINS[:3]:
+        out = tmp_path / "out.wav"
+        r = subprocess.run([H.GPU_CLI, "--rate", str(rate), "--mode", mode, "-v", str(src), str(out)],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        assert "engine=gfx950" in r.stderr
+        assert H.sha(out.read_bytes()) == want
+
+
+def _c2_plan_run(mode):
+    """Full benchmark workload: stereo 96 kHz -> 44.1 kHz, 180 s, through plan/stage/execute on device buffers."""
+    import torch
+    x = H.c2_input()
+    assert H.sha(H.wav_f64(x, 2, 96000)) == H.C2_IN
+    with gpu().Plan(len(x), 2, 96000, 44100, mode) as plan:
+        info = plan.info
+        xx = torch.empty(2 * info["tot"], dtype=torch.float64, device="cuda")
+        out = torch.empty(2 * info["nout"], dtype=torch.float64, device="cuda")
+        plan.stage(x, xx.data_ptr())
+        plan.execute(xx.data_ptr(), out.data_ptr())
+        torch.cuda.synchronize()
+        return out.cpu().numpy().reshape(2, info["nout"]), info
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_c2_full_size_pins(mode):
+    say(f"c2 full size {mode}")
+    y, info = _c2_plan_run(mode)
+    assert y.shape == (2, 7_938_000)
+    assert H.wav_sha_planar(y, 44100) == H.C2_PINS[f"{mode}_f64"]
+
+
+def test_c2_quantised_outputs_bit_exact(tmp_path):
+    """Undithered s24/s16/s32 through the CLI (host quantisation of the GPU output) match the reference's bytes."""
+    say("c2 CLI s24/s16/s32")
+    src = tmp_path / "c2.wav"
+    with open(src, "wb") as f:
+        f.write(H.wav_f64(H.c2_input(), 2, 96000))
+    for fmt in ("s24", "s16", "s32"):
+        out = tmp_path / f"c2_{fmt}.wav"
+        r = subprocess.run([H.GPU_CLI, "--rate", "44100", "--format", fmt, str(src), str(out)], capture_output=True,
+                           text=True, timeout=110)
+        assert r.returncode == 0, r.stderr
+        assert H.sha(out.read_bytes()) == H.C2_PINS[f"exact_{fmt}"], fmt
+        os.remove(out)
+
+
+def test_c3_upsampling_full_size_pin():
+    say("c3 44.1k->48k full size")
+    x = H.c3_input()
+    y = gpu().resample(x, 44100, 48000, "exact")
+    assert H.wav_sha_planar(y, 48000) == H.C3_EXACT
