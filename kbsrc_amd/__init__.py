@@ -13,6 +13,11 @@ import os
 
 import numpy as np
 
+try:   # one HIP runtime per process: when torch is present its runtime must be the one this library binds to
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 __all__ = ["resample", "output_frames", "Plan", "Stats", "version", "library_path", "FAST", "EXACT",
            "KbsrcError", "PLAN_FIELDS"]
 

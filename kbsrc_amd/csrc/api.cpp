@@ -125,7 +125,8 @@ void stage(kbsrc_plan* pl, const double* in, double* d_padded, hipStream_t s) {
         }
     }
     DevBuf din, dh, dt;
-    din.upload(in, sizeof(double) * (size_t)std::max(1LL, p.nin * p.ch), s);
+    if (p.nin > 0) din.upload(in, sizeof(double) * (size_t)(p.nin * p.ch), s);
+    else din.alloc(sizeof(double));
     dh.upload(head.data(), sizeof(double) * head.size(), s);
     dt.upload(tail.data(), sizeof(double) * tail.size(), s);
     StageArgs a;

@@ -139,6 +139,14 @@ int pick_group(int policy, int n, int nsub) {
     return std::max(1, std::min({G, nsub, 16}));
 }
 
+// launch one sub-FFT pass; errors name the pass and its shape
+void run_pass(int policy, int ld, int st, const PassArgs& a, hipStream_t s, const char* what) {
+    const hipError_t e = launch_pass(policy, ld, st, a, kThreads, s);
+    if (e != hipSuccess)
+        throw HipError(std::string(what) + " (n=" + std::to_string(a.sd.n) + ", G=" + std::to_string(a.G) +
+                       ", LDS " + std::to_string(pass_lds_bytes(policy, a.sd.n, a.G)) + " B): " + hipGetErrorString(e));
+}
+
 void fill_sub(PassArgs& a, const SubFFT& sf, const DeviceConsts& k, bool as_dd, int policy, int nsub) {
     a.sd = sf.desc;
     a.tw = sf.tw.d();
@@ -165,18 +173,18 @@ std::vector<dd> kernel_spectrum_gpu(const std::vector<dd>& h, int n, double scal
         fill_sub(a, c.s1, k, true, POL_DDA, c.N2);
         a.src = din.d(); a.lde = c.N2; a.ldg = 1;
         a.dst = w1.d(); a.ste = c.N2; a.stg = 1; a.tw4 = c.tw4.d(); a.twmode = 2;
-        hip_check(launch_pass(POL_DDA, LD_REAL_DD, ST_CPLX, a, kThreads, s), "spectrum column pass");
+        run_pass(POL_DDA, LD_REAL_DD, ST_CPLX, a, s, "spectrum column pass");
         PassArgs b;
         fill_sub(b, c.s2, k, true, POL_DDA, c.N1);
         b.src = w1.d(); b.lde = 1; b.ldg = c.N2;
         b.dst = w2.d(); b.ste = c.N1; b.stg = 1;
-        hip_check(launch_pass(POL_DDA, LD_CPLX, ST_CPLX, b, kThreads, s), "spectrum row pass");
+        run_pass(POL_DDA, LD_CPLX, ST_CPLX, b, s, "spectrum row pass");
     } else {
         PassArgs a;
         fill_sub(a, c.sd, k, true, POL_DDA, 1);
         a.src = din.d(); a.lde = 1; a.ldg = 0;
         a.dst = w2.d(); a.ste = 1; a.stg = 0;
-        hip_check(launch_pass(POL_DDA, LD_REAL_DD, ST_CPLX, a, kThreads, s), "spectrum pass");
+        run_pass(POL_DDA, LD_REAL_DD, ST_CPLX, a, s, "spectrum pass");
     }
     SpecArgs sa;
     sa.z = w2.d(); sa.w = R.w.d(); sa.H = dH.d(); sa.h = R.h; sa.scale = scale;
@@ -256,20 +264,20 @@ void Engine::run_cfft(const CFFTPlan& f, int ld, const double* src, int st, doub
         a.src = in_src; a.lde = f.N2; a.ldg = 1;
         a.dst = mid; a.dst_tstride = mid_ts; a.ste = f.N2; a.stg = 1;
         a.tw4 = f.tw4.d(); a.twmode = 1; a.renorm = dd_;
-        timed(t_col, [&] { hip_check(launch_pass(pol_four, ld, ST_CPLX, a, kThreads, s), "column pass"); });
+        timed(t_col, [&] { run_pass(pol_four, ld, ST_CPLX, a, s, "column pass"); });
         PassArgs b;
         fill_sub(b, f.s2, k_, dd_, pol_four, f.N1);
         common(b);
         b.src = mid; b.src_tstride = mid_ts; b.lde = 1; b.ldg = f.N2;
         b.dst = dst; b.dst_tstride = dst_tstride; b.ste = f.N1; b.stg = 1; b.renorm = dd_;
-        timed(t_row, [&] { hip_check(launch_pass(pol_four, LD_CPLX, st, b, kThreads, s), "row pass"); });
+        timed(t_row, [&] { run_pass(pol_four, LD_CPLX, st, b, s, "row pass"); });
     } else {
         PassArgs a;
         fill_sub(a, f.sd, k_, dd_, pol_one, 1);
         common(a);
         a.src = in_src; a.lde = 1; a.ldg = 0;
         a.dst = dst; a.dst_tstride = dst_tstride; a.ste = 1; a.stg = 0;
-        timed(t_row, [&] { hip_check(launch_pass(pol_one, ld, st, a, kThreads, s), "single pass"); });
+        timed(t_row, [&] { run_pass(pol_one, ld, st, a, s, "single pass"); });
     }
 }
 

@@ -66,7 +66,7 @@ CASES = [
     (48000, 44100, 6, 2000, "fast", 0.01, 0, True),
     (16000, 48000, 1, 3000, "exact", 0.01, 0, True),
     (48000, 16000, 2, 1000, "exact", 0.002, 0, True),
-    (44100, 88200, 1, 20000, "exact", 0.05, 2000, True),
+    (44100, 88200, 1, 20000, "exact", 0.05, 6000, True),
 ]
 
 
