@@ -121,6 +121,8 @@ def main():
     ap.add_argument("--mode", default="exact", choices=["exact", "fast"])
     ap.add_argument("--cpu-threads", type=int, default=0, help="reference CPU engine threads (0: min(16, cores))")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile-copy", action="store_true",
+                    help="also run a plain 256 MiB device copy (calibrates HBM byte counters under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}): the path shards by block ranges with no "
@@ -140,7 +142,7 @@ def main():
     t_design = time.perf_counter() - t0
     info = plan.info
     dev = torch.device("cuda:0")
-    xx = torch.empty(CH * info["tot"], dtype=torch.float64, device=dev)
+    xx = torch.empty(CH * info["xstride"], dtype=torch.float64, device=dev)
     out = torch.empty(CH * info["nout"], dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream(dev).cuda_stream
     t0 = time.perf_counter()
@@ -150,6 +152,13 @@ def main():
     print(f"plan: nb={info['nb']} Nb={info['Nb']} Mb={info['Mb']} tasks={info['tasks']} "
           f"design {t_design:.3f}s stage {t_stage:.3f}s", file=sys.stderr, flush=True)
 
+    if args.profile_copy:
+        a = torch.ones(1 << 25, dtype=torch.float64, device=dev)
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        del a, b
     for _ in range(args.warmup):
         plan.execute(xx.data_ptr(), out.data_ptr(), stream)
     torch.cuda.synchronize()

@@ -67,16 +67,16 @@ int kbsrc_plan_create(long long frames, int channels, int rate_in, int rate_out,
                       kbsrc_plan** plan);
 
 /* Plan integers, in this order: P, Q, nout, nb, marg, Nb, Mb, hop, nblk, o, tot, base, pre, E, L, Lk, c2, gen,
- * up, equal, tasks, batch (22 values). */
-#define KBSRC_PLAN_FIELDS 22
+ * up, equal, tasks, batch, xstride (23 values). xstride is the channel stride of the padded device input. */
+#define KBSRC_PLAN_FIELDS 23
 int kbsrc_plan_query(const kbsrc_plan* plan, long long* values);
 
 /* Staging (src/resampler.cpp:168-186): host interleaved input -> device padded planar buffer d_padded of
- * channels x tot doubles, with the Burg edge extensions. stream: a hipStream_t, or NULL for the default. */
+ * channels x xstride doubles (channel c at d_padded + c * xstride), with the Burg edge extensions. stream: a hipStream_t, or NULL for the default. */
 int kbsrc_plan_stage(kbsrc_plan* plan, const double* interleaved, double* d_padded, void* stream);
 
-/* The engine (src/resampler.cpp:34-88): every (channel, block) task, device to device. d_out: channels x nout
- * doubles, planar. Asynchronous on stream. */
+/* The engine (src/resampler.cpp:34-88): every (channel, block) task, device to device. d_padded as staged;
+ * d_out: channels x nout doubles, planar. Asynchronous on stream. */
 int kbsrc_plan_execute(kbsrc_plan* plan, const double* d_padded, double* d_out, void* stream);
 
 /* Same, synchronous, with HIP-event times in milliseconds of the five kernel passes summed over the launch:

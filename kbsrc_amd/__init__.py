@@ -24,7 +24,7 @@ __all__ = ["resample", "output_frames", "Plan", "Stats", "version", "library_pat
 FAST, EXACT, LONG_DOUBLE = 0, 1, 2
 OK, EINVAL, ERUNTIME, ENOMEM = 0, 1, 2, 3
 PLAN_FIELDS = ["P", "Q", "nout", "nb", "marg", "Nb", "Mb", "hop", "nblk", "o", "tot", "base", "pre", "E", "L", "Lk",
-               "c2", "gen", "up", "equal", "tasks", "batch"]
+               "c2", "gen", "up", "equal", "tasks", "batch", "xstride"]
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 

@@ -131,7 +131,7 @@ void stage(kbsrc_plan* pl, const double* in, double* d_padded, hipStream_t s) {
     dt.upload(tail.data(), sizeof(double) * tail.size(), s);
     StageArgs a;
     a.in = din.d(); a.head = dh.d(); a.tail = dt.d(); a.xx = d_padded;
-    a.nin = p.nin; a.tot = p.tot; a.off = p.base + p.pre; a.ext = ext; a.ch = p.ch;
+    a.nin = p.nin; a.tot = p.tot; a.stride = p.xstride; a.off = p.base + p.pre; a.ext = ext; a.ch = p.ch;
     hip_check(launch_stage(a, s), "staging kernel");
     hip_check(hipStreamSynchronize(s), "staging sync");
 }
@@ -182,7 +182,7 @@ void resample_host(const double* in, long long frames, int channels, int rate_in
     auto pl = make(frames, channels, rate_in, rate_out, o, s);
     const Plan& p = pl->p;
     const double t1 = now();
-    DevBuf xx((size_t)p.ch * p.tot * sizeof(double)), dout((size_t)std::max(1LL, p.ch * p.nout) * sizeof(double));
+    DevBuf xx((size_t)p.ch * p.xstride * sizeof(double)), dout((size_t)std::max(1LL, p.ch * p.nout) * sizeof(double));
     stage(pl.get(), in, xx.d(), s);
     pl->design_seconds += now() - t1;
     const double t2 = now();
@@ -235,7 +235,8 @@ int kbsrc_plan_query(const kbsrc_plan* pl, long long* v) {
         const Plan& p = pl->p;
         const long long a[KBSRC_PLAN_FIELDS] = {p.P, p.Q, p.nout, p.nb, p.marg, p.Nb, p.Mb, p.hop, p.nblk, p.o, p.tot,
                                                 p.base, p.pre, p.E, p.L, p.Lk, p.c2, p.gen, p.up, p.equal,
-                                                pl->eng ? pl->eng->tasks() : 0, pl->eng ? pl->eng->batch() : 0};
+                                                pl->eng ? pl->eng->tasks() : 0, pl->eng ? pl->eng->batch() : 0,
+                                                p.xstride};
         std::memcpy(v, a, sizeof a);
     });
 }

@@ -2,6 +2,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstring>
 
 #pragma clang fp contract(off)
@@ -140,8 +141,17 @@ int pick_group(int policy, int n, int nsub) {
 }
 
 // launch one sub-FFT pass; errors name the pass and its shape
-void run_pass(int policy, int ld, int st, const PassArgs& a, hipStream_t s, const char* what) {
-    const hipError_t e = launch_pass(policy, ld, st, a, kThreads, s);
+bool use_fixed() {
+    static const bool on = [] {
+        const char* v = std::getenv("KBSRC_GENERIC_PASSES");
+        return !(v && v[0] == '1');
+    }();
+    return on;
+}
+
+void run_pass(int policy, int ld, int st, PassArgs a, hipStream_t s, const char* what) {
+    hipError_t e = hipSuccess;
+    if (!(use_fixed() && launch_fixed(policy, ld, st, a, s, &e))) e = launch_pass(policy, ld, st, a, kThreads, s);
     if (e != hipSuccess)
         throw HipError(std::string(what) + " (n=" + std::to_string(a.sd.n) + ", G=" + std::to_string(a.G) +
                        ", LDS " + std::to_string(pass_lds_bytes(policy, a.sd.n, a.G)) + " B): " + hipGetErrorString(e));
@@ -215,6 +225,10 @@ Engine::Engine(const Plan& p, const std::vector<dd>& H, hipStream_t s) : p_(p), 
     ts_ = (long long)(per_task / sizeof(double));
     const size_t budget = (size_t)1 << 30;
     tb_ = (int)std::max<long long>(1, std::min<long long>(ntask_, (long long)(budget / per_task)));
+    if (const char* v = std::getenv("KBSRC_BATCH_TASKS")) {   // tuning knob: tasks per launch
+        const long long want = std::atoll(v);
+        if (want > 0) tb_ = (int)std::min<long long>(tb_, want);
+    }
     w1_.alloc(per_task * tb_);
     w2_.alloc(per_task * tb_);
 }
@@ -228,7 +242,7 @@ void Engine::run_cfft(const CFFTPlan& f, int ld, const double* src, int st, doub
         a.ntask = nt;
         a.task0 = task0;
         a.nblk = p_.nblk;
-        a.in_cstride = p_.tot;
+        a.in_cstride = p_.xstride;
         a.in_bstride = (long long)p_.hop * p_.Q;
         a.src_tstride = src_tstride;
         a.out = out;

@@ -102,6 +102,7 @@ Plan make_plan(long long frames, int channels, int rate_in, int rate_out, bool f
     p.nblk = (p.span + (long long)p.hop * Q - 1) / ((long long)p.hop * Q);
     p.base = (long long)marg * Q;
     p.tot = p.base + p.nblk * p.hop * Q + p.Nb;
+    p.xstride = (p.tot + 31) / 32 * 32;
     p.o = p.pre / Q * P;
     p.gen = (double)(std::max(p.fi, p.fo) - p.mn) / 2 < 3 * p.fml;
     return p;

@@ -18,6 +18,7 @@ struct Plan {
     double fml = 0;
     int nb = 0, marg = 0, Nb = 0, Mb = 0, hop = 0;
     long long nblk = 0, base = 0, tot = 0, E = 0, pre = 0, span = 0, o = 0;
+    long long xstride = 0;   // doubles between channels of the padded device input: tot rounded up to 256 bytes
 };
 
 // Throws std::invalid_argument with the reference's messages.

@@ -314,7 +314,7 @@ __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
     if (pos >= 0 && pos < a.nin) v = a.in[pos * a.ch + c];
     else if (pos >= a.nin && pos < a.nin + a.ext) v = a.tail[c * a.ext + (pos - a.nin)];
     else if (pos < 0 && pos >= -a.ext) v = a.head[c * a.ext + (-1 - pos)];
-    a.xx[c * a.tot + i] = v;
+    a.xx[c * a.stride + i] = v;
 }
 
 __global__ __launch_bounds__(256) void k_spec_split(SpecArgs a) {

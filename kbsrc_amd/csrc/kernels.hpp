@@ -65,8 +65,8 @@ struct StageArgs {
     const double* in = nullptr;     // interleaved frames x ch
     const double* head = nullptr;   // ch x ext: sample -1-i
     const double* tail = nullptr;   // ch x ext: sample nin+i
-    double* xx = nullptr;           // ch x tot
-    long long nin = 0, tot = 0, off = 0, ext = 0;
+    double* xx = nullptr;           // ch x stride, the first tot of each channel written
+    long long nin = 0, tot = 0, stride = 0, off = 0, ext = 0;
     int ch = 0;
 };
 
@@ -81,6 +81,9 @@ struct SpecArgs {
 
 size_t pass_lds_bytes(int policy, int n, int G);
 hipError_t launch_pass(int policy, int ld, int st, const PassArgs& a, int threads, hipStream_t s);
+// Compile-time specialised pass for the benchmark transform lengths; false when (policy, n, ld, st) has none.
+// Sets a.G and a.ngrp to the specialisation's group size.
+bool launch_fixed(int policy, int ld, int st, PassArgs& a, hipStream_t s, hipError_t* err);
 hipError_t launch_mid(int policy, const MidArgs& a, hipStream_t s);
 hipError_t launch_stage(const StageArgs& a, hipStream_t s);
 hipError_t launch_spec_split(const SpecArgs& a, hipStream_t s);

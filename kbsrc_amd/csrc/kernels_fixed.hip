@@ -1,0 +1,319 @@
+// Sub-FFT passes specialised at compile time for the transform lengths of the benchmark workloads.
+//
+// Same arithmetic as the generic pass in kernels.hip (and so as the reference: src/dsp/fft.hpp:86-151), but the
+// radix sequence, the number of sub-FFTs per workgroup G and the thread count T are template constants. That
+// turns every index division into shifts or multiplies, unrolls the butterflies, and lets the stages run in a
+// single LDS buffer: each thread computes all of its butterflies of a stage into registers, the workgroup
+// synchronises, and the results are written back in place. Half the LDS of the ping-pong form, so twice the
+// workgroups per CU.
+#include <utility>
+
+#include "kernels.hpp"
+
+#pragma clang fp contract(off)
+
+namespace kbsrc::gpu {
+
+namespace {
+
+template <class A> using Val = typename A::T;
+template <class A> using Cx = cx<typename A::T>;
+template <class A> constexpr int kPlanes = A::kDD ? 4 : 2;
+template <class A> constexpr int kElt = A::kDD ? 4 : 2;   // doubles per global complex element
+
+template <class A> __device__ __forceinline__ Cx<A> gld(const double* base, long long i) {
+    if constexpr (A::kDD) {
+        const double2* p = reinterpret_cast<const double2*>(base) + 2 * i;
+        const double2 h = p[0], l = p[1];
+        return {dd{h.x, l.x}, dd{h.y, l.y}};
+    } else {
+        const double2 v = reinterpret_cast<const double2*>(base)[i];
+        return {v.x, v.y};
+    }
+}
+template <class A> __device__ __forceinline__ void gst(double* base, long long i, Cx<A> v) {
+    if constexpr (A::kDD) {
+        double2* p = reinterpret_cast<double2*>(base) + 2 * i;
+        p[0] = make_double2(v.re.hi, v.im.hi);
+        p[1] = make_double2(v.re.lo, v.im.lo);
+    } else {
+        reinterpret_cast<double2*>(base)[i] = make_double2(v.re, v.im);
+    }
+}
+
+template <class A, int CAP> struct Lds {
+    double* b;
+    __device__ __forceinline__ Cx<A> get(int i) const {
+        if constexpr (A::kDD) return {dd{b[i], b[CAP + i]}, dd{b[2 * CAP + i], b[3 * CAP + i]}};
+        else return {b[i], b[CAP + i]};
+    }
+    __device__ __forceinline__ void put(int i, Cx<A> v) const {
+        if constexpr (A::kDD) {
+            b[i] = v.re.hi; b[CAP + i] = v.re.lo; b[2 * CAP + i] = v.im.hi; b[3 * CAP + i] = v.im.lo;
+        } else {
+            b[i] = v.re; b[CAP + i] = v.im;
+        }
+    }
+};
+
+// Compile-time stage list in the reference's radix order.
+template <int... R> struct Radices {
+    static constexpr int nst = sizeof...(R);
+    static constexpr int r[nst] = {R...};
+    static constexpr int N = (R * ...);
+    static constexpr int stride(int i) {   // product of the earlier radices
+        int p = 1;
+        for (int k = 0; k < i; ++k) p *= r[k];
+        return p;
+    }
+    static constexpr int twoff(int i) {    // offset of stage i's twiddles: sum of m_k (r_k - 1) before it
+        int o = 0;
+        for (int k = 0; k < i; ++k) o += (N / (stride(k) * r[k])) * (r[k] - 1);
+        return o;
+    }
+    // complex values one thread holds in the widest stage, for G sub-FFTs on T threads
+    static constexpr int hold(int G, int T) {
+        int h = 0;
+        for (int k = 0; k < nst; ++k) {
+            const int W = G * (N / r[k]), it = (W + T - 1) / T;
+            h = it * r[k] > h ? it * r[k] : h;
+        }
+        return h;
+    }
+};
+
+// Radix-R butterfly of the reference's Stockham step on x[0..R): out[u] goes to destination q + u*s.
+template <class A, int R>
+__device__ __forceinline__ void butterfly(const Cx<A> (&x)[R], Cx<A> (&out)[R], bool twid, const double* tw,
+                                          const double* oddc) {
+    if constexpr (R == 4) {
+        const Cx<A> t0 = c_add<A>(x[0], x[2]), t1 = c_sub<A>(x[0], x[2]), t2 = c_add<A>(x[1], x[3]);
+        const Cx<A> t3 = c_mul_mi<A>(c_sub<A>(x[1], x[3]));
+        out[0] = c_add<A>(t0, t2);
+        out[1] = c_add<A>(t1, t3);
+        out[2] = c_sub<A>(t0, t2);
+        out[3] = c_sub<A>(t1, t3);
+        if (twid) {
+            out[1] = c_mul<A>(out[1], gld<A>(tw, 0));
+            out[2] = c_mul<A>(out[2], gld<A>(tw, 1));
+            out[3] = c_mul<A>(out[3], gld<A>(tw, 2));
+        }
+    } else if constexpr (R == 2) {
+        out[0] = c_add<A>(x[0], x[1]);
+        out[1] = c_sub<A>(x[0], x[1]);
+        if (twid) out[1] = c_mul<A>(out[1], gld<A>(tw, 0));
+    } else {   // odd radix by symmetric sums (src/dsp/fft.hpp:62-84)
+        constexpr int H = (R - 1) / 2;
+        constexpr int ri = R == 3 ? 0 : R == 5 ? 1 : 2;
+        Cx<A> S[H], D[H];
+#pragma unroll
+        for (int t = 1; t <= H; ++t) {
+            S[t - 1] = c_add<A>(x[t], x[R - t]);
+            D[t - 1] = c_sub<A>(x[t], x[R - t]);
+        }
+        Cx<A> sum = x[0];
+#pragma unroll
+        for (int t = 0; t < H; ++t) sum = c_add<A>(sum, S[t]);
+        out[0] = sum;
+#pragma unroll
+        for (int u = 1; u <= H; ++u) {
+            Cx<A> ac = x[0], bc = {A::lift(0.0), A::lift(0.0)};
+#pragma unroll
+            for (int t = 1; t <= H; ++t) {
+                const Cx<A> cs = gld<A>(oddc, ri * 7 + (t * u) % R);
+                ac = {A::add(ac.re, A::mul(S[t - 1].re, cs.re)), A::add(ac.im, A::mul(S[t - 1].im, cs.re))};
+                bc = {A::add(bc.re, A::mul(D[t - 1].re, cs.im)), A::add(bc.im, A::mul(D[t - 1].im, cs.im))};
+            }
+            Cx<A> xu = {A::add(ac.re, bc.im), A::sub(ac.im, bc.re)};
+            Cx<A> xr = {A::sub(ac.re, bc.im), A::add(ac.im, bc.re)};
+            if (twid) {
+                xu = c_mul<A>(xu, gld<A>(tw, u - 1));
+                xr = c_mul<A>(xr, gld<A>(tw, R - u - 1));
+            }
+            out[u] = xu;
+            out[R - u] = xr;
+        }
+    }
+}
+
+// One Stockham stage over G sub-FFTs in place: compute all of this thread's butterflies, sync, write back.
+template <class A, class RL, int G, int T, int I>
+__device__ __forceinline__ void stage(const Lds<A, RL::N * G>& L, int tid, const double* tw_all, const double* oddc) {
+    constexpr int N = RL::N, r = RL::r[I], s = RL::stride(I), nbf = N / r, W = G * nbf, IT = (W + T - 1) / T;
+    const double* tws = tw_all + (long long)RL::twoff(I) * kElt<A>;
+    Cx<A> held[IT][r];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int w = tid + it * T;
+        if (W % T == 0 || w < W) {
+            const int g = w % G, b = w / G, p = b / s;
+            Cx<A> x[r];
+#pragma unroll
+            for (int t = 0; t < r; ++t) x[t] = L.get((b + t * nbf) * G + g);
+            butterfly<A, r>(x, held[it], p != 0, tws + (long long)p * (r - 1) * kElt<A>, oddc);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int w = tid + it * T;
+        if (W % T == 0 || w < W) {
+            const int g = w % G, b = w / G, p = b / s, q = b - p * s;
+            const int dbase = s * r * p + q;
+#pragma unroll
+            for (int u = 0; u < r; ++u) L.put((dbase + u * s) * G + g, held[it][u]);
+        }
+    }
+    __syncthreads();
+}
+
+template <class A, class RL, int G, int T, int... I>
+__device__ __forceinline__ void all_stages(const Lds<A, RL::N * G>& L, int tid, const double* tw, const double* oddc,
+                                           std::integer_sequence<int, I...>) {
+    (stage<A, RL, G, T, I>(L, tid, tw, oddc), ...);
+}
+
+// Workgroup index remapped so that each of the 8 XCDs (which take workgroups round-robin) works on a contiguous
+// range of the grid: neighbouring sub-FFT groups share cache lines in that XCD's L2.
+__device__ __forceinline__ int xcd_wg() {
+    const int n = gridDim.x, i = blockIdx.x, full = n & ~7;
+    if (i >= full) return i;
+    return (i & 7) * (full >> 3) + (i >> 3);
+}
+
+// __launch_bounds__'s second value is the minimum number of waves per SIMD the register allocation must allow:
+// 4 caps a thread at 128 VGPRs, so that LDS (not registers) limits occupancy.
+template <class A, class RL, int G, int T, int LD, int ST>
+__global__ __launch_bounds__(T, 4) void k_fixed(PassArgs a) {
+    constexpr int N = RL::N, CAP = N * G;
+    extern __shared__ double lds[];   // kPlanes * CAP doubles
+    const Lds<A, CAP> L{lds};
+    const int tid = threadIdx.x;
+    const int wg = xcd_wg();
+    const int task = wg / a.ngrp, grp = wg - task * a.ngrp;
+    const int j0 = grp * G;
+    const int Gv = min(G, a.nsub - j0);
+    const long long gtask = a.task0 + task;
+    const long long ch = gtask / a.nblk, blk = gtask - ch * a.nblk;
+
+    {   // load: consecutive lanes take consecutive addresses
+        const bool gfast = a.ldg == 1 && a.lde != 1;
+        for (int w = tid; w < G * N; w += T) {
+            const int g = gfast ? w % G : w / N, e = gfast ? w / G : w % N;
+            if (g >= Gv) continue;
+            const long long i = e * a.lde + (long long)(j0 + g) * a.ldg;
+            Cx<A> v;
+            if constexpr (LD == LD_REAL_F64) {
+                const double2 pr = reinterpret_cast<const double2*>(a.src + ch * a.in_cstride + blk * a.in_bstride)[i];
+                v = {A::lift(pr.x), A::lift(pr.y)};
+            } else {
+                v = gld<A>(a.src + task * a.src_tstride, i);
+            }
+            L.put(e * G + g, v);
+        }
+    }
+    __syncthreads();
+    all_stages<A, RL, G, T>(L, tid, a.tw, a.oddc, std::make_integer_sequence<int, RL::nst>{});
+    {
+        const bool gfast = a.stg == 1 && a.ste != 1;
+        for (int w = tid; w < G * N; w += T) {
+            const int g = gfast ? w % G : w / N, e = gfast ? w / G : w % N;
+            if (g >= Gv) continue;
+            const long long jg = j0 + g;
+            const long long i = e * a.ste + jg * a.stg;
+            Cx<A> v = L.get(e * G + g);
+            if (a.twmode && e != 0 && (a.twmode == 1 || jg != 0)) v = c_mul<A>(v, gld<A>(a.tw4, i));
+            if constexpr (A::kDD) {
+                if (a.renorm) v = {quick2(v.re.hi, v.re.lo), quick2(v.im.hi, v.im.lo)};
+            }
+            if constexpr (ST == ST_CPLX) {
+                gst<A>(a.dst + task * a.dst_tstride, i, v);
+            } else {
+                double* o = a.out + ch * a.nout;
+                const long long x0 = 2 * i - a.margP;
+                if (x0 >= 0 && x0 < a.hopP) {
+                    const long long d = blk * a.hopP + x0 - a.o;
+                    if (d >= 0 && d < a.nout) {
+                        if constexpr (A::kDD) o[d] = v.re.hi + v.re.lo;
+                        else o[d] = v.re;
+                    }
+                }
+                const long long x1 = x0 + 1;
+                if (x1 >= 0 && x1 < a.hopP) {
+                    const long long d = blk * a.hopP + x1 - a.o;
+                    if (d >= 0 && d < a.nout) {
+                        const Val<A> ni = A::neg(v.im);
+                        if constexpr (A::kDD) o[d] = ni.hi + ni.lo;
+                        else o[d] = ni;
+                    }
+                }
+            }
+        }
+    }
+}
+
+struct Entry {
+    int policy, n, ld, st, G, T;
+    hipError_t (*launch)(const PassArgs&, hipStream_t);
+};
+
+template <class A, class RL, int G, int T, int LD, int ST> hipError_t launch_fixed_one(const PassArgs& a, hipStream_t s) {
+    static_assert(RL::hold(G, T) <= (A::kDD ? 8 : 16), "too many values held per thread");
+    constexpr size_t bytes = (size_t)kPlanes<A> * RL::N * G * sizeof(double);
+    static_assert(bytes <= 160 * 1024, "LDS tile too large");
+    static const hipError_t attr = bytes > 64 * 1024
+        ? hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fixed<A, RL, G, T, LD, ST>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes)
+        : hipSuccess;
+    if (attr != hipSuccess) return attr;
+    hipLaunchKernelGGL((k_fixed<A, RL, G, T, LD, ST>), dim3(a.ngrp * a.ntask), dim3(T), bytes, s, a);
+    return hipGetLastError();
+}
+
+template <int P> struct PolicyOf;
+template <> struct PolicyOf<POL_F64> { using type = F64; };
+template <> struct PolicyOf<POL_DDA> { using type = DDA; };
+template <> struct PolicyOf<POL_DDS> { using type = DDS; };
+
+#define KB_FIXED(POL, G, T, LD, ST, ...)                                                                            \
+    Entry{POL, Radices<__VA_ARGS__>::N, LD, ST, G, T,                                                              \
+          &launch_fixed_one<PolicyOf<POL>::type, Radices<__VA_ARGS__>, G, T, LD, ST>}
+
+// Transform lengths of the benchmark workloads (c1: 48k->44.1k mono; c2: 96k->44.1k exact and fast; c3: 44.1k->48k).
+const Entry kEntries[] = {
+    // exact (four-step sub-FFTs with the one-TwoSum add)
+    KB_FIXED(POL_DDS, 2, 128, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),    // 315: c2 forward columns
+    KB_FIXED(POL_DDS, 4, 320, LD_CPLX, ST_CPLX, 4, 4, 4, 5),        // 320: c2 forward rows
+    KB_FIXED(POL_DDS, 5, 192, LD_CPLX, ST_CPLX, 3, 3, 3, 7),        // 189: c2 inverse columns
+    KB_FIXED(POL_DDS, 9, 448, LD_CPLX, ST_OUT, 5, 7, 7),            // 245: c2 inverse rows
+    KB_FIXED(POL_DDS, 5, 192, LD_REAL_F64, ST_CPLX, 3, 3, 3, 7),    // 189: c3 forward columns
+    KB_FIXED(POL_DDS, 9, 448, LD_CPLX, ST_CPLX, 5, 7, 7),           // 245: c3 forward rows
+    KB_FIXED(POL_DDS, 8, 448, LD_CPLX, ST_CPLX, 4, 4, 2, 7),        // 224: c3 inverse columns
+    KB_FIXED(POL_DDS, 7, 320, LD_CPLX, ST_OUT, 3, 3, 5, 5),         // 225: c3 inverse rows
+    KB_FIXED(POL_DDS, 5, 128, LD_REAL_F64, ST_CPLX, 5, 5, 5),       // 125: c1 forward columns
+    KB_FIXED(POL_DDS, 8, 320, LD_CPLX, ST_CPLX, 4, 4, 2, 5),        // 160: c1 forward rows
+    KB_FIXED(POL_DDS, 5, 128, LD_CPLX, ST_CPLX, 5, 5, 5),           // 125: c1 inverse columns
+    KB_FIXED(POL_DDS, 11, 320, LD_CPLX, ST_OUT, 3, 7, 7),           // 147: c1 inverse rows
+    // fast (FP64)
+    KB_FIXED(POL_F64, 6, 128, LD_REAL_F64, ST_CPLX, 4, 2, 5, 5),    // 200: c2 fast forward columns
+    KB_FIXED(POL_F64, 6, 128, LD_CPLX, ST_CPLX, 4, 2, 5, 5),        // 200: c2 fast forward rows
+    KB_FIXED(POL_F64, 10, 128, LD_CPLX, ST_CPLX, 5, 5, 5),          // 125: c2 fast inverse columns
+    KB_FIXED(POL_F64, 6, 128, LD_CPLX, ST_OUT, 3, 7, 7),            // 147: c2 fast inverse rows
+};
+#undef KB_FIXED
+
+}  // namespace
+
+bool launch_fixed(int policy, int ld, int st, PassArgs& a, hipStream_t s, hipError_t* err) {
+    for (const Entry& e : kEntries) {
+        if (e.policy == policy && e.n == a.sd.n && e.ld == ld && e.st == st) {
+            a.G = e.G;
+            a.ngrp = (a.nsub + e.G - 1) / e.G;
+            *err = e.launch(a, s);
+            return true;
+        }
+    }
+    return false;
+}
+
+}  // namespace kbsrc::gpu

@@ -110,7 +110,7 @@ def _c2_plan_run(mode):
     assert H.sha(H.wav_f64(x, 2, 96000)) == H.C2_IN
     with gpu().Plan(len(x), 2, 96000, 44100, mode) as plan:
         info = plan.info
-        xx = torch.empty(2 * info["tot"], dtype=torch.float64, device="cuda")
+        xx = torch.empty(2 * info["xstride"], dtype=torch.float64, device="cuda")
         out = torch.empty(2 * info["nout"], dtype=torch.float64, device="cuda")
         plan.stage(x, xx.data_ptr())
         plan.execute(xx.data_ptr(), out.data_ptr())
