@@ -141,13 +141,15 @@ int pick_group(int policy, int n, int nsub) {
 }
 
 // launch one sub-FFT pass; errors name the pass and its shape
-bool use_fixed() {
-    static const bool on = [] {
-        const char* v = std::getenv("KBSRC_GENERIC_PASSES");
-        return !(v && v[0] == '1');
-    }();
-    return on;
+// Test and tuning switches, read at every execute: KBSRC_GENERIC_PASSES=1 runs every sub-FFT pass through the
+// generic kernel; KBSRC_FUSED=1 folds the split/multiply/fold kernel into the passes next to it (same bits; it
+// saves a kernel and ~1.2 GB of HBM traffic per benchmark step but measured no faster, the passes being
+// FP64-bound).
+bool env_flag(const char* name) {
+    const char* v = std::getenv(name);
+    return v && v[0] == '1';
 }
+bool use_fixed() { return !env_flag("KBSRC_GENERIC_PASSES"); }
 
 void run_pass(int policy, int ld, int st, PassArgs a, hipStream_t s, const char* what) {
     hipError_t e = hipSuccess;
@@ -221,7 +223,7 @@ Engine::Engine(const Plan& p, const std::vector<dd>& H, hipStream_t s) : p_(p), 
     }
     H_.upload(hv.data(), hv.size() * sizeof(double), s);
     const size_t elt = dd_ ? 32 : 16;
-    const size_t per_task = (size_t)std::max(fw_.h, iv_.h) * elt;
+    const size_t per_task = (size_t)(std::max(fw_.h, iv_.h) + 1) * elt;   // + 1: the split's X[h]
     ts_ = (long long)(per_task / sizeof(double));
     const size_t budget = (size_t)1 << 30;
     tb_ = (int)std::max<long long>(1, std::min<long long>(ntask_, (long long)(budget / per_task)));
@@ -250,6 +252,10 @@ void Engine::run_cfft(const CFFTPlan& f, int ld, const double* src, int st, doub
         a.o = p_.o;
         a.margP = (long long)p_.marg * p_.P;
         a.hopP = (long long)p_.hop * p_.P;
+        FuseArgs& fa = a.f;
+        fa.H = H_.d(); fa.wf = fw_.w.d(); fa.wi = iv_.w.d();
+        fa.h = fw_.h; fa.hinv = iv_.h; fa.Nb = p_.Nb; fa.Mb = p_.Mb; fa.K = p_.Mb / 2 + 1;
+        fa.jmax = std::min(p_.Nb / 2, p_.Mb - 1); fa.up = p_.up; fa.xmax = p_.up ? fw_.h : fa.jmax;
     };
     auto timed = [&](double* acc, auto&& fn) {
         hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -278,12 +284,14 @@ void Engine::run_cfft(const CFFTPlan& f, int ld, const double* src, int st, doub
         a.src = in_src; a.lde = f.N2; a.ldg = 1;
         a.dst = mid; a.dst_tstride = mid_ts; a.ste = f.N2; a.stg = 1;
         a.tw4 = f.tw4.d(); a.twmode = 1; a.renorm = dd_;
+        a.f.M = f.N2;
         timed(t_col, [&] { run_pass(pol_four, ld, ST_CPLX, a, s, "column pass"); });
         PassArgs b;
         fill_sub(b, f.s2, k_, dd_, pol_four, f.N1);
         common(b);
         b.src = mid; b.src_tstride = mid_ts; b.lde = 1; b.ldg = f.N2;
         b.dst = dst; b.dst_tstride = dst_tstride; b.ste = f.N1; b.stg = 1; b.renorm = dd_;
+        b.f.M = f.N1;
         timed(t_row, [&] { run_pass(pol_four, LD_CPLX, st, b, s, "row pass"); });
     } else {
         PassArgs a;
@@ -299,8 +307,19 @@ void Engine::execute(const double* xx, double* out, hipStream_t s, PassTimes* ti
     const long long ts = ts_;
     PassTimes local;
     PassTimes* tp = times ? &local : nullptr;
+    const int pol_four = dd_ ? POL_DDS : POL_F64;
+    const bool fused = use_fixed() && env_flag("KBSRC_FUSED") && fw_.c.four && iv_.c.four &&
+                       has_fixed(pol_four, fw_.c.N2, LD_CPLX, ST_SPLIT) && has_fixed(pol_four, iv_.c.N1, LD_FOLD, ST_CPLX);
     for (long long t0 = 0; t0 < ntask_; t0 += tb_) {
         const int nt = (int)std::min<long long>(tb_, ntask_ - t0);
+        if (fused) {
+            // padded input -> (w1) -> X = split of z (w2) -> bin map and pre-split folded into the inverse columns
+            run_cfft(fw_.c, LD_REAL_F64, nullptr, ST_SPLIT, w1_.d(), w2_.d(), t0, nt, xx, out, s, &local.fwd_col,
+                     &local.fwd_row, tp);
+            run_cfft(iv_.c, LD_FOLD, w2_.d(), ST_OUT, w1_.d(), nullptr, t0, nt, xx, out, s, &local.inv_col,
+                     &local.inv_row, tp);
+            continue;
+        }
         // forward real FFT: padded input -> z (w2)
         run_cfft(fw_.c, LD_REAL_F64, nullptr, ST_CPLX, w1_.d(), w2_.d(), t0, nt, xx, out, s, &local.fwd_col,
                  &local.fwd_row, tp);

@@ -18,8 +18,22 @@ struct SubDesc {
     StageDesc st[kMaxStages];
 };
 
-enum LoadKind { LD_REAL_F64 = 0, LD_REAL_DD = 1, LD_CPLX = 2 };
-enum StoreKind { ST_CPLX = 0, ST_OUT = 1 };
+enum LoadKind { LD_REAL_F64 = 0, LD_REAL_DD = 1, LD_CPLX = 2, LD_FOLD = 3 };
+enum StoreKind { ST_CPLX = 0, ST_OUT = 1, ST_SPLIT = 2 };
+
+// Real-FFT split, spectrum product and bin map fused into the passes next to them (specialised passes only):
+//   ST_SPLIT  forward row pass: stores X[i] = forward split of z (src/dsp/fft.hpp:365-407) for i <= xmax;
+//   LD_FOLD   inverse column pass: loads X, forms Y = X H with the bin map (src/resampler.cpp:64-78) and the
+//             inverse pre-split (src/dsp/fft.hpp:412-458) before its sub-FFTs.
+// Both need each sub-FFT j in the same workgroup as its mirror M - j: the workgroup takes slot pairs
+// {s, M - s} (slot 0 and, for even M, slot M/2 alone).
+struct FuseArgs {
+    const double* H = nullptr;      // real kernel spectrum, (hi, lo) pairs or FP64
+    const double* wf = nullptr;     // forward split twiddles w[0..h]
+    const double* wi = nullptr;     // inverse split twiddles w[0..hinv]
+    int h = 0, hinv = 0, Nb = 0, Mb = 0, K = 0, jmax = 0, up = 0, xmax = 0;
+    int M = 0;                      // mirror modulus of the sub-FFT index: N1 (rows) or N2 (columns)
+};
 enum Policy { POL_F64 = 0, POL_DDA = 1, POL_DDS = 2 };
 
 // A batch of equal sub-FFTs, G of them per workgroup, staged through LDS. Element e of sub-FFT j (within a
@@ -45,6 +59,7 @@ struct PassArgs {
     // ST_OUT: unpack r[2i] = re, r[2i+1] = -im, keep [margP, margP + hopP), write out[c*nout + b*hopP + x - o]
     double* out = nullptr;
     long long nout = 0, o = 0, margP = 0, hopP = 0;
+    FuseArgs f;
 };
 
 // Spectrum x H, bin map and inverse pre-split for one batch of tasks (src/fft.hpp:361-409,411-458;
@@ -84,6 +99,7 @@ hipError_t launch_pass(int policy, int ld, int st, const PassArgs& a, int thread
 // Compile-time specialised pass for the benchmark transform lengths; false when (policy, n, ld, st) has none.
 // Sets a.G and a.ngrp to the specialisation's group size.
 bool launch_fixed(int policy, int ld, int st, PassArgs& a, hipStream_t s, hipError_t* err);
+bool has_fixed(int policy, int n, int ld, int st);
 hipError_t launch_mid(int policy, const MidArgs& a, hipStream_t s);
 hipError_t launch_stage(const StageArgs& a, hipStream_t s);
 hipError_t launch_spec_split(const SpecArgs& a, hipStream_t s);

@@ -6,6 +6,7 @@
 // single LDS buffer: each thread computes all of its butterflies of a stage into registers, the workgroup
 // synchronises, and the results are written back in place. Half the LDS of the ping-pong form, so twice the
 // workgroups per CU.
+#include <type_traits>
 #include <utility>
 
 #include "kernels.hpp"
@@ -181,27 +182,167 @@ __device__ __forceinline__ int xcd_wg() {
     return (i & 7) * (full >> 3) + (i >> 3);
 }
 
+// Arithmetic of the splits and the bin map: the reference's scalar dd (accurate add) or FP64.
+template <class A> using SplitPol = std::conditional_t<A::kDD, DDA, F64>;
+
+template <class A> __device__ __forceinline__ Val<A> ldH(const double* H, long long i) {
+    if constexpr (A::kDD) {
+        const double2 v = reinterpret_cast<const double2*>(H)[i];
+        return dd{v.x, v.y};
+    } else {
+        return H[i];
+    }
+}
+
+// Slot pairing of fused passes: local sub-FFT g is member (g & 1) of slot grp*G/2 + g/2; member 0 is the slot's
+// own index s, member 1 its mirror M - s. Returns the sub-FFT index, or -1 for an empty place.
+template <int G> __device__ __forceinline__ int slot_index(int g, int grp, int M) {
+    const int slot = grp * (G / 2) + g / 2, mem = g & 1;
+    if (2 * slot > M) return -1;
+    if (mem && (slot == 0 || 2 * slot == M)) return -1;
+    return mem ? M - slot : slot;
+}
+template <int G> __device__ __forceinline__ int slot_local(int j, int grp, int M) {
+    const int mem = (j != 0 && 2 * j > M) ? 1 : 0;
+    const int slot = mem ? M - j : j;
+    return 2 * (slot - grp * (G / 2)) + mem;
+}
+
+// Y[idx] = X[idx] H[idx] with the bin map (src/resampler.cpp:64-78): down folds bins above the output Nyquist onto
+// Mb - j, up extends X periodically.
+template <class S> __device__ __forceinline__ Cx<S> ybin(const FuseArgs& f, const double* X, int idx) {
+    if (!f.up) {
+        const Cx<S> x = gld<S>(X, idx);
+        const Val<S> hk = ldH<S>(f.H, idx);
+        if (idx == f.hinv) return {S::mul(S::mul(S::lift(2.0), x.re), hk), S::lift(0.0)};
+        Cx<S> y = {S::mul(x.re, hk), S::mul(x.im, hk)};
+        const int j = f.Mb - idx;
+        if (j >= f.K && j <= f.jmax) {
+            const Cx<S> xj = gld<S>(X, j);
+            const Val<S> hj = ldH<S>(f.H, j);
+            y = c_add<S>(y, c_conj<S>(Cx<S>{S::mul(xj.re, hj), S::mul(xj.im, hj)}));
+        }
+        return y;
+    }
+    const int kk = idx % f.Nb;
+    const Cx<S> v = kk <= f.Nb / 2 ? gld<S>(X, kk) : c_conj<S>(gld<S>(X, f.Nb - kk));
+    const Val<S> hk = ldH<S>(f.H, idx);
+    Cx<S> y = {S::mul(v.re, hk), S::mul(v.im, hk)};
+    if (idx == f.hinv) y.im = S::lift(0.0);
+    return y;
+}
+
+// inverse pre-split of bin k from Y[k] and its partner Y[hinv - k] (src/dsp/fft.hpp:412-458)
+template <class S> __device__ __forceinline__ Cx<S> presplit(const FuseArgs& f, int k, Cx<S> Y, Cx<S> Yp) {
+    if (!S::kDD || 2 * k <= f.hinv) {   // v-form: A = Y[k], B = conj(Y[hinv-k])
+        const Cx<S> B = c_conj<S>(Yp);
+        const Cx<S> E = c_add<S>(Y, B), D = c_mul<S>(c_sub<S>(Y, B), c_conj<S>(gld<S>(f.wi, k)));
+        return c_conj<S>(Cx<S>{S::sub(E.re, D.im), S::add(E.im, D.re)});
+    }
+    // dd, upper half: the partner's E and split (A = Y[hinv-k], B = conj(Y[k])) with this bin's twiddle
+    const Cx<S> B = c_conj<S>(Y);
+    const Cx<S> E = c_add<S>(Yp, B), sp = c_sub<S>(Yp, B);
+    const Cx<S> Ej = c_conj<S>(E);
+    const Cx<S> Dj = c_mul<S>(Cx<S>{S::neg(sp.re), sp.im}, c_conj<S>(gld<S>(f.wi, k)));
+    return c_conj<S>(Cx<S>{S::sub(Ej.re, Dj.im), S::add(Ej.im, Dj.re)});
+}
+
+// forward split X[i] from z[i] and z[h - i] (src/dsp/fft.hpp:365-407), 0 < i < h
+template <class S> __device__ __forceinline__ Cx<S> fsplit(const FuseArgs& f, int i, Cx<S> zi, Cx<S> zp) {
+    const Val<S> half = S::lift(0.5);
+    Cx<S> v;
+    if (!S::kDD || 2 * i <= f.h) {   // v-form: A = z[i], B = conj(z[h-i])
+        const Cx<S> B = c_conj<S>(zp);
+        const Cx<S> E = c_add<S>(zi, B), O = c_mul_mi<S>(c_sub<S>(zi, B));
+        v = c_add<S>(E, c_mul<S>(O, gld<S>(f.wf, i)));
+    } else {                          // dd, upper half: from the partner's E and O (A = z[h-i], B = conj(z[i]))
+        const Cx<S> B = c_conj<S>(zi);
+        const Cx<S> E = c_add<S>(zp, B), O = c_mul_mi<S>(c_sub<S>(zp, B));
+        v = c_add<S>(c_conj<S>(E), c_mul<S>(c_conj<S>(O), gld<S>(f.wf, i)));
+    }
+    return {S::mul(v.re, half), S::mul(v.im, half)};
+}
+
+template <class A> __device__ __forceinline__ Cx<A> renorm_dd(Cx<A> v) {
+    if constexpr (A::kDD) return {quick2(v.re.hi, v.re.lo), quick2(v.im.hi, v.im.lo)};
+    else return v;
+}
+
 // __launch_bounds__'s second value is the minimum number of waves per SIMD the register allocation must allow:
 // 4 caps a thread at 128 VGPRs, so that LDS (not registers) limits occupancy.
 template <class A, class RL, int G, int T, int LD, int ST>
 __global__ __launch_bounds__(T, 4) void k_fixed(PassArgs a) {
     constexpr int N = RL::N, CAP = N * G;
+    constexpr bool kPaired = LD == LD_FOLD || ST == ST_SPLIT;
+    using S = SplitPol<A>;
+    static_assert(std::is_same_v<Val<S>, Val<A>>);
     extern __shared__ double lds[];   // kPlanes * CAP doubles
     const Lds<A, CAP> L{lds};
     const int tid = threadIdx.x;
     const int wg = xcd_wg();
     const int task = wg / a.ngrp, grp = wg - task * a.ngrp;
     const int j0 = grp * G;
-    const int Gv = min(G, a.nsub - j0);
     const long long gtask = a.task0 + task;
     const long long ch = gtask / a.nblk, blk = gtask - ch * a.nblk;
+    // sub-FFT index of local g, or -1
+    auto jof = [&](int g) -> int {
+        if constexpr (kPaired) return slot_index<G>(g, grp, a.f.M);
+        else return j0 + g < a.nsub ? j0 + g : -1;
+    };
 
-    {   // load: consecutive lanes take consecutive addresses
+    if constexpr (LD == LD_FOLD) {
+        // Y = X H with the bin map for every element, then the pre-split of each with its mirror bin
+        const double* X = a.src + task * a.src_tstride;
+        constexpr int IT = (G * N + T - 1) / T;
+        Cx<A> yh = {A::lift(0.0), A::lift(0.0)};
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int w = tid + it * T;
+            if (w < G * N) {
+                const int g = w % G, e = w / G, j = jof(g);
+                if (j >= 0) {
+                    const int k = e * a.f.M + j;
+                    L.put(e * G + g, ybin<S>(a.f, X, k));
+                    if (k == 0) yh = ybin<S>(a.f, X, a.f.hinv);
+                }
+            }
+        }
+        __syncthreads();
+        Cx<A> zs[IT];
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int w = tid + it * T;
+            if (w < G * N) {
+                const int g = w % G, e = w / G, j = jof(g);
+                if (j >= 0) {
+                    const int k = e * a.f.M + j;
+                    Cx<A> yp;
+                    if (k == 0) {
+                        yp = yh;
+                    } else {
+                        const int kp = a.f.hinv - k, ep = kp / a.f.M, jp = kp - ep * a.f.M;
+                        yp = L.get(ep * G + slot_local<G>(jp, grp, a.f.M));
+                    }
+                    zs[it] = presplit<S>(a.f, k, L.get(e * G + g), yp);
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int w = tid + it * T;
+            if (w < G * N) {
+                const int g = w % G, e = w / G;
+                if (jof(g) >= 0) L.put(e * G + g, zs[it]);
+            }
+        }
+    } else {   // load: consecutive lanes take consecutive addresses
         const bool gfast = a.ldg == 1 && a.lde != 1;
         for (int w = tid; w < G * N; w += T) {
             const int g = gfast ? w % G : w / N, e = gfast ? w / G : w % N;
-            if (g >= Gv) continue;
-            const long long i = e * a.lde + (long long)(j0 + g) * a.ldg;
+            const int j = jof(g);
+            if (j < 0) continue;
+            const long long i = e * a.lde + (long long)j * a.ldg;
             Cx<A> v;
             if constexpr (LD == LD_REAL_F64) {
                 const double2 pr = reinterpret_cast<const double2*>(a.src + ch * a.in_cstride + blk * a.in_bstride)[i];
@@ -214,17 +355,37 @@ __global__ __launch_bounds__(T, 4) void k_fixed(PassArgs a) {
     }
     __syncthreads();
     all_stages<A, RL, G, T>(L, tid, a.tw, a.oddc, std::make_integer_sequence<int, RL::nst>{});
-    {
+
+    if constexpr (ST == ST_SPLIT) {
+        // the sub-FFT outputs are z[M*e + j] (natural order); X[i] needs z[i] and z[h - i]
+        double* X = a.dst + task * a.dst_tstride;
+        for (int w = tid; w < G * N; w += T) {
+            const int g = w % G, e = w / G, j = jof(g);
+            if (j < 0) continue;
+            const int i = e * a.f.M + j;
+            const Cx<S> zi = renorm_dd<A>(L.get(e * G + g));
+            if (i == 0) {
+                gst<S>(X, 0, Cx<S>{S::add(zi.re, zi.im), S::lift(0.0)});
+                if (a.f.h <= a.f.xmax) gst<S>(X, a.f.h, Cx<S>{S::sub(zi.re, zi.im), S::lift(0.0)});
+                continue;
+            }
+            if (i > a.f.xmax) continue;
+            const int ip = a.f.h - i, ep = ip / a.f.M, jp = ip - ep * a.f.M;
+            const Cx<S> zp = renorm_dd<A>(L.get(ep * G + slot_local<G>(jp, grp, a.f.M)));
+            gst<S>(X, i, fsplit<S>(a.f, i, zi, zp));
+        }
+    } else {
         const bool gfast = a.stg == 1 && a.ste != 1;
         for (int w = tid; w < G * N; w += T) {
             const int g = gfast ? w % G : w / N, e = gfast ? w / G : w % N;
-            if (g >= Gv) continue;
-            const long long jg = j0 + g;
+            const int j = jof(g);
+            if (j < 0) continue;
+            const long long jg = j;
             const long long i = e * a.ste + jg * a.stg;
             Cx<A> v = L.get(e * G + g);
             if (a.twmode && e != 0 && (a.twmode == 1 || jg != 0)) v = c_mul<A>(v, gld<A>(a.tw4, i));
             if constexpr (A::kDD) {
-                if (a.renorm) v = {quick2(v.re.hi, v.re.lo), quick2(v.im.hi, v.im.lo)};
+                if (a.renorm) v = renorm_dd<A>(v);
             }
             if constexpr (ST == ST_CPLX) {
                 gst<A>(a.dst + task * a.dst_tstride, i, v);
@@ -259,6 +420,8 @@ struct Entry {
 
 template <class A, class RL, int G, int T, int LD, int ST> hipError_t launch_fixed_one(const PassArgs& a, hipStream_t s) {
     static_assert(RL::hold(G, T) <= (A::kDD ? 8 : 16), "too many values held per thread");
+    static_assert(LD != LD_FOLD || (G * RL::N + T - 1) / T <= (A::kDD ? 8 : 16), "too many values held per thread");
+    static_assert((LD != LD_FOLD && ST != ST_SPLIT) || G % 2 == 0, "paired passes need an even group");
     constexpr size_t bytes = (size_t)kPlanes<A> * RL::N * G * sizeof(double);
     static_assert(bytes <= 160 * 1024, "LDS tile too large");
     static const hipError_t attr = bytes > 64 * 1024
@@ -294,21 +457,37 @@ const Entry kEntries[] = {
     KB_FIXED(POL_DDS, 8, 320, LD_CPLX, ST_CPLX, 4, 4, 2, 5),        // 160: c1 forward rows
     KB_FIXED(POL_DDS, 5, 128, LD_CPLX, ST_CPLX, 5, 5, 5),           // 125: c1 inverse columns
     KB_FIXED(POL_DDS, 11, 320, LD_CPLX, ST_OUT, 3, 7, 7),           // 147: c1 inverse rows
+    // exact, fused: forward rows with the real-FFT split, inverse columns with x H, bin map and pre-split
+    KB_FIXED(POL_DDS, 4, 320, LD_CPLX, ST_SPLIT, 4, 4, 4, 5),       // 320: c2
+    KB_FIXED(POL_DDS, 6, 192, LD_FOLD, ST_CPLX, 3, 3, 3, 7),        // 189: c2
+    KB_FIXED(POL_DDS, 8, 448, LD_CPLX, ST_SPLIT, 5, 7, 7),          // 245: c3
+    KB_FIXED(POL_DDS, 8, 448, LD_FOLD, ST_CPLX, 4, 4, 2, 7),        // 224: c3
+    KB_FIXED(POL_DDS, 8, 320, LD_CPLX, ST_SPLIT, 4, 4, 2, 5),       // 160: c1
+    KB_FIXED(POL_DDS, 4, 128, LD_FOLD, ST_CPLX, 5, 5, 5),           // 125: c1
     // fast (FP64)
     KB_FIXED(POL_F64, 6, 128, LD_REAL_F64, ST_CPLX, 4, 2, 5, 5),    // 200: c2 fast forward columns
     KB_FIXED(POL_F64, 6, 128, LD_CPLX, ST_CPLX, 4, 2, 5, 5),        // 200: c2 fast forward rows
     KB_FIXED(POL_F64, 10, 128, LD_CPLX, ST_CPLX, 5, 5, 5),          // 125: c2 fast inverse columns
     KB_FIXED(POL_F64, 6, 128, LD_CPLX, ST_OUT, 3, 7, 7),            // 147: c2 fast inverse rows
+    KB_FIXED(POL_F64, 6, 128, LD_CPLX, ST_SPLIT, 4, 2, 5, 5),       // 200: c2 fast, fused split
+    KB_FIXED(POL_F64, 10, 128, LD_FOLD, ST_CPLX, 5, 5, 5),          // 125: c2 fast, fused fold
 };
 #undef KB_FIXED
 
 }  // namespace
 
+bool has_fixed(int policy, int n, int ld, int st) {
+    for (const Entry& e : kEntries)
+        if (e.policy == policy && e.n == n && e.ld == ld && e.st == st) return true;
+    return false;
+}
+
 bool launch_fixed(int policy, int ld, int st, PassArgs& a, hipStream_t s, hipError_t* err) {
     for (const Entry& e : kEntries) {
         if (e.policy == policy && e.n == a.sd.n && e.ld == ld && e.st == st) {
             a.G = e.G;
-            a.ngrp = (a.nsub + e.G - 1) / e.G;
+            if (ld == LD_FOLD || st == ST_SPLIT) a.ngrp = (a.f.M / 2 + 1 + e.G / 2 - 1) / (e.G / 2);
+            else a.ngrp = (a.nsub + e.G - 1) / e.G;
             *err = e.launch(a, s);
             return true;
         }

@@ -146,3 +146,18 @@ def test_c3_upsampling_full_size_pin():
     x = H.c3_input()
     y = gpu().resample(x, 44100, 48000, "exact")
     assert H.wav_sha_planar(y, 48000) == H.C3_EXACT
+
+
+@pytest.mark.parametrize("env", [{"KBSRC_FUSED": "1"}, {"KBSRC_GENERIC_PASSES": "1"}], ids=["fused", "generic"])
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_engine_variants_agree(env, mode, monkeypatch):
+    """The fused (split and bin map inside the passes) and the generic-pass engines give the pinned bits too."""
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    say(f"variant {env} {mode}")
+    y = gpu().resample(H.c1_input(), 48000, 44100, mode)
+    assert H.wav_sha_planar(y, 44100) == H.C1_PINS[(mode, True)]
+    x = np.random.default_rng(11).uniform(-0.5, 0.5, 2 * 30000)
+    want = H.oracle_resample(x, 2, 96000, 44100, mode, length=0.05)
+    got = gpu().resample(x.reshape(-1, 2), 96000, 44100, mode, length=0.05)
+    assert got.tobytes() == want.tobytes()
