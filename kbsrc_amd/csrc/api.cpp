@@ -2,6 +2,7 @@
 #include <xmmintrin.h>
 
 #include <chrono>
+#include <cmath>
 #include <climits>
 #include <numeric>
 #include <cstring>
@@ -87,6 +88,7 @@ struct kbsrc_plan {
     kbsrc_options opt;
     std::vector<dd> H;
     std::unique_ptr<Engine> eng;
+    int Jg = 0;
     double design_seconds = 0;
 };
 
@@ -102,12 +104,27 @@ std::unique_ptr<kbsrc_plan> make(long long frames, int channels, int rate_in, in
     pl->p = make_plan(frames, channels, rate_in, rate_out, o.precision == KBSRC_FAST, o.length, o.beta, o.block);
     const Plan& p = pl->p;
     if (p.equal) return pl;
-    if (p.gen) throw std::runtime_error("near-unity rate ratios (general fold) are not supported yet");
     DeviceConsts k;
     k.init(s);
-    const std::vector<dd> h = kernel_taps((int)p.Lk, p.c2 / p.gc, p.fk / p.gc, o.beta);
-    pl->H = kernel_spectrum_gpu(h, p.up ? p.Mb : p.Nb, (double)p.Nb, k, s);
-    pl->eng = std::make_unique<Engine>(p, pl->H, s);
+    if (p.gen) {
+        // near-unity ratios: the kernel on a finer grid Nk, its true response folded modulo Mb in the engine
+        // (src/resampler.cpp:190-202)
+        long long Nk = (long long)std::max(p.Nb, p.Mb) * 5 / 4;
+        Nk += Nk & 1;
+        while (!smooth7(Nk / 2)) Nk += 2;
+        const __int128 A = (__int128)p.c2 * p.Nb, B = (__int128)Nk * p.fi;
+        const long long g2 = (long long)std::gcd((unsigned long long)p.c2 * (unsigned long long)p.Nb,
+                                                 (unsigned long long)Nk * (unsigned long long)p.fi);
+        const double fk2 = (double)Nk * p.fi / p.Nb;
+        const std::vector<dd> h2 = kernel_taps((int)std::floor(o.length * fk2), (long long)(A / g2),
+                                               (long long)(B / g2), o.beta);
+        pl->Jg = (int)std::min<long long>(Nk / 2, (long long)std::ceil((p.mn / 2.0 + 4 * p.fml) * p.Nb / p.fi));
+        pl->H = kernel_spectrum_gpu(h2, (int)Nk, (double)p.Nb, k, s);
+    } else {
+        const std::vector<dd> h = kernel_taps((int)p.Lk, p.c2 / p.gc, p.fk / p.gc, o.beta);
+        pl->H = kernel_spectrum_gpu(h, p.up ? p.Mb : p.Nb, (double)p.Nb, k, s);
+    }
+    pl->eng = std::make_unique<Engine>(p, pl->H, pl->Jg, s);
     pl->design_seconds = now() - t0;
     return pl;
 }
@@ -144,7 +161,7 @@ void fill_stats(const kbsrc_plan& pl, double process_seconds, kbsrc_stats* st) {
     st->taps = p.equal ? 0 : 2 * p.Lk + 1;
     st->block = p.nb;
     st->threads = 1;
-    st->general_fold = 0;
+    st->general_fold = pl.Jg != 0;
     st->design_seconds = pl.design_seconds;
     st->process_seconds = process_seconds;
     st->passband = p.equal ? 0 : (double)p.c2 / 2 - p.fml;

@@ -79,55 +79,6 @@ void SubFFT::init(int n, bool as_dd, hipStream_t s) {
     tw.upload(v.data(), v.size() * sizeof(double), s);
 }
 
-void CFFTPlan::init(int n_, bool as_dd_, hipStream_t s) {
-    n = n_;
-    as_dd = as_dd_;
-    if (!smooth7(n)) throw std::runtime_error("transform lengths with prime factors above 7 are not supported yet");
-    four = n > 2048;
-    if (four) {
-        int best = 1;
-        for (int d = 1; (long long)d * d <= n; ++d)
-            if (n % d == 0) best = d;
-        N1 = best;
-        N2 = n / best;
-        if (N1 < 8) four = false;
-    }
-    if (!four) {
-        sd.init(n, as_dd, s);
-        return;
-    }
-    s1.init(N1, as_dd, s);
-    s2.init(N2, as_dd, s);
-    CisTable ct(n);
-    std::vector<dd> re((size_t)n), im((size_t)n);
-    for (int k1 = 0; k1 < N1; ++k1)
-        for (int n2 = 0; n2 < N2; ++n2) ct.get(-(long long)k1 * n2, re[(size_t)k1 * N2 + n2], im[(size_t)k1 * N2 + n2]);
-    const std::vector<double> v = pack_cx(re, im, as_dd);
-    tw4.upload(v.data(), v.size() * sizeof(double), s);
-}
-
-void RFFTPlan::init(int N_, bool as_dd, hipStream_t s) {
-    N = N_;
-    h = N / 2;
-    if (N % 2) throw std::runtime_error("RFFT length must be even");
-    c.init(h, as_dd, s);
-    CisTable ct(N);
-    std::vector<dd> re(h + 1), im(h + 1);
-    for (int k = 0; k <= h; ++k) ct.get(-k, re[k], im[k]);
-    const std::vector<double> v = pack_cx(re, im, as_dd);
-    w.upload(v.data(), v.size() * sizeof(double), s);
-}
-
-void DeviceConsts::init(hipStream_t s) {
-    std::vector<dd> re(21, mkdd(0.0)), im(21, mkdd(0.0));
-    const int rr[3] = {3, 5, 7};
-    for (int i = 0; i < 3; ++i)
-        for (int k = 0; k < rr[i]; ++k) cis(k, rr[i], re[i * 7 + k], im[i * 7 + k]);
-    const std::vector<double> a = pack_cx(re, im, true), b = pack_cx(re, im, false);
-    odd_dd.upload(a.data(), a.size() * sizeof(double), s);
-    odd_f64.upload(b.data(), b.size() * sizeof(double), s);
-}
-
 namespace {
 
 constexpr int kThreads = 256;
@@ -170,36 +121,186 @@ void fill_sub(PassArgs& a, const SubFFT& sf, const DeviceConsts& k, bool as_dd, 
 
 }  // namespace
 
+void DeviceConsts::init(hipStream_t s) {
+    std::vector<dd> re(21, mkdd(0.0)), im(21, mkdd(0.0));
+    const int rr[3] = {3, 5, 7};
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < rr[i]; ++k) cis(k, rr[i], re[i * 7 + k], im[i * 7 + k]);
+    const std::vector<double> a = pack_cx(re, im, true), b = pack_cx(re, im, false);
+    odd_dd.upload(a.data(), a.size() * sizeof(double), s);
+    odd_f64.upload(b.data(), b.size() * sizeof(double), s);
+}
+
+void CFFTPlan::init(int n_, bool as_dd_, const DeviceConsts& k, hipStream_t s) {
+    n = n_;
+    as_dd = as_dd_;
+    blue = !smooth7(n);
+    if (blue) {   // Bluestein (src/dsp/fft.hpp:228-240)
+        m = (int)next_smooth7(2LL * n - 1);
+        inner = std::make_unique<CFFTPlan>();
+        inner->init(m, as_dd, k, s);
+        std::vector<dd> cr(n), ci(n), vr(m, mkdd(0.0)), vi(m, mkdd(0.0));
+        for (long long t = 0; t < n; ++t) {
+            const long long e = (long long)((__int128)t * t % (2LL * n));
+            cis(e, 2LL * n, cr[t], ci[t]);
+            vr[t] = cr[t];
+            vi[t] = ci[t];
+            if (t) { vr[m - t] = cr[t]; vi[m - t] = ci[t]; }
+        }
+        const std::vector<double> bv = pack_cx(cr, ci, as_dd);
+        bc.upload(bv.data(), bv.size() * sizeof(double), s);
+        // the chirp sequence's spectrum in the reference's scalar-dd form, divided by m (src/dsp/fft.hpp:236-238)
+        CFFTPlan vp;
+        vp.init(m, true, k, s);
+        const std::vector<double> vv = pack_cx(vr, vi, true);
+        const long long ts = 4LL * vp.work_elems();
+        DevBuf src, dst, s0(ts * 8), s1(ts * 8), s2(ts * 8);
+        src.upload(vv.data(), vv.size() * sizeof(double), s);
+        dst.alloc(ts * 8);
+        RunCtx c;
+        c.ts = ts; c.k = &k; c.dd = true; c.lane = false; c.s = s;
+        c.proto.ntask = 1;
+        double* const scr[3] = {s0.d(), s1.d(), s2.d()};
+        double tc = 0, tr = 0;
+        run_cfft(vp, c, LD_CPLX, src.d(), ST_CPLX, dst.d(), scr, &tc, &tr);
+        std::vector<double> V(4 * (size_t)m);
+        hip_check(hipMemcpyAsync(V.data(), dst.d(), V.size() * sizeof(double), hipMemcpyDeviceToHost, s), "chirp copy");
+        hip_check(hipStreamSynchronize(s), "chirp sync");
+        std::vector<dd> fr(m), fi(m);
+        for (int q = 0; q < m; ++q) {
+            fr[q] = dddiv1(dd{V[4 * q], V[4 * q + 2]}, (double)m);
+            fi[q] = dddiv1(dd{V[4 * q + 1], V[4 * q + 3]}, (double)m);
+        }
+        const std::vector<double> fv = pack_cx(fr, fi, as_dd);
+        vf.upload(fv.data(), fv.size() * sizeof(double), s);
+        return;
+    }
+    four = n > 2048;
+    if (four) {
+        int best = 1;
+        for (int d = 1; (long long)d * d <= n; ++d)
+            if (n % d == 0) best = d;
+        N1 = best;
+        N2 = n / best;
+        if (N1 < 8) four = false;
+    }
+    if (!four) {
+        sd.init(n, as_dd, s);
+        return;
+    }
+    s1.init(N1, as_dd, s);
+    s2.init(N2, as_dd, s);
+    CisTable ct(n);
+    std::vector<dd> re((size_t)n), im((size_t)n);
+    for (int k1 = 0; k1 < N1; ++k1)
+        for (int n2 = 0; n2 < N2; ++n2) ct.get(-(long long)k1 * n2, re[(size_t)k1 * N2 + n2], im[(size_t)k1 * N2 + n2]);
+    const std::vector<double> v = pack_cx(re, im, as_dd);
+    tw4.upload(v.data(), v.size() * sizeof(double), s);
+}
+
+void RFFTPlan::init(int N_, bool as_dd, const DeviceConsts& k, hipStream_t s) {
+    N = N_;
+    h = N / 2;
+    if (N % 2) throw std::runtime_error("RFFT length must be even");
+    c.init(h, as_dd, k, s);
+    CisTable ct(N);
+    std::vector<dd> re(h + 1), im(h + 1);
+    for (int q = 0; q <= h; ++q) ct.get(-q, re[q], im[q]);
+    const std::vector<double> v = pack_cx(re, im, as_dd);
+    w.upload(v.data(), v.size() * sizeof(double), s);
+}
+
+namespace {
+
+// brackets one launch with HIP events on the launch stream when timing is asked for
+template <class F> void timed(const RunCtx& c, double* acc, F&& fn) {
+    if (!c.times) {
+        fn();
+        return;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hip_check(hipEventCreate(&e0), "event");
+    hip_check(hipEventCreate(&e1), "event");
+    hip_check(hipEventRecord(e0, c.s), "event record");
+    fn();
+    hip_check(hipEventRecord(e1, c.s), "event record");
+    hip_check(hipEventSynchronize(e1), "event sync");
+    float ms = 0;
+    hip_check(hipEventElapsedTime(&ms, e0, e1), "event time");
+    *acc += ms;
+    c.times->launches++;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
+}  // namespace
+
+void run_cfft(const CFFTPlan& f, const RunCtx& c, int ld, const double* src, int st, double* dst,
+              double* const scr[3], double* t_col, double* t_row) {
+    const bool lane = c.lane;
+    const int pol_four = !c.dd ? POL_F64 : lane ? POL_DDS : POL_DDA;
+    const int pol_one = c.dd ? POL_DDA : POL_F64;
+    const long long ts = c.ts;
+    if (f.blue) {   // X_k = conj(b_k) sum_j x_j conj(b_j) b_{k-j}, through the inner FFT twice (src/dsp/fft.hpp:268-278)
+        BlueArgs b;
+        b.n = f.n; b.m = f.m; b.ntask = c.proto.ntask; b.task0 = c.proto.task0; b.nblk = c.proto.nblk;
+        b.in_cstride = c.proto.in_cstride; b.in_bstride = c.proto.in_bstride;
+        b.bc = f.bc.d(); b.vf = f.vf.d();
+        b.out = c.proto.out; b.nout = c.proto.nout; b.o = c.proto.o; b.margP = c.proto.margP; b.hopP = c.proto.hopP;
+        const int pol = pol_one;
+        b.src = src; b.src_tstride = ld == LD_REAL_DD ? 0 : ts; b.dst = scr[0]; b.dst_tstride = ts;
+        timed(c, t_col, [&] { hip_check(launch_blue(pol, BLUE_PRE, ld, b, c.s), "Bluestein pre-multiply"); });
+        double* const inner_scr[3] = {scr[1], nullptr, nullptr};
+        run_cfft(*f.inner, c, LD_CPLX, scr[0], ST_CPLX, scr[2], inner_scr, t_col, t_row);
+        b.dst = scr[2];
+        timed(c, t_row, [&] { hip_check(launch_blue(pol, BLUE_MID, 0, b, c.s), "Bluestein chirp product"); });
+        run_cfft(*f.inner, c, LD_CPLX, scr[2], ST_CPLX, scr[0], inner_scr, t_col, t_row);
+        b.src = scr[0]; b.src_tstride = ts; b.dst = dst; b.dst_tstride = ts;
+        timed(c, t_row, [&] { hip_check(launch_blue(pol, BLUE_POST, st, b, c.s), "Bluestein post-multiply"); });
+        return;
+    }
+    const long long src_ts = ld == LD_REAL_DD ? 0 : ts;
+    if (f.four) {
+        PassArgs a = c.proto;
+        fill_sub(a, f.s1, *c.k, c.dd, pol_four, f.N2);
+        a.src = src; a.src_tstride = src_ts; a.lde = f.N2; a.ldg = 1;
+        a.dst = scr[0]; a.dst_tstride = ts; a.ste = f.N2; a.stg = 1;
+        a.tw4 = f.tw4.d(); a.twmode = lane ? 1 : 2; a.renorm = lane && c.dd;
+        a.f.M = f.N2;
+        timed(c, t_col, [&] { run_pass(pol_four, ld, ST_CPLX, a, c.s, "column pass"); });
+        PassArgs b = c.proto;
+        fill_sub(b, f.s2, *c.k, c.dd, pol_four, f.N1);
+        b.src = scr[0]; b.src_tstride = ts; b.lde = 1; b.ldg = f.N2;
+        b.dst = dst; b.dst_tstride = ts; b.ste = f.N1; b.stg = 1; b.renorm = lane && c.dd;
+        b.f.M = f.N1;
+        timed(c, t_row, [&] { run_pass(pol_four, LD_CPLX, st, b, c.s, "row pass"); });
+    } else {
+        PassArgs a = c.proto;
+        fill_sub(a, f.sd, *c.k, c.dd, pol_one, 1);
+        a.src = src; a.src_tstride = src_ts; a.lde = 1; a.ldg = 0;
+        a.dst = dst; a.dst_tstride = ts; a.ste = 1; a.stg = 0;
+        timed(c, t_row, [&] { run_pass(pol_one, ld, st, a, c.s, "single pass"); });
+    }
+}
+
 std::vector<dd> kernel_spectrum_gpu(const std::vector<dd>& h, int n, double scale, const DeviceConsts& k,
                                     hipStream_t s) {
     RFFTPlan R;
-    R.init(n, true, s);
+    R.init(n, true, k, s);
     const std::vector<dd> r = circular_kernel(h, n);
     std::vector<double> rv(2 * (size_t)n);
     for (int i = 0; i < n; ++i) { rv[2 * i] = r[i].hi; rv[2 * i + 1] = r[i].lo; }
-    DevBuf din, w1((size_t)R.h * 32), w2((size_t)R.h * 32), dH((size_t)(R.h + 1) * 16);
+    const long long ts = 4LL * std::max<long long>(R.c.work_elems(), R.h + 1);
+    DevBuf din, z(ts * 8), s0(ts * 8), s1(ts * 8), s2(ts * 8), dH((size_t)(R.h + 1) * 16);
     din.upload(rv.data(), rv.size() * sizeof(double), s);
-    const CFFTPlan& c = R.c;
-    if (c.four) {   // scalar-dd four-step: accurate adds, no twiddle on column 0, values stored as computed
-        PassArgs a;
-        fill_sub(a, c.s1, k, true, POL_DDA, c.N2);
-        a.src = din.d(); a.lde = c.N2; a.ldg = 1;
-        a.dst = w1.d(); a.ste = c.N2; a.stg = 1; a.tw4 = c.tw4.d(); a.twmode = 2;
-        run_pass(POL_DDA, LD_REAL_DD, ST_CPLX, a, s, "spectrum column pass");
-        PassArgs b;
-        fill_sub(b, c.s2, k, true, POL_DDA, c.N1);
-        b.src = w1.d(); b.lde = 1; b.ldg = c.N2;
-        b.dst = w2.d(); b.ste = c.N1; b.stg = 1;
-        run_pass(POL_DDA, LD_CPLX, ST_CPLX, b, s, "spectrum row pass");
-    } else {
-        PassArgs a;
-        fill_sub(a, c.sd, k, true, POL_DDA, 1);
-        a.src = din.d(); a.lde = 1; a.ldg = 0;
-        a.dst = w2.d(); a.ste = 1; a.stg = 0;
-        run_pass(POL_DDA, LD_REAL_DD, ST_CPLX, a, s, "spectrum pass");
-    }
+    RunCtx c;
+    c.ts = ts; c.k = &k; c.dd = true; c.lane = false; c.s = s;
+    c.proto.ntask = 1;
+    double* const scr[3] = {s0.d(), s1.d(), s2.d()};
+    double tc = 0, tr = 0;
+    run_cfft(R.c, c, LD_REAL_DD, din.d(), ST_CPLX, z.d(), scr, &tc, &tr);
     SpecArgs sa;
-    sa.z = w2.d(); sa.w = R.w.d(); sa.H = dH.d(); sa.h = R.h; sa.scale = scale;
+    sa.z = z.d(); sa.w = R.w.d(); sa.H = dH.d(); sa.h = R.h; sa.scale = scale;
     hip_check(launch_spec_split(sa, s), "spectrum split");
     std::vector<double> hv(2 * (size_t)(R.h + 1));
     hip_check(hipMemcpyAsync(hv.data(), dH.d(), hv.size() * sizeof(double), hipMemcpyDeviceToHost, s), "spectrum copy");
@@ -209,12 +310,11 @@ std::vector<dd> kernel_spectrum_gpu(const std::vector<dd>& h, int n, double scal
     return H;
 }
 
-Engine::Engine(const Plan& p, const std::vector<dd>& H, hipStream_t s) : p_(p), dd_(!p.fast) {
-    if (p.gen) throw std::runtime_error("near-unity rate ratios (general fold) are not supported yet");
+Engine::Engine(const Plan& p, const std::vector<dd>& H, int Jg, hipStream_t s) : p_(p), dd_(!p.fast), Jg_(Jg) {
     ntask_ = (long long)p.ch * p.nblk;
     k_.init(s);
-    fw_.init(p.Nb, dd_, s);
-    iv_.init(p.Mb, dd_, s);
+    fw_.init(p.Nb, dd_, k_, s);
+    iv_.init(p.Mb, dd_, k_, s);
     // H in the element format of the engine: (hi, lo) pairs or FP64
     std::vector<double> hv(H.size() * (dd_ ? 2 : 1));
     for (size_t i = 0; i < H.size(); ++i) {
@@ -223,131 +323,68 @@ Engine::Engine(const Plan& p, const std::vector<dd>& H, hipStream_t s) : p_(p), 
     }
     H_.upload(hv.data(), hv.size() * sizeof(double), s);
     const size_t elt = dd_ ? 32 : 16;
-    const size_t per_task = (size_t)(std::max(fw_.h, iv_.h) + 1) * elt;   // + 1: the split's X[h]
+    const long long elems = std::max({(long long)fw_.h + 1, (long long)iv_.h + 1, fw_.c.work_elems(),
+                                      iv_.c.work_elems()});   // + 1: the split's X[h]
+    const size_t per_task = (size_t)elems * elt;
     ts_ = (long long)(per_task / sizeof(double));
+    const bool blue = fw_.c.blue || iv_.c.blue;
     const size_t budget = (size_t)1 << 30;
     tb_ = (int)std::max<long long>(1, std::min<long long>(ntask_, (long long)(budget / per_task)));
     if (const char* v = std::getenv("KBSRC_BATCH_TASKS")) {   // tuning knob: tasks per launch
         const long long want = std::atoll(v);
         if (want > 0) tb_ = (int)std::min<long long>(tb_, want);
     }
-    w1_.alloc(per_task * tb_);
-    w2_.alloc(per_task * tb_);
-}
-
-void Engine::run_cfft(const CFFTPlan& f, int ld, const double* src, int st, double* mid, double* dst,
-                      long long task0, int nt, const double* xx, double* out, hipStream_t s, double* t_col,
-                      double* t_row, PassTimes* times) {
-    const int pol_four = dd_ ? POL_DDS : POL_F64, pol_one = dd_ ? POL_DDA : POL_F64;
-    const long long mid_ts = ts_, src_tstride = ts_, dst_tstride = ts_;
-    auto common = [&](PassArgs& a) {
-        a.ntask = nt;
-        a.task0 = task0;
-        a.nblk = p_.nblk;
-        a.in_cstride = p_.xstride;
-        a.in_bstride = (long long)p_.hop * p_.Q;
-        a.src_tstride = src_tstride;
-        a.out = out;
-        a.nout = p_.nout;
-        a.o = p_.o;
-        a.margP = (long long)p_.marg * p_.P;
-        a.hopP = (long long)p_.hop * p_.P;
-        FuseArgs& fa = a.f;
-        fa.H = H_.d(); fa.wf = fw_.w.d(); fa.wi = iv_.w.d();
-        fa.h = fw_.h; fa.hinv = iv_.h; fa.Nb = p_.Nb; fa.Mb = p_.Mb; fa.K = p_.Mb / 2 + 1;
-        fa.jmax = std::min(p_.Nb / 2, p_.Mb - 1); fa.up = p_.up; fa.xmax = p_.up ? fw_.h : fa.jmax;
-    };
-    auto timed = [&](double* acc, auto&& fn) {
-        hipEvent_t e0 = nullptr, e1 = nullptr;
-        if (times) {
-            hip_check(hipEventCreate(&e0), "event");
-            hip_check(hipEventCreate(&e1), "event");
-            hip_check(hipEventRecord(e0, s), "event record");
-        }
-        fn();
-        if (times) {
-            hip_check(hipEventRecord(e1, s), "event record");
-            hip_check(hipEventSynchronize(e1), "event sync");
-            float ms = 0;
-            hip_check(hipEventElapsedTime(&ms, e0, e1), "event time");
-            *acc += ms;
-            times->launches++;
-            (void)hipEventDestroy(e0);
-            (void)hipEventDestroy(e1);
-        }
-    };
-    const double* in_src = ld == LD_REAL_F64 ? xx : src;
-    if (f.four) {
-        PassArgs a;
-        fill_sub(a, f.s1, k_, dd_, pol_four, f.N2);
-        common(a);
-        a.src = in_src; a.lde = f.N2; a.ldg = 1;
-        a.dst = mid; a.dst_tstride = mid_ts; a.ste = f.N2; a.stg = 1;
-        a.tw4 = f.tw4.d(); a.twmode = 1; a.renorm = dd_;
-        a.f.M = f.N2;
-        timed(t_col, [&] { run_pass(pol_four, ld, ST_CPLX, a, s, "column pass"); });
-        PassArgs b;
-        fill_sub(b, f.s2, k_, dd_, pol_four, f.N1);
-        common(b);
-        b.src = mid; b.src_tstride = mid_ts; b.lde = 1; b.ldg = f.N2;
-        b.dst = dst; b.dst_tstride = dst_tstride; b.ste = f.N1; b.stg = 1; b.renorm = dd_;
-        b.f.M = f.N1;
-        timed(t_row, [&] { run_pass(pol_four, LD_CPLX, st, b, s, "row pass"); });
-    } else {
-        PassArgs a;
-        fill_sub(a, f.sd, k_, dd_, pol_one, 1);
-        common(a);
-        a.src = in_src; a.lde = 1; a.ldg = 0;
-        a.dst = dst; a.dst_tstride = dst_tstride; a.ste = 1; a.stg = 0;
-        timed(t_row, [&] { run_pass(pol_one, ld, st, a, s, "single pass"); });
-    }
+    for (int i = 0; i < (blue ? 4 : 2); ++i) w_[i].alloc(per_task * tb_);
 }
 
 void Engine::execute(const double* xx, double* out, hipStream_t s, PassTimes* times) {
-    const long long ts = ts_;
     PassTimes local;
-    PassTimes* tp = times ? &local : nullptr;
     const int pol_four = dd_ ? POL_DDS : POL_F64;
-    const bool fused = use_fixed() && env_flag("KBSRC_FUSED") && fw_.c.four && iv_.c.four &&
+    const bool fused = use_fixed() && env_flag("KBSRC_FUSED") && !Jg_ && fw_.c.four && iv_.c.four &&
                        has_fixed(pol_four, fw_.c.N2, LD_CPLX, ST_SPLIT) && has_fixed(pol_four, iv_.c.N1, LD_FOLD, ST_CPLX);
+    RunCtx c;
+    c.ts = ts_; c.k = &k_; c.dd = dd_; c.lane = true; c.s = s; c.times = times ? &local : nullptr;
+    PassArgs& a = c.proto;
+    a.nblk = p_.nblk;
+    a.in_cstride = p_.xstride;
+    a.in_bstride = (long long)p_.hop * p_.Q;
+    a.out = out;
+    a.nout = p_.nout;
+    a.o = p_.o;
+    a.margP = (long long)p_.marg * p_.P;
+    a.hopP = (long long)p_.hop * p_.P;
+    FuseArgs& fa = a.f;
+    fa.H = H_.d(); fa.wf = fw_.w.d(); fa.wi = iv_.w.d();
+    fa.h = fw_.h; fa.hinv = iv_.h; fa.Nb = p_.Nb; fa.Mb = p_.Mb; fa.K = p_.Mb / 2 + 1;
+    fa.jmax = std::min(p_.Nb / 2, p_.Mb - 1); fa.up = p_.up; fa.xmax = p_.up ? fw_.h : fa.jmax;
+    double* W0 = w_[0].d();
+    double* W1 = w_[1].d();
+    double* W2 = w_[2].d();
+    double* W3 = w_[3].d();
     for (long long t0 = 0; t0 < ntask_; t0 += tb_) {
         const int nt = (int)std::min<long long>(tb_, ntask_ - t0);
+        a.ntask = nt;
+        a.task0 = t0;
         if (fused) {
-            // padded input -> (w1) -> X = split of z (w2) -> bin map and pre-split folded into the inverse columns
-            run_cfft(fw_.c, LD_REAL_F64, nullptr, ST_SPLIT, w1_.d(), w2_.d(), t0, nt, xx, out, s, &local.fwd_col,
-                     &local.fwd_row, tp);
-            run_cfft(iv_.c, LD_FOLD, w2_.d(), ST_OUT, w1_.d(), nullptr, t0, nt, xx, out, s, &local.inv_col,
-                     &local.inv_row, tp);
+            // padded input -> (W1) -> X = split of z (W0) -> bin map and pre-split folded into the inverse columns
+            double* const sf[3] = {W1, W2, W3};
+            run_cfft(fw_.c, c, LD_REAL_F64, xx, ST_SPLIT, W0, sf, &local.fwd_col, &local.fwd_row);
+            run_cfft(iv_.c, c, LD_FOLD, W0, ST_OUT, nullptr, sf, &local.inv_col, &local.inv_row);
             continue;
         }
-        // forward real FFT: padded input -> z (w2)
-        run_cfft(fw_.c, LD_REAL_F64, nullptr, ST_CPLX, w1_.d(), w2_.d(), t0, nt, xx, out, s, &local.fwd_col,
-                 &local.fwd_row, tp);
-        // split, x H, bin map, inverse pre-split: z (w2) -> z' (w1)
+        // forward real FFT: padded input -> z (W0)
+        double* const sf[3] = {W1, W2, W3};
+        run_cfft(fw_.c, c, LD_REAL_F64, xx, ST_CPLX, W0, sf, &local.fwd_col, &local.fwd_row);
+        // split, x H, bin map, inverse pre-split: z (W0) -> z' (W1)
         MidArgs m;
-        m.z = w2_.d(); m.z_tstride = ts; m.zi = w1_.d(); m.zi_tstride = ts;
+        m.z = W0; m.z_tstride = ts_; m.zi = W1; m.zi_tstride = ts_;
         m.H = H_.d(); m.wf = fw_.w.d(); m.wi = iv_.w.d();
         m.h = fw_.h; m.hinv = iv_.h; m.Nb = p_.Nb; m.Mb = p_.Mb; m.K = p_.Mb / 2 + 1;
-        m.jmax = std::min(p_.Nb / 2, p_.Mb - 1); m.up = p_.up; m.ntask = nt;
-        {
-            hipEvent_t e0 = nullptr, e1 = nullptr;
-            if (tp) {
-                hip_check(hipEventCreate(&e0), "event"); hip_check(hipEventCreate(&e1), "event");
-                hip_check(hipEventRecord(e0, s), "event record");
-            }
-            hip_check(launch_mid(dd_ ? POL_DDA : POL_F64, m, s), "mid kernel");
-            if (tp) {
-                hip_check(hipEventRecord(e1, s), "event record");
-                hip_check(hipEventSynchronize(e1), "event sync");
-                float ms = 0;
-                hip_check(hipEventElapsedTime(&ms, e0, e1), "event time");
-                local.mid += ms; local.launches++;
-                (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
-            }
-        }
-        // inverse: z' (w1) -> (w2) -> output samples
-        run_cfft(iv_.c, LD_CPLX, w1_.d(), ST_OUT, w2_.d(), nullptr, t0, nt, xx, out, s, &local.inv_col,
-                 &local.inv_row, tp);
+        m.jmax = std::min(p_.Nb / 2, p_.Mb - 1); m.up = p_.up; m.ntask = nt; m.Jg = Jg_;
+        timed(c, &local.mid, [&] { hip_check(launch_mid(dd_ ? POL_DDA : POL_F64, m, s), "mid kernel"); });
+        // inverse: z' (W1) -> output samples
+        double* const si[3] = {W0, W2, W3};
+        run_cfft(iv_.c, c, LD_CPLX, W1, ST_OUT, nullptr, si, &local.inv_col, &local.inv_row);
     }
     if (times) *times = local;
 }

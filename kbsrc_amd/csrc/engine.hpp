@@ -1,5 +1,5 @@
 // GPU engine: FFT plans with device-resident twiddle tables, the kernel spectrum, and the overlap-save block
-// engine over all (channel, block) tasks. Host-side orchestration only; the kernels are in kernels.hip.
+// engine over all (channel, block) tasks. Host-side orchestration only; the kernels are in kernels*.hip.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -45,6 +45,12 @@ private:
 // Element format of a policy's device tables: dd = {re.hi, im.hi, re.lo, im.lo}, FP64 = {re, im}.
 std::vector<double> pack_cx(const std::vector<dd>& re, const std::vector<dd>& im, bool as_dd);
 
+// Shared device constants (odd-radix butterfly constants in both element formats).
+struct DeviceConsts {
+    DevBuf odd_dd, odd_f64;
+    void init(hipStream_t s);
+};
+
 // One Stockham sub-FFT length with its stage list (radix order 4s, 2, 3s, 5s, 7s: src/dsp/fft.hpp:91-107).
 struct SubFFT {
     SubDesc desc;
@@ -53,13 +59,19 @@ struct SubFFT {
 };
 
 // Complex FFT of length n as the reference computes it (src/dsp/fft.hpp:226-265): four-step N1 x N2 above
-// 2048 points (N1 the largest divisor <= sqrt n, at least 8), otherwise one Stockham.
+// 2048 points (N1 the largest divisor <= sqrt n, at least 8), otherwise one Stockham; lengths with a prime factor
+// above 7 by Bluestein around an inner 7-smooth FFT of length m = next 7-smooth >= 2n - 1.
 struct CFFTPlan {
     int n = 0, N1 = 0, N2 = 0;
     bool four = false, as_dd = true;
     SubFFT s1, s2, sd;
     DevBuf tw4;   // four-step twiddles w_n^{-k1 n2} at k1*N2 + n2
-    void init(int n, bool as_dd, hipStream_t s);
+    bool blue = false;
+    int m = 0;
+    std::unique_ptr<CFFTPlan> inner;
+    DevBuf bc, vf;   // chirp (n) and the inner FFT of the chirp sequence divided by m (m)
+    void init(int n, bool as_dd, const DeviceConsts& k, hipStream_t s);
+    long long work_elems() const { return blue ? m : n; }   // elements per task a run needs in each buffer
 };
 
 // Real FFT of even length N through a complex FFT of N/2 (src/dsp/fft.hpp:352-358).
@@ -67,7 +79,7 @@ struct RFFTPlan {
     int N = 0, h = 0;
     CFFTPlan c;
     DevBuf w;   // w[k] = e^{-2 pi i k / N}, k = 0..h
-    void init(int N, bool as_dd, hipStream_t s);
+    void init(int N, bool as_dd, const DeviceConsts& k, hipStream_t s);
 };
 
 struct PassTimes {   // milliseconds per kernel kind, summed over one execute()
@@ -75,38 +87,47 @@ struct PassTimes {   // milliseconds per kernel kind, summed over one execute()
     int launches = 0;
 };
 
-// Shared device constants (odd-radix butterfly constants in both element formats).
-struct DeviceConsts {
-    DevBuf odd_dd, odd_f64;
-    void init(hipStream_t s);
+// How a complex FFT runs: lane = the reference's engine form (sub-FFTs on its 4-lane types: one-TwoSum dd adds,
+// renormalised stores, four-step twiddle on every k1 != 0), otherwise its scalar form (accurate dd adds, no twiddle
+// on column 0), which the kernel design and the Bluestein chirp spectrum use.
+struct RunCtx {
+    PassArgs proto;        // tasks, block addressing, output window, fused-pass arguments
+    long long ts = 0;      // doubles per task in every work buffer
+    const DeviceConsts* k = nullptr;
+    bool dd = true, lane = true;
+    hipStream_t s = nullptr;
+    PassTimes* times = nullptr;
 };
 
-// The kernel spectrum H (src/design.cpp:40-49) on the GPU: the circularly placed dd kernel goes through the
-// same sub-FFT passes in the reference's scalar-dd form, then a split that keeps Re X / scale.
+// Runs f over proto.ntask tasks: load kind ld from src, store kind st to dst (work buffers, stride ctx.ts).
+// scr: three work buffers the run may use (the four-step's middle buffer; Bluestein's).
+void run_cfft(const CFFTPlan& f, const RunCtx& c, int ld, const double* src, int st, double* dst,
+              double* const scr[3], double* t_col, double* t_row);
+
+// The kernel spectrum H (src/design.cpp:40-49) on the GPU: the circularly placed dd kernel through the passes in
+// the reference's scalar-dd form, then a split that keeps Re X / scale.
 std::vector<dd> kernel_spectrum_gpu(const std::vector<dd>& h, int n, double scale, const DeviceConsts& k,
                                     hipStream_t s);
 
-// The overlap-save engine for one plan (src/resampler.cpp:34-88).
+// The overlap-save engine for one plan (src/resampler.cpp:34-88). Jg > 0 selects the general fold.
 class Engine {
 public:
-    Engine(const Plan& p, const std::vector<dd>& H, hipStream_t s);
-    // xx: device, ch x plan.tot padded planar input. out: device, ch x plan.nout. times: optional per-pass timing.
+    Engine(const Plan& p, const std::vector<dd>& H, int Jg, hipStream_t s);
+    // xx: device, ch x plan.xstride padded planar input. out: device, ch x plan.nout. times: optional per-pass timing.
     void execute(const double* xx, double* out, hipStream_t s, PassTimes* times = nullptr);
     long long tasks() const { return ntask_; }
     int batch() const { return tb_; }
     const Plan& plan() const { return p_; }
 
 private:
-    void run_cfft(const CFFTPlan& f, int ld, const double* src, int st, double* mid, double* dst, long long task0,
-                  int nt, const double* xx, double* out, hipStream_t s, double* t_col, double* t_row,
-                  PassTimes* times);
     Plan p_;
     bool dd_;
+    int Jg_ = 0;
     long long ntask_ = 0, ts_ = 0;   // ts_: doubles per task in the work buffers
     int tb_ = 1;
-    RFFTPlan fw_, iv_;
     DeviceConsts k_;
-    DevBuf H_, w1_, w2_;
+    RFFTPlan fw_, iv_;
+    DevBuf H_, w_[4];
 };
 
 }  // namespace kbsrc::gpu

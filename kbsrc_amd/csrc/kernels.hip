@@ -250,7 +250,38 @@ template <class A> __device__ __forceinline__ Cx<A> fwd_split(const double* z, c
 template <class A> __device__ __forceinline__ Val<A> load_H(const double* H, int k) { return gload_real<A>(H, k); }
 
 // spectrum bin idx in [0, hinv] after the kernel multiply and the bin map (src/resampler.cpp:64-78)
+// general fold (src/resampler.cpp:53-63): Y[idx] sums X~[j mod Nb] H[j] over every j <= Jg with j = +-idx mod Mb, in
+// ascending j, so each output bin accumulates in the reference's order without atomics
+template <class A> __device__ __forceinline__ Cx<A> gen_term(const MidArgs& a, const double* z, int j) {
+    const int jj = j % a.Nb;
+    const Cx<A> x = jj <= a.Nb / 2 ? fwd_split<A>(z, a.wf, a.h, jj) : c_conj<A>(fwd_split<A>(z, a.wf, a.h, a.Nb - jj));
+    const Val<A> g = load_H<A>(a.H, j);
+    return {A::mul(x.re, g), A::mul(x.im, g)};
+}
+template <class A> __device__ __forceinline__ Cx<A> gen_bin(const MidArgs& a, const double* z, int idx) {
+    Cx<A> y = {A::lift(0.0), A::lift(0.0)};
+    if (idx == 0 || 2 * idx == a.Mb) {   // k == 0 or k == Mb/2: real part doubled (j = 0 itself adds whole)
+        for (int j = idx; j <= a.Jg; j += a.Mb) {
+            const Cx<A> v = gen_term<A>(a, z, j);
+            if (j == 0) y = c_add<A>(y, v);
+            else y.re = A::add(y.re, A::mul(A::lift(2.0), v.re));
+        }
+        y.im = A::lift(0.0);
+        return y;
+    }
+    for (int t = 0;; ++t) {   // j = idx + t Mb adds v, j = Mb - idx + t Mb adds conj(v)
+        const int j1 = idx + t * a.Mb;
+        if (j1 > a.Jg) break;
+        y = c_add<A>(y, gen_term<A>(a, z, j1));
+        const int j2 = a.Mb - idx + t * a.Mb;
+        if (j2 > a.Jg) break;
+        y = c_add<A>(y, c_conj<A>(gen_term<A>(a, z, j2)));
+    }
+    return y;
+}
+
 template <class A> __device__ __forceinline__ Cx<A> mid_bin(const MidArgs& a, const double* z, int idx) {
+    if (a.Jg) return gen_bin<A>(a, z, idx);
     if (!a.up) {
         if (idx == a.hinv) {   // output Nyquist bin: twice the real part (src/resampler.cpp:71)
             const Cx<A> x = fwd_split<A>(z, a.wf, a.h, idx);
@@ -326,6 +357,56 @@ __global__ __launch_bounds__(256) void k_spec_split(SpecArgs a) {
     a.H[2 * i + 1] = v.lo;
 }
 
+// ---- Bluestein steps (src/dsp/fft.hpp:268-278); arithmetic of the storage type (scalar dd or FP64)
+template <class A, int LD> __global__ __launch_bounds__(256) void k_blue_pre(BlueArgs a) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x, task = blockIdx.y;
+    if (j >= a.m) return;
+    Cx<A> u = {A::lift(0.0), A::lift(0.0)};
+    if (j < a.n) {
+        Cx<A> x;
+        if constexpr (LD == LD_REAL_F64) {
+            const long long gtask = a.task0 + task, ch = gtask / a.nblk, blk = gtask - ch * a.nblk;
+            const double* s = a.src + ch * a.in_cstride + blk * a.in_bstride;
+            x = {A::lift(s[2 * j]), A::lift(s[2 * j + 1])};
+        } else if constexpr (LD == LD_REAL_DD) {
+            const double* s = a.src + task * a.src_tstride;
+            x = {gload_real<A>(s, 2 * j), gload_real<A>(s, 2 * j + 1)};
+        } else {
+            x = gload<A>(a.src + task * a.src_tstride, j);
+        }
+        u = c_mul<A>(x, c_conj<A>(gload<A>(a.bc, j)));
+    }
+    gstore<A>(a.dst + task * a.dst_tstride, j, u);
+}
+
+template <class A> __global__ __launch_bounds__(256) void k_blue_mid(BlueArgs a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, task = blockIdx.y;
+    if (k >= a.m) return;
+    double* U = a.dst + task * a.dst_tstride;
+    gstore<A>(U, k, c_conj<A>(c_mul<A>(gload<A>(U, k), gload<A>(a.vf, k))));
+}
+
+template <class A, int ST> __global__ __launch_bounds__(256) void k_blue_post(BlueArgs a) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, task = blockIdx.y;
+    if (k >= a.n) return;
+    const Cx<A> y = c_mul<A>(c_conj<A>(gload<A>(a.src + task * a.src_tstride, k)), c_conj<A>(gload<A>(a.bc, k)));
+    if constexpr (ST == ST_CPLX) {
+        gstore<A>(a.dst + task * a.dst_tstride, k, y);
+    } else {
+        const long long gtask = a.task0 + task, ch = gtask / a.nblk, blk = gtask - ch * a.nblk;
+        double* o = a.out + ch * a.nout;
+        const long long x0 = 2LL * k - a.margP, x1 = x0 + 1;
+        if (x0 >= 0 && x0 < a.hopP) {
+            const long long d = blk * a.hopP + x0 - a.o;
+            if (d >= 0 && d < a.nout) o[d] = to_f64<A>(y.re);
+        }
+        if (x1 >= 0 && x1 < a.hopP) {
+            const long long d = blk * a.hopP + x1 - a.o;
+            if (d >= 0 && d < a.nout) o[d] = to_f64<A>(A::neg(y.im));
+        }
+    }
+}
+
 template <class A, int LD, int ST> hipError_t launch_one(const PassArgs& a, int threads, hipStream_t s) {
     const size_t lds = (size_t)2 * kPlanes<A> * a.sd.n * a.G * sizeof(double);
     static bool attr_set = false;
@@ -366,6 +447,25 @@ hipError_t launch_mid(int policy, const MidArgs& a, hipStream_t s) {
     const dim3 grid((a.hinv / 2 + 1 + 255) / 256, a.ntask);
     if (policy == POL_F64) hipLaunchKernelGGL(k_mid<F64>, grid, dim3(256), 0, s, a);
     else hipLaunchKernelGGL(k_mid<DDA>, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_blue(int policy, int phase, int kind, const BlueArgs& a, hipStream_t s) {
+    const dim3 grid((unsigned)(((phase == BLUE_POST ? a.n : a.m) + 255) / 256), a.ntask), block(256);
+#define KB_BLUE(POL)                                                                                             \
+    if (phase == BLUE_PRE && kind == LD_REAL_F64) hipLaunchKernelGGL((k_blue_pre<POL, LD_REAL_F64>), grid, block, 0, s, a); \
+    else if (phase == BLUE_PRE && kind == LD_REAL_DD) hipLaunchKernelGGL((k_blue_pre<POL, LD_REAL_DD>), grid, block, 0, s, a); \
+    else if (phase == BLUE_PRE) hipLaunchKernelGGL((k_blue_pre<POL, LD_CPLX>), grid, block, 0, s, a);                \
+    else if (phase == BLUE_MID) hipLaunchKernelGGL((k_blue_mid<POL>), grid, block, 0, s, a);                        \
+    else if (kind == ST_OUT) hipLaunchKernelGGL((k_blue_post<POL, ST_OUT>), grid, block, 0, s, a);                  \
+    else hipLaunchKernelGGL((k_blue_post<POL, ST_CPLX>), grid, block, 0, s, a);
+    if (policy == POL_F64) {
+        if (kind == LD_REAL_DD && phase == BLUE_PRE) return hipErrorInvalidValue;
+        KB_BLUE(F64)
+    } else {
+        KB_BLUE(DDA)
+    }
+#undef KB_BLUE
     return hipGetLastError();
 }
 

@@ -73,7 +73,27 @@ struct MidArgs {
     const double* wf = nullptr;     // forward split twiddles w[0..h]
     const double* wi = nullptr;     // inverse split twiddles w[0..hi]
     int h = 0, hinv = 0, Nb = 0, Mb = 0, K = 0, jmax = 0, up = 0, ntask = 0;
+    int Jg = 0;                     // > 0: general fold of H[0..Jg] on the finer kernel grid (near-unity ratios)
 };
+
+// Bluestein (chirp-z) steps of a complex FFT whose length n has a prime factor above 7 (src/dsp/fft.hpp:228-278),
+// around an inner 7-smooth FFT of length m >= 2n - 1:
+//   pre   u[j] = x[j] conj(b[j]) for j < n, 0 up to m        (x read like a pass's load side)
+//   mid   U[k] = conj(U[k] Vf[k])                           (in place)
+//   post  y[k] = conj(C[k]) conj(b[k]) for k < n             (stored like a pass's store side)
+struct BlueArgs {
+    int n = 0, m = 0, ntask = 0;
+    long long task0 = 0, nblk = 1, in_cstride = 0, in_bstride = 0;
+    const double* src = nullptr;
+    long long src_tstride = 0;
+    double* dst = nullptr;
+    long long dst_tstride = 0;
+    const double* bc = nullptr;     // chirp b[t] = cis(t^2 mod 2n, 2n), n elements
+    const double* vf = nullptr;     // inner FFT of the chirp sequence / m, m elements
+    double* out = nullptr;          // ST_OUT, as PassArgs
+    long long nout = 0, o = 0, margP = 0, hopP = 0;
+};
+enum BluePhase { BLUE_PRE = 0, BLUE_MID = 1, BLUE_POST = 2 };
 
 // Zero-padded planar staging of interleaved input with the edge extensions (src/resampler.cpp:168-186).
 struct StageArgs {
@@ -102,6 +122,8 @@ bool launch_fixed(int policy, int ld, int st, PassArgs& a, hipStream_t s, hipErr
 bool has_fixed(int policy, int n, int ld, int st);
 hipError_t launch_mid(int policy, const MidArgs& a, hipStream_t s);
 hipError_t launch_stage(const StageArgs& a, hipStream_t s);
+// policy: POL_DDA or POL_F64; kind: the LD kind (pre) or ST kind (post)
+hipError_t launch_blue(int policy, int phase, int kind, const BlueArgs& a, hipStream_t s);
 hipError_t launch_spec_split(const SpecArgs& a, hipStream_t s);
 
 }  // namespace kbsrc::gpu

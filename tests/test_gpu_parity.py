@@ -67,6 +67,14 @@ CASES = [
     (16000, 48000, 1, 3000, "exact", 0.01, 0, True),
     (48000, 16000, 2, 1000, "exact", 0.002, 0, True),
     (44100, 88200, 1, 20000, "exact", 0.05, 6000, True),
+    # lengths with prime factors above 7 (Bluestein) and near-unity ratios (general fold)
+    (96000, 32001, 1, 5000, "exact", 0.02, 0, True),
+    (44100, 44101, 1, 3000, "exact", 0.02, 0, True),
+    (48000, 48001, 1, 3000, "fast", 0.02, 0, False),
+    (22050, 48001, 1, 3000, "exact", 0.02, 0, True),
+    (1001, 1000, 1, 4000, "exact", 0.2, 0, True),
+    (44100, 44200, 2, 3000, "fast", 0.02, 0, True),
+    (48000, 44101, 1, 3000, "exact", 0.02, 0, True),
 ]
 
 

@@ -29,6 +29,10 @@ CASES = [
     (96000, 32001, 1, 5000, "exact", 0.02, 0, True),    # Bluestein lengths
     (44100, 44101, 1, 3000, "exact", 0.02, 0, True),    # near-unity: general fold
     (48000, 48001, 1, 3000, "fast", 0.02, 0, False),
+    (22050, 48001, 1, 3000, "exact", 0.02, 0, True),
+    (1001, 1000, 1, 4000, "exact", 0.2, 0, True),
+    (44100, 44200, 2, 3000, "fast", 0.02, 0, True),
+    (48000, 44101, 1, 3000, "exact", 0.02, 0, True),
 ]
 
 
