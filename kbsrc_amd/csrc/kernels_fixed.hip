@@ -6,6 +6,7 @@
 // single LDS buffer: each thread computes all of its butterflies of a stage into registers, the workgroup
 // synchronises, and the results are written back in place. Half the LDS of the ping-pong form, so twice the
 // workgroups per CU.
+#include <cstdlib>
 #include <type_traits>
 #include <utility>
 
@@ -336,10 +337,11 @@ __global__ __launch_bounds__(T, 4) void k_fixed(PassArgs a) {
                 if (jof(g) >= 0) L.put(e * G + g, zs[it]);
             }
         }
-    } else {   // load: consecutive lanes take consecutive addresses
-        const bool gfast = a.ldg == 1 && a.lde != 1;
+    } else {
+        // load, lanes sub-FFT-fastest: a wave writes one contiguous LDS range (no bank conflicts) and reads 64/G
+        // consecutive elements of each of its G sub-FFTs (or G neighbouring columns of 64/G rows)
         for (int w = tid; w < G * N; w += T) {
-            const int g = gfast ? w % G : w / N, e = gfast ? w / G : w % N;
+            const int g = w % G, e = w / G;
             const int j = jof(g);
             if (j < 0) continue;
             const long long i = e * a.lde + (long long)j * a.ldg;
@@ -375,9 +377,8 @@ __global__ __launch_bounds__(T, 4) void k_fixed(PassArgs a) {
             gst<S>(X, i, fsplit<S>(a.f, i, zi, zp));
         }
     } else {
-        const bool gfast = a.stg == 1 && a.ste != 1;
-        for (int w = tid; w < G * N; w += T) {
-            const int g = gfast ? w % G : w / N, e = gfast ? w / G : w % N;
+        for (int w = tid; w < G * N; w += T) {   // lanes sub-FFT-fastest, as the load
+            const int g = w % G, e = w / G;
             const int j = jof(g);
             if (j < 0) continue;
             const long long jg = j;
@@ -443,12 +444,31 @@ template <> struct PolicyOf<POL_DDS> { using type = DDS; };
           &launch_fixed_one<PolicyOf<POL>::type, Radices<__VA_ARGS__>, G, T, LD, ST>}
 
 // Transform lengths of the benchmark workloads (c1: 48k->44.1k mono; c2: 96k->44.1k exact and fast; c3: 44.1k->48k).
+// Several (G, T) variants may follow one another for the same (policy, n, load, store); the first is the default
+// and KBSRC_TUNE="n:k,..." selects the k-th for length n (the sweep that picks the defaults).
 const Entry kEntries[] = {
     // exact (four-step sub-FFTs with the one-TwoSum add)
     KB_FIXED(POL_DDS, 2, 128, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),    // 315: c2 forward columns
-    KB_FIXED(POL_DDS, 4, 320, LD_CPLX, ST_CPLX, 4, 4, 4, 5),        // 320: c2 forward rows
-    KB_FIXED(POL_DDS, 5, 192, LD_CPLX, ST_CPLX, 3, 3, 3, 7),        // 189: c2 inverse columns
-    KB_FIXED(POL_DDS, 9, 448, LD_CPLX, ST_OUT, 5, 7, 7),            // 245: c2 inverse rows
+    KB_FIXED(POL_DDS, 4, 448, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),
+    KB_FIXED(POL_DDS, 4, 512, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),
+    KB_FIXED(POL_DDS, 7, 512, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),
+    KB_FIXED(POL_DDS, 8, 512, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),
+    KB_FIXED(POL_DDS, 8, 512, LD_CPLX, ST_CPLX, 4, 4, 4, 5),        // 320: c2 forward rows
+    KB_FIXED(POL_DDS, 4, 320, LD_CPLX, ST_CPLX, 4, 4, 4, 5),
+    KB_FIXED(POL_DDS, 4, 256, LD_CPLX, ST_CPLX, 4, 4, 4, 5),
+    KB_FIXED(POL_DDS, 4, 512, LD_CPLX, ST_CPLX, 4, 4, 4, 5),
+    KB_FIXED(POL_DDS, 2, 128, LD_CPLX, ST_CPLX, 4, 4, 4, 5),
+    KB_FIXED(POL_DDS, 6, 384, LD_CPLX, ST_CPLX, 4, 4, 4, 5),
+    KB_FIXED(POL_DDS, 8, 512, LD_CPLX, ST_CPLX, 3, 3, 3, 7),        // 189: c2 inverse columns
+    KB_FIXED(POL_DDS, 5, 192, LD_CPLX, ST_CPLX, 3, 3, 3, 7),
+    KB_FIXED(POL_DDS, 7, 320, LD_CPLX, ST_CPLX, 3, 3, 3, 7),
+    KB_FIXED(POL_DDS, 7, 512, LD_CPLX, ST_CPLX, 3, 3, 3, 7),
+    KB_FIXED(POL_DDS, 4, 256, LD_CPLX, ST_CPLX, 3, 3, 3, 7),
+    KB_FIXED(POL_DDS, 7, 512, LD_CPLX, ST_OUT, 5, 7, 7),            // 245: c2 inverse rows
+    KB_FIXED(POL_DDS, 9, 448, LD_CPLX, ST_OUT, 5, 7, 7),
+    KB_FIXED(POL_DDS, 9, 512, LD_CPLX, ST_OUT, 5, 7, 7),
+    KB_FIXED(POL_DDS, 4, 256, LD_CPLX, ST_OUT, 5, 7, 7),
+    KB_FIXED(POL_DDS, 2, 128, LD_CPLX, ST_OUT, 5, 7, 7),
     KB_FIXED(POL_DDS, 5, 192, LD_REAL_F64, ST_CPLX, 3, 3, 3, 7),    // 189: c3 forward columns
     KB_FIXED(POL_DDS, 9, 448, LD_CPLX, ST_CPLX, 5, 7, 7),           // 245: c3 forward rows
     KB_FIXED(POL_DDS, 8, 448, LD_CPLX, ST_CPLX, 4, 4, 2, 7),        // 224: c3 inverse columns
@@ -465,10 +485,16 @@ const Entry kEntries[] = {
     KB_FIXED(POL_DDS, 8, 320, LD_CPLX, ST_SPLIT, 4, 4, 2, 5),       // 160: c1
     KB_FIXED(POL_DDS, 4, 128, LD_FOLD, ST_CPLX, 5, 5, 5),           // 125: c1
     // fast (FP64)
-    KB_FIXED(POL_F64, 6, 128, LD_REAL_F64, ST_CPLX, 4, 2, 5, 5),    // 200: c2 fast forward columns
-    KB_FIXED(POL_F64, 6, 128, LD_CPLX, ST_CPLX, 4, 2, 5, 5),        // 200: c2 fast forward rows
+    KB_FIXED(POL_F64, 8, 256, LD_REAL_F64, ST_CPLX, 4, 2, 5, 5),    // 200: c2 fast forward columns
+    KB_FIXED(POL_F64, 6, 128, LD_REAL_F64, ST_CPLX, 4, 2, 5, 5),
+    KB_FIXED(POL_F64, 16, 320, LD_REAL_F64, ST_CPLX, 4, 2, 5, 5),
+    KB_FIXED(POL_F64, 8, 256, LD_CPLX, ST_CPLX, 4, 2, 5, 5),        // 200: c2 fast forward rows
+    KB_FIXED(POL_F64, 6, 128, LD_CPLX, ST_CPLX, 4, 2, 5, 5),
+    KB_FIXED(POL_F64, 16, 320, LD_CPLX, ST_CPLX, 4, 2, 5, 5),
     KB_FIXED(POL_F64, 10, 128, LD_CPLX, ST_CPLX, 5, 5, 5),          // 125: c2 fast inverse columns
-    KB_FIXED(POL_F64, 6, 128, LD_CPLX, ST_OUT, 3, 7, 7),            // 147: c2 fast inverse rows
+    KB_FIXED(POL_F64, 5, 128, LD_CPLX, ST_CPLX, 5, 5, 5),
+    KB_FIXED(POL_F64, 9, 256, LD_CPLX, ST_OUT, 3, 7, 7),            // 147: c2 fast inverse rows
+    KB_FIXED(POL_F64, 6, 128, LD_CPLX, ST_OUT, 3, 7, 7),
     KB_FIXED(POL_F64, 6, 128, LD_CPLX, ST_SPLIT, 4, 2, 5, 5),       // 200: c2 fast, fused split
     KB_FIXED(POL_F64, 10, 128, LD_FOLD, ST_CPLX, 5, 5, 5),          // 125: c2 fast, fused fold
 };
@@ -482,9 +508,26 @@ bool has_fixed(int policy, int n, int ld, int st) {
     return false;
 }
 
+// variant index for length n from KBSRC_TUNE="n:k,n:k,..." (0 when absent)
+int tune_index(int n) {
+    const char* v = std::getenv("KBSRC_TUNE");
+    if (!v) return 0;
+    for (const char* p = v; *p;) {
+        char* end = nullptr;
+        const long len = std::strtol(p, &end, 10);
+        if (end == p || *end != ':') break;
+        const long k = std::strtol(end + 1, &end, 10);
+        if (len == n) return (int)k;
+        p = (*end == ',') ? end + 1 : end;
+        if (*end != ',') break;
+    }
+    return 0;
+}
+
 bool launch_fixed(int policy, int ld, int st, PassArgs& a, hipStream_t s, hipError_t* err) {
+    int skip = tune_index(a.sd.n);
     for (const Entry& e : kEntries) {
-        if (e.policy == policy && e.n == a.sd.n && e.ld == ld && e.st == st) {
+        if (e.policy == policy && e.n == a.sd.n && e.ld == ld && e.st == st && skip-- <= 0) {
             a.G = e.G;
             if (ld == LD_FOLD || st == ST_SPLIT) a.ngrp = (a.f.M / 2 + 1 + e.G / 2 - 1) / (e.G / 2);
             else a.ngrp = (a.nsub + e.G - 1) / e.G;
