@@ -84,27 +84,36 @@ namespace {
 constexpr int kThreads = 256;
 constexpr size_t kLdsBudget = 48 * 1024;
 
+constexpr size_t kLdsMax = 160 * 1024;
+
 int pick_group(int policy, int n, int nsub) {
     const size_t one = pass_lds_bytes(policy, n, 1);
-    if (one > 160 * 1024) throw std::runtime_error("sub-FFT too long for the LDS kernel");
     int G = (int)std::max<size_t>(1, kLdsBudget / one);
     return std::max(1, std::min({G, nsub, 16}));
 }
 
-// launch one sub-FFT pass; errors name the pass and its shape
-// Test and tuning switches, read at every execute: KBSRC_GENERIC_PASSES=1 runs every sub-FFT pass through the
-// generic kernel; KBSRC_FUSED=1 folds the split/multiply/fold kernel into the passes next to it (same bits; it
-// saves a kernel and ~1.2 GB of HBM traffic per benchmark step but measured no faster, the passes being
-// FP64-bound).
+// Test and tuning switches. Read at every execute: KBSRC_GENERIC_PASSES=1 runs every sub-FFT pass through the
+// generic kernel; KBSRC_FUSED=1 folds the split/multiply/fold kernel into the passes next to it (same bits; it saves
+// a kernel and ~1.2 GB of HBM traffic per benchmark step but measured no faster, the passes being FP64-bound).
+// Read when a plan is made: KBSRC_GLOBAL_PASSES=1 runs the passes through global memory, one launch per stage, as
+// sub-FFTs too long for an LDS tile always do.
 bool env_flag(const char* name) {
     const char* v = std::getenv(name);
     return v && v[0] == '1';
 }
 bool use_fixed() { return !env_flag("KBSRC_GENERIC_PASSES"); }
 
+// launch one sub-FFT pass; errors name the pass and its shape
 void run_pass(int policy, int ld, int st, PassArgs a, hipStream_t s, const char* what) {
     hipError_t e = hipSuccess;
-    if (!(use_fixed() && launch_fixed(policy, ld, st, a, s, &e))) e = launch_pass(policy, ld, st, a, kThreads, s);
+    const bool forced = a.gscr0 && ld != LD_FOLD && st != ST_SPLIT && env_flag("KBSRC_GLOBAL_PASSES");
+    if (!forced && use_fixed() && launch_fixed(policy, ld, st, a, s, &e)) {
+    } else if (forced || pass_lds_bytes(policy, a.sd.n, 1) > kLdsMax) {
+        if (!a.gscr0) throw std::runtime_error(std::string(what) + ": no global scratch for a long sub-FFT");
+        e = launch_global(policy, ld, st, a, s);
+    } else {
+        e = launch_pass(policy, ld, st, a, kThreads, s);
+    }
     if (e != hipSuccess)
         throw HipError(std::string(what) + " (n=" + std::to_string(a.sd.n) + ", G=" + std::to_string(a.G) +
                        ", LDS " + std::to_string(pass_lds_bytes(policy, a.sd.n, a.G)) + " B): " + hipGetErrorString(e));
@@ -160,6 +169,13 @@ void CFFTPlan::init(int n_, bool as_dd_, const DeviceConsts& k, hipStream_t s) {
         RunCtx c;
         c.ts = ts; c.k = &k; c.dd = true; c.lane = false; c.s = s;
         c.proto.ntask = 1;
+        DevBuf g0, g1;
+        if (vp.needs_global() || env_flag("KBSRC_GLOBAL_PASSES")) {
+            g0.alloc(ts * 8);
+            g1.alloc(ts * 8);
+            c.proto.gscr0 = g0.d();
+            c.proto.gscr1 = g1.d();
+        }
         double* const scr[3] = {s0.d(), s1.d(), s2.d()};
         double tc = 0, tr = 0;
         run_cfft(vp, c, LD_CPLX, src.d(), ST_CPLX, dst.d(), scr, &tc, &tr);
@@ -208,6 +224,13 @@ void RFFTPlan::init(int N_, bool as_dd, const DeviceConsts& k, hipStream_t s) {
     for (int q = 0; q <= h; ++q) ct.get(-q, re[q], im[q]);
     const std::vector<double> v = pack_cx(re, im, as_dd);
     w.upload(v.data(), v.size() * sizeof(double), s);
+}
+
+bool CFFTPlan::needs_global() const {
+    const int pol = as_dd ? POL_DDS : POL_F64;
+    if (blue) return inner->needs_global();
+    if (four) return pass_lds_bytes(pol, N1, 1) > kLdsMax || pass_lds_bytes(pol, N2, 1) > kLdsMax;
+    return pass_lds_bytes(pol, n, 1) > kLdsMax;
 }
 
 namespace {
@@ -291,11 +314,17 @@ std::vector<dd> kernel_spectrum_gpu(const std::vector<dd>& h, int n, double scal
     std::vector<double> rv(2 * (size_t)n);
     for (int i = 0; i < n; ++i) { rv[2 * i] = r[i].hi; rv[2 * i + 1] = r[i].lo; }
     const long long ts = 4LL * std::max<long long>(R.c.work_elems(), R.h + 1);
-    DevBuf din, z(ts * 8), s0(ts * 8), s1(ts * 8), s2(ts * 8), dH((size_t)(R.h + 1) * 16);
+    DevBuf din, z(ts * 8), s0(ts * 8), s1(ts * 8), s2(ts * 8), dH((size_t)(R.h + 1) * 16), g0, g1;
     din.upload(rv.data(), rv.size() * sizeof(double), s);
     RunCtx c;
     c.ts = ts; c.k = &k; c.dd = true; c.lane = false; c.s = s;
     c.proto.ntask = 1;
+    if (R.c.needs_global() || env_flag("KBSRC_GLOBAL_PASSES")) {
+        g0.alloc(ts * 8);
+        g1.alloc(ts * 8);
+        c.proto.gscr0 = g0.d();
+        c.proto.gscr1 = g1.d();
+    }
     double* const scr[3] = {s0.d(), s1.d(), s2.d()};
     double tc = 0, tr = 0;
     run_cfft(R.c, c, LD_REAL_DD, din.d(), ST_CPLX, z.d(), scr, &tc, &tr);
@@ -335,6 +364,10 @@ Engine::Engine(const Plan& p, const std::vector<dd>& H, int Jg, hipStream_t s) :
         if (want > 0) tb_ = (int)std::min<long long>(tb_, want);
     }
     for (int i = 0; i < (blue ? 4 : 2); ++i) w_[i].alloc(per_task * tb_);
+    if (fw_.c.needs_global() || iv_.c.needs_global() || env_flag("KBSRC_GLOBAL_PASSES")) {
+        g_[0].alloc(per_task * tb_);
+        g_[1].alloc(per_task * tb_);
+    }
 }
 
 void Engine::execute(const double* xx, double* out, hipStream_t s, PassTimes* times) {
@@ -345,6 +378,8 @@ void Engine::execute(const double* xx, double* out, hipStream_t s, PassTimes* ti
     RunCtx c;
     c.ts = ts_; c.k = &k_; c.dd = dd_; c.lane = true; c.s = s; c.times = times ? &local : nullptr;
     PassArgs& a = c.proto;
+    a.gscr0 = g_[0].d();
+    a.gscr1 = g_[1].d();
     a.nblk = p_.nblk;
     a.in_cstride = p_.xstride;
     a.in_bstride = (long long)p_.hop * p_.Q;

@@ -72,6 +72,7 @@ struct CFFTPlan {
     DevBuf bc, vf;   // chirp (n) and the inner FFT of the chirp sequence divided by m (m)
     void init(int n, bool as_dd, const DeviceConsts& k, hipStream_t s);
     long long work_elems() const { return blue ? m : n; }   // elements per task a run needs in each buffer
+    bool needs_global() const;   // some sub-FFT too long for an LDS tile: passes go through global scratch
 };
 
 // Real FFT of even length N through a complex FFT of N/2 (src/dsp/fft.hpp:352-358).
@@ -127,7 +128,7 @@ private:
     int tb_ = 1;
     DeviceConsts k_;
     RFFTPlan fw_, iv_;
-    DevBuf H_, w_[4];
+    DevBuf H_, w_[4], g_[2];
 };
 
 }  // namespace kbsrc::gpu

@@ -60,6 +60,8 @@ struct PassArgs {
     double* out = nullptr;
     long long nout = 0, o = 0, margP = 0, hopP = 0;
     FuseArgs f;
+    double* gscr0 = nullptr;        // global scratch for sub-FFTs too long for LDS: ntask * nsub * n elements each
+    double* gscr1 = nullptr;
 };
 
 // Spectrum x H, bin map and inverse pre-split for one batch of tasks (src/fft.hpp:361-409,411-458;
@@ -120,6 +122,8 @@ hipError_t launch_pass(int policy, int ld, int st, const PassArgs& a, int thread
 // Sets a.G and a.ngrp to the specialisation's group size.
 bool launch_fixed(int policy, int ld, int st, PassArgs& a, hipStream_t s, hipError_t* err);
 bool has_fixed(int policy, int n, int ld, int st);
+// Sub-FFT pass through global memory, one launch per stage (lengths whose tile does not fit LDS).
+hipError_t launch_global(int policy, int ld, int st, const PassArgs& a, hipStream_t s);
 hipError_t launch_mid(int policy, const MidArgs& a, hipStream_t s);
 hipError_t launch_stage(const StageArgs& a, hipStream_t s);
 // policy: POL_DDA or POL_F64; kind: the LD kind (pre) or ST kind (post)

@@ -43,6 +43,15 @@ C2_PINS = {"exact_f64": "53dbcdc814aba411c04913eec37e65a59f508a7d7b45496e2289aa1
            "exact_s32": "06c4edb4e089bb988b91a98fe2aa04e6fdce59b4ce6762d7d6e2a97e13ca0d4f"}
 C3_IN = "4d52e1161c75429ba5ae91dd4eca67135a8cf749c957a1166e3a4ff49ed94166"
 C3_EXACT = "6ed5128888063dca0ec7f8254dba14ac4312a008d69fbe9c6daa95940b5de58e"
+# 10 MHz -> 48 kHz mono, --len 0.5 (5,000,000-tap kernel, 20,160,000-point block FFT: 3150 x 3200 four-step), on
+# c5_short_input(): sha256 of the raw float64 output samples. Measured on the compiled reference; the oracle gives
+# the same bits.
+C5_SHORT = {"exact": "cb6ebf41f1876e33961823a63536dbe1576ba05355d50bc5690691b6af408820",
+            "fast": "045b10e3c725eec048444ac74117755bf2507286630753a972785fdf34986c77"}
+
+
+def c5_short_input():
+    return np.random.default_rng(5).uniform(-0.5, 0.5, 200000)
 
 
 def primary97():

@@ -156,7 +156,34 @@ def test_c3_upsampling_full_size_pin():
     assert H.wav_sha_planar(y, 48000) == H.C3_EXACT
 
 
-@pytest.mark.parametrize("env", [{"KBSRC_FUSED": "1"}, {"KBSRC_GENERIC_PASSES": "1"}], ids=["fused", "generic"])
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_long_subfft_extreme_ratio_pin(mode):
+    """10 MHz -> 48 kHz at --len 0.5: the exact engine's 3200-point rows exceed an LDS tile and run as
+    global-memory passes; both modes give the reference's bits."""
+    import hashlib
+    say(f"10M->48k len 0.5 {mode}")
+    y = gpu().resample(H.c5_short_input().reshape(-1, 1), 10000000, 48000, mode, length=0.5)
+    assert y.shape == (1, 960)
+    assert hashlib.sha256(y.tobytes()).hexdigest() == H.C5_SHORT[mode]
+
+
+GLOBAL_CASES = [c for c in CASES if c[3] >= 1000 and c[2] <= 2][::2]
+
+
+@pytest.mark.parametrize("case", GLOBAL_CASES, ids=lambda c: f"{c[0]}-{c[1]}-{c[2]}ch-{c[3]}-{c[4]}")
+def test_global_passes_equal_oracle(case, monkeypatch):
+    """Every sub-FFT pass forced through the global-memory form gives the oracle's bits."""
+    monkeypatch.setenv("KBSRC_GLOBAL_PASSES", "1")
+    rin, rout, ch, frames, mode, length, block, lpc = case
+    say(f"global passes {case}")
+    x = np.random.default_rng(frames + rin + rout).uniform(-0.5, 0.5, frames * ch)
+    want = H.oracle_resample(x, ch, rin, rout, mode, length=length, block=block, lpc=lpc)
+    got = gpu().resample(x.reshape(frames, ch), rin, rout, mode, length=length, block=block, lpc_edges=lpc)
+    assert got.tobytes() == want.tobytes()
+
+
+@pytest.mark.parametrize("env", [{"KBSRC_FUSED": "1"}, {"KBSRC_GENERIC_PASSES": "1"}, {"KBSRC_GLOBAL_PASSES": "1"}],
+                         ids=["fused", "generic", "global"])
 @pytest.mark.parametrize("mode", ["exact", "fast"])
 def test_engine_variants_agree(env, mode, monkeypatch):
     """The fused (split and bin map inside the passes) and the generic-pass engines give the pinned bits too."""
