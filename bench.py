@@ -89,6 +89,108 @@ def block_flops(Nb, Mb, dd, up):
     return {"fwd_col": fc, "fwd_row": fr, "mid": mid, "inv_col": ic, "inv_row": ir + Mb}
 
 
+def block_bytes(Nb, Mb, dd, hop_out):
+    """Compulsory HBM bytes per task of each pass: every pass reads its input and writes its output once. Complex
+    elements are 32 B (dd) or 16 B; the forward pass reads Nb real float64 samples, the last pass writes the hop's
+    hop_out float64 samples. The kernel spectrum H read by the split/multiply/fold is shared by all tasks and is
+    added once per launch by the caller."""
+    elt = 32 if dd else 16
+    h, hi = Nb // 2, Mb // 2
+    return {"fwd_col": 8 * Nb + elt * h, "fwd_row": 2 * elt * h, "mid": elt * (h + hi), "inv_col": 2 * elt * hi,
+            "inv_row": elt * hi + 8 * hop_out}
+
+
+# ---- rocprofv3 summaries ---------------------------------------------------------------------------------------
+PASSES = ("fwd_col", "fwd_row", "mid", "inv_col", "inv_row")
+
+
+def _csv_rows(path):
+    import csv
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def _short_kernel(name):
+    name = name.replace("kbsrc::gpu::", "").replace("(anonymous namespace)::", "").replace("void ", "")
+    return re.sub(r"\(.*\)$", "", name).replace("Radices<", "R<").replace(", ", ",")
+
+
+def _pass_of_dispatches(names):
+    """Engine dispatch order is fwd_col, fwd_row, mid, inv_col, inv_row per execute (unfused, no Bluestein): label
+    each dispatch by its offset from the nearest k_mid."""
+    lab = [None] * len(names)
+    for i, n in enumerate(names):
+        if "k_mid<" in n:
+            for off, p in zip((-2, -1, 0, 1, 2), PASSES):
+                if 0 <= i + off < len(names):
+                    lab[i + off] = p
+    return lab
+
+
+def summarize_rocprof(d, mode):
+    """Reads <d>/stats (--kernel-trace --stats), <d>/fetch (--pmc FETCH_SIZE) and <d>/write (--pmc WRITE_SIZE),
+    all csv, and prints the per-pass summary kept under profiles/ (HBM bytes scaled by what each counter reads for
+    the 256 MiB copy of --profile-copy)."""
+    trace = _csv_rows(os.path.join(d, "stats", "run_kernel_trace.csv"))
+    names = [r["Kernel_Name"] for r in trace]
+    lab = _pass_of_dispatches(names)
+    dur, kname = {p: [] for p in PASSES}, {}
+    for r, p in zip(trace, lab):
+        if p:
+            dur[p].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+            kname[p] = _short_kernel(r["Kernel_Name"])
+    copy_bytes = (1 << 25) * 8
+    cnt, factor = {}, {}
+    for c, sub in (("FETCH_SIZE", "fetch"), ("WRITE_SIZE", "write")):
+        rows = _csv_rows(os.path.join(d, sub, "run_counter_collection.csv"))
+        rows = [r for r in rows if r["Counter_Name"] == c]
+        copies = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]]
+        factor[c] = copy_bytes / 1024 / max(copies)
+        lab_c = _pass_of_dispatches([r["Kernel_Name"] for r in rows])
+        cnt[c] = {p: [float(r["Counter_Value"]) for r, q in zip(rows, lab_c) if q == p] for p in PASSES}
+    out = []
+    out.append(f"# Engine passes, stereo 96000 -> 44100 Hz, 180 s, {mode} mode: MI355X, rocprofv3 (csv output)")
+    out.append("# commands, from /tmp with TMPDIR=/tmp, R = the repository root, D = an output folder:")
+    out.append(f"#   rocprofv3 --kernel-trace --stats -d D/stats -o run --output-format csv -- python3 R/bench.py "
+               f"--mode {mode} --steps 5 --warmup 1 --no-cpu")
+    for c, sub in (("FETCH_SIZE", "fetch"), ("WRITE_SIZE", "write")):
+        out.append(f"#   rocprofv3 --pmc {c} -d D/{sub} -o run --output-format csv -- python3 R/bench.py --mode {mode} "
+                   "--steps 2 --warmup 1 --no-cpu --profile-copy")
+    out.append(f"#   python3 R/bench.py --summarize-rocprof D --mode {mode}")
+    out.append(f"# calibration, 256 MiB device copy: FETCH_SIZE x {factor['FETCH_SIZE']:.3f}, "
+               f"WRITE_SIZE x {factor['WRITE_SIZE']:.3f} (counter KiB -> bytes moved)")
+    out.append(f"# {'pass':8s} {'launches':>8s} {'avg_us':>9s} {'min_us':>9s} {'max_us':>9s} {'fetch_MB':>9s} "
+               f"{'write_MB':>9s} {'GB/s':>8s}  kernel")
+    for p in PASSES:
+        t = dur[p]
+        if not t:
+            continue
+        fb = sum(cnt["FETCH_SIZE"][p]) / len(cnt["FETCH_SIZE"][p]) * 1024 * factor["FETCH_SIZE"]
+        wb = sum(cnt["WRITE_SIZE"][p]) / len(cnt["WRITE_SIZE"][p]) * 1024 * factor["WRITE_SIZE"]
+        avg = sum(t) / len(t)
+        out.append(f"  {p:8s} {len(t):8d} {avg:9.1f} {min(t):9.1f} {max(t):9.1f} {fb / 1e6:9.1f} {wb / 1e6:9.1f} "
+                   f"{(fb + wb) / avg / 1e3:8.0f}  {kname[p]}")
+    for p in PASSES:
+        if cnt["FETCH_SIZE"][p]:
+            fb = sum(cnt["FETCH_SIZE"][p]) / len(cnt["FETCH_SIZE"][p]) * 1024 * factor["FETCH_SIZE"]
+            wb = sum(cnt["WRITE_SIZE"][p]) / len(cnt["WRITE_SIZE"][p]) * 1024 * factor["WRITE_SIZE"]
+            out.append(f"traffic {p} {int(fb + wb)}")
+    print("\n".join(out))
+    return 0
+
+
+def committed_traffic(mode, pass_name):
+    """HBM bytes per launch of a pass from the committed rocprofv3 summary, or None."""
+    path = os.path.join(ROOT, "profiles", f"passes_c2_{mode}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic" and f[1] == pass_name:
+            return int(f[2])
+    return None
+
+
 def cpu_baseline(x, threads, workdir):
     """The reference's engine (oracle/_ref/kbsrc, compiled from its sources) on the same input, on this host:
     its `-v` line reports the resample time without design and file I/O (app/main.cpp:144-145)."""
@@ -123,7 +225,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (calibrates HBM byte counters under rocprofv3 --pmc)")
+    ap.add_argument("--summarize-rocprof", metavar="DIR",
+                    help="print the per-pass summary of rocprofv3 csv output under DIR/{stats,fetch,write} and exit")
     args = ap.parse_args()
+    if args.summarize_rocprof:
+        return summarize_rocprof(args.summarize_rocprof, args.mode)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}): the path shards by block ranges with no "
               "exchange, see DESIGN.md; run with --gpus 1", file=sys.stderr)
@@ -185,7 +291,27 @@ def main():
         tf = fl[k] * tasks / (kt[k] / 1e3) / 1e12 if kt[k] > 0 else 0.0
         passes[k] = {"ms": round(kt[k], 4), "gflop": round(fl[k] * tasks / 1e9, 3), "tflops": round(tf, 2)}
     dom = max(kt, key=kt.get)
-    achieved = fl[dom] * tasks / (kt[dom] / 1e3) / 1e12
+    dd = args.mode == "exact"
+    by = block_bytes(info["Nb"], info["Mb"], dd, info["hop"] * info["P"])
+    for k in kt:
+        nbytes = by[k] * tasks + (k == "mid") * (info["Nb"] // 2 + 1) * (16 if dd else 8)
+        passes[k]["algorithmic_gb"] = round(nbytes / 1e9, 4)
+        passes[k]["gbs"] = round(nbytes / (kt[k] / 1e3) / 1e9, 1) if kt[k] > 0 else 0.0
+    if dd:
+        # dd arithmetic: ~12 flops per compulsory byte in the dominant pass, above the FP64 ridge (78.6 TFLOP/s over
+        # 8 TB/s = 9.8 flop/B): FP64-issue bound. MI355X's vector and matrix FP64 peaks are both 78.6 TFLOP/s.
+        achieved = fl[dom] * tasks / (kt[dom] / 1e3) / 1e12
+        roof = {"bound": "mfma", "pipe": "fp64 (VALU; vector and matrix FP64 peaks are equal)", "kernel": dom,
+                "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP64_PEAK_TFLOPS, 4)}
+    else:
+        # FP64 arithmetic: ~1.4 flops per compulsory byte, far below the ridge: HBM bound
+        achieved = passes[dom]["gbs"]
+        roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4)}
+    roof["algorithmic_per_launch"] = (f"{fl[dom] * tasks / 1e9:.3f} GFLOP" if dd
+                                      else f"{passes[dom]['algorithmic_gb']:.4f} GB")
+    roof["traffic"] = committed_traffic(args.mode, dom)
 
     # end to end (excluding file I/O): output copied back to the host
     torch.cuda.synchronize()
@@ -224,8 +350,7 @@ def main():
         "config": {"workload": "stereo 96000->44100 Hz, 180 s float64 uniform noise, beta 40, len 0.2 s, LPC on, "
                                f"{args.mode} mode", "frames_per_channel": frames, "channels": CH,
                    "nb": info["nb"], "tasks": tasks, "parallelism": "single GPU"},
-        "roofline": {"bound": "fp64", "kernel": dom, "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4), "traffic": None},
+        "roofline": roof,
         "cpu_baseline": cpu,
         "passes": passes,
         "algorithmic_gflop_per_step": round(algo_gflop, 2),

@@ -315,8 +315,10 @@ template <class A> __device__ __forceinline__ Cx<A> pre_v(Cx<A> Y, Cx<A> Yp, Cx<
 }
 
 template <class A> __global__ __launch_bounds__(256) void k_mid(MidArgs a) {
-    const int task = blockIdx.y;
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    // task-fastest workgroup order: the tasks of one bin range run together, so the H and split-twiddle slices
+    // they share stay in L2
+    const int task = (int)(blockIdx.x % (unsigned)a.ntask);
+    const int k = (int)(blockIdx.x / (unsigned)a.ntask) * blockDim.x + threadIdx.x;
     if (k > a.hinv / 2) return;
     const double* z = a.z + task * a.z_tstride;
     double* zi = a.zi + task * a.zi_tstride;
@@ -444,7 +446,9 @@ hipError_t launch_pass(int policy, int ld, int st, const PassArgs& a, int thread
 }
 
 hipError_t launch_mid(int policy, const MidArgs& a, hipStream_t s) {
-    const dim3 grid((a.hinv / 2 + 1 + 255) / 256, a.ntask);
+    const unsigned long long n = (unsigned long long)((a.hinv / 2 + 1 + 255) / 256) * (unsigned long long)a.ntask;
+    if (a.ntask < 1 || n > 0x7fffffffULL) return hipErrorInvalidConfiguration;
+    const dim3 grid((unsigned)n);
     if (policy == POL_F64) hipLaunchKernelGGL(k_mid<F64>, grid, dim3(256), 0, s, a);
     else hipLaunchKernelGGL(k_mid<DDA>, grid, dim3(256), 0, s, a);
     return hipGetLastError();
