@@ -85,6 +85,17 @@ int kbsrc_plan_execute_timed(kbsrc_plan* plan, const double* d_padded, double* d
 
 void kbsrc_plan_destroy(kbsrc_plan* plan);
 
+/* ---- The kernel design on the GPU (host buffers in and out; the plan calls do the same device to device). */
+
+/* kernel_taps (src/design.hpp:13-15, src/design.cpp:17-37): one-sided Kaiser-sinc taps h[0..L], relative cutoff
+ * a/b (L >= 1, a, b >= 1, beta > 0), as (hi, lo) pairs: out[2n] = h[n].hi, out[2n + 1] = h[n].lo, 2 (L + 1)
+ * doubles. */
+int kbsrc_kernel_taps(int L, long long a, long long b, double beta, double* out);
+
+/* kernel_spectrum (src/design.hpp:17-18, src/design.cpp:40-49): taps h[0..Lk] as (hi, lo) pairs, placed
+ * circularly in even length n > Lk; out: the n/2 + 1 values Re X[k] / scale as (hi, lo) pairs. */
+int kbsrc_kernel_spectrum(const double* taps, int Lk, int n, double scale, double* out);
+
 #ifdef __cplusplus
 }
 #endif

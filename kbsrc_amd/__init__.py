@@ -19,7 +19,7 @@ except ImportError:
     pass
 
 __all__ = ["resample", "output_frames", "Plan", "Stats", "version", "library_path", "FAST", "EXACT",
-           "KbsrcError", "PLAN_FIELDS"]
+           "KbsrcError", "PLAN_FIELDS", "kernel_taps", "kernel_spectrum"]
 
 FAST, EXACT, LONG_DOUBLE = 0, 1, 2
 OK, EINVAL, ERUNTIME, ENOMEM = 0, 1, 2, 3
@@ -67,6 +67,8 @@ def _load():
         "kbsrc_plan_execute": (I, [P, P, P, P]),
         "kbsrc_plan_execute_timed": (I, [P, P, P, P, ctypes.POINTER(D)]),
         "kbsrc_plan_destroy": (None, [P]),
+        "kbsrc_kernel_taps": (I, [I, LL, LL, D, P]),
+        "kbsrc_kernel_spectrum": (I, [P, I, I, D, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -128,6 +130,23 @@ def resample(x, rate_in: int, rate_out: int, precision="exact", length=0.2, beta
     _check(_lib.kbsrc_resample(a.ctypes.data if frames else None, frames, ch, rate_in, rate_out, ctypes.byref(o),
                                out.ctypes.data if out.size else None, ctypes.byref(st)))
     return (out, st) if return_stats else out
+
+
+def kernel_taps(L: int, a: int, b: int, beta: float) -> np.ndarray:
+    """The reference's kernel_taps (src/design.cpp:17-37) on the GPU: taps h[0..L] as an (L+1, 2) array of
+    (hi, lo) double-double parts."""
+    out = np.empty((L + 1, 2) if L >= 0 else (1, 2), dtype=np.float64)
+    _check(_lib.kbsrc_kernel_taps(int(L), int(a), int(b), float(beta), out.ctypes.data))
+    return out
+
+
+def kernel_spectrum(taps, n: int, scale: float) -> np.ndarray:
+    """The reference's kernel_spectrum (src/design.cpp:40-49) on the GPU: taps as an (Lk+1, 2) (hi, lo) array,
+    placed circularly in length n; returns Re X[k] / scale for k = 0..n/2 as an (n/2+1, 2) array."""
+    t = np.ascontiguousarray(np.asarray(taps, dtype=np.float64).reshape(-1, 2))
+    out = np.empty((max(n, 0) // 2 + 1, 2), dtype=np.float64)
+    _check(_lib.kbsrc_kernel_spectrum(t.ctypes.data, len(t) - 1, int(n), float(scale), out.ctypes.data))
+    return out
 
 
 class Plan:

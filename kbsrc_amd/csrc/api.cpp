@@ -86,7 +86,6 @@ template <class F> int guarded(F&& f) {
 struct kbsrc_plan {
     Plan p;
     kbsrc_options opt;
-    std::vector<dd> H;
     std::unique_ptr<Engine> eng;
     int Jg = 0;
     double design_seconds = 0;
@@ -94,7 +93,7 @@ struct kbsrc_plan {
 
 namespace {
 
-// Planning and design (src/resampler.cpp:92-167,187-206): taps on the host, spectrum on the GPU.
+// Planning (host) and design (taps and spectrum on the GPU) (src/resampler.cpp:92-167,187-206).
 std::unique_ptr<kbsrc_plan> make(long long frames, int channels, int rate_in, int rate_out, const kbsrc_options& o,
                                  hipStream_t s) {
     check_options(o);
@@ -106,6 +105,8 @@ std::unique_ptr<kbsrc_plan> make(long long frames, int channels, int rate_in, in
     if (p.equal) return pl;
     DeviceConsts k;
     k.init(s);
+    DevBuf H;   // kernel spectrum, (hi, lo) pairs
+    long long nH = 0;
     if (p.gen) {
         // near-unity ratios: the kernel on a finer grid Nk, its true response folded modulo Mb in the engine
         // (src/resampler.cpp:190-202)
@@ -116,15 +117,18 @@ std::unique_ptr<kbsrc_plan> make(long long frames, int channels, int rate_in, in
         const long long g2 = (long long)std::gcd((unsigned long long)p.c2 * (unsigned long long)p.Nb,
                                                  (unsigned long long)Nk * (unsigned long long)p.fi);
         const double fk2 = (double)Nk * p.fi / p.Nb;
-        const std::vector<dd> h2 = kernel_taps((int)std::floor(o.length * fk2), (long long)(A / g2),
-                                               (long long)(B / g2), o.beta);
+        const long long L2 = (long long)(int)std::floor(o.length * fk2);
+        const DevBuf h2 = kernel_taps_gpu(L2, (long long)(A / g2), (long long)(B / g2), o.beta, s);
         pl->Jg = (int)std::min<long long>(Nk / 2, (long long)std::ceil((p.mn / 2.0 + 4 * p.fml) * p.Nb / p.fi));
-        pl->H = kernel_spectrum_gpu(h2, (int)Nk, (double)p.Nb, k, s);
+        H = kernel_spectrum_gpu(h2.d(), L2, (int)Nk, (double)p.Nb, k, s);
+        nH = Nk / 2 + 1;
     } else {
-        const std::vector<dd> h = kernel_taps((int)p.Lk, p.c2 / p.gc, p.fk / p.gc, o.beta);
-        pl->H = kernel_spectrum_gpu(h, p.up ? p.Mb : p.Nb, (double)p.Nb, k, s);
+        const DevBuf h = kernel_taps_gpu(p.Lk, p.c2 / p.gc, p.fk / p.gc, o.beta, s);
+        const int n = p.up ? p.Mb : p.Nb;
+        H = kernel_spectrum_gpu(h.d(), p.Lk, n, (double)p.Nb, k, s);
+        nH = n / 2 + 1;
     }
-    pl->eng = std::make_unique<Engine>(p, pl->H, pl->Jg, s);
+    pl->eng = std::make_unique<Engine>(p, H.d(), nH, pl->Jg, s);   // keeps its own copy of H
     pl->design_seconds = now() - t0;
     return pl;
 }
@@ -284,6 +288,30 @@ int kbsrc_plan_execute_timed(kbsrc_plan* pl, const double* d_padded, double* d_o
 }
 
 void kbsrc_plan_destroy(kbsrc_plan* pl) { delete pl; }
+
+int kbsrc_kernel_taps(int L, long long a, long long b, double beta, double* out) {
+    return guarded([&] {
+        if (L < 1 || a < 1 || b < 1 || !(beta > 0) || !out) throw std::invalid_argument("option out of range");
+        NearestRounding rn(true);
+        const DevBuf h = kernel_taps_gpu(L, a, b, beta, nullptr);
+        hip_check(hipMemcpy(out, h.d(), sizeof(double) * 2 * ((size_t)L + 1), hipMemcpyDeviceToHost), "taps download");
+    });
+}
+
+int kbsrc_kernel_spectrum(const double* taps, int Lk, int n, double scale, double* out) {
+    return guarded([&] {
+        if (!taps || !out || Lk < 0 || n < 2 || (n & 1) || Lk >= n || !(scale != 0))
+            throw std::invalid_argument("option out of range");
+        NearestRounding rn(true);
+        DevBuf h;
+        h.upload(taps, sizeof(double) * 2 * ((size_t)Lk + 1), nullptr);
+        DeviceConsts k;
+        k.init(nullptr);
+        const DevBuf H = kernel_spectrum_gpu(h.d(), Lk, n, scale, k, nullptr);
+        hip_check(hipMemcpy(out, H.d(), sizeof(double) * 2 * ((size_t)n / 2 + 1), hipMemcpyDeviceToHost),
+                  "spectrum download");
+    });
+}
 
 }  // extern "C"
 

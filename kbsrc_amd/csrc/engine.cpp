@@ -306,16 +306,24 @@ void run_cfft(const CFFTPlan& f, const RunCtx& c, int ld, const double* src, int
     }
 }
 
-std::vector<dd> kernel_spectrum_gpu(const std::vector<dd>& h, int n, double scale, const DeviceConsts& k,
-                                    hipStream_t s) {
+DevBuf kernel_taps_gpu(long long L, long long a, long long b, double beta, hipStream_t s) {
+    TapArgs t;
+    t.L = L; t.a = a; t.b = b; t.beta = beta;
+    const dd ratio = dddiv1(mkdd((double)a), (double)b), i0 = bessel_i0(mkdd(beta));
+    t.ratio_hi = ratio.hi; t.ratio_lo = ratio.lo; t.i0_hi = i0.hi; t.i0_lo = i0.lo;
+    DevBuf h((size_t)(L + 1) * 16);
+    t.h = h.d();
+    hip_check(launch_taps(t, s), "kernel taps");
+    return h;
+}
+
+DevBuf kernel_spectrum_gpu(const double* taps, long long Lk, int n, double scale, const DeviceConsts& k,
+                           hipStream_t s) {
     RFFTPlan R;
     R.init(n, true, k, s);
-    const std::vector<dd> r = circular_kernel(h, n);
-    std::vector<double> rv(2 * (size_t)n);
-    for (int i = 0; i < n; ++i) { rv[2 * i] = r[i].hi; rv[2 * i + 1] = r[i].lo; }
     const long long ts = 4LL * std::max<long long>(R.c.work_elems(), R.h + 1);
-    DevBuf din, z(ts * 8), s0(ts * 8), s1(ts * 8), s2(ts * 8), dH((size_t)(R.h + 1) * 16), g0, g1;
-    din.upload(rv.data(), rv.size() * sizeof(double), s);
+    DevBuf din((size_t)n * 16), z(ts * 8), s0(ts * 8), s1(ts * 8), s2(ts * 8), dH((size_t)(R.h + 1) * 16), g0, g1;
+    hip_check(launch_circ(taps, Lk, n, din.d(), s), "circular kernel");
     RunCtx c;
     c.ts = ts; c.k = &k; c.dd = true; c.lane = false; c.s = s;
     c.proto.ntask = 1;
@@ -331,33 +339,29 @@ std::vector<dd> kernel_spectrum_gpu(const std::vector<dd>& h, int n, double scal
     SpecArgs sa;
     sa.z = z.d(); sa.w = R.w.d(); sa.H = dH.d(); sa.h = R.h; sa.scale = scale;
     hip_check(launch_spec_split(sa, s), "spectrum split");
-    std::vector<double> hv(2 * (size_t)(R.h + 1));
-    hip_check(hipMemcpyAsync(hv.data(), dH.d(), hv.size() * sizeof(double), hipMemcpyDeviceToHost, s), "spectrum copy");
-    hip_check(hipStreamSynchronize(s), "spectrum sync");
-    std::vector<dd> H(R.h + 1);
-    for (int i = 0; i <= R.h; ++i) H[i] = dd{hv[2 * i], hv[2 * i + 1]};
-    return H;
+    hip_check(hipStreamSynchronize(s), "spectrum sync");   // the work buffers are freed on return
+    return dH;
 }
 
-Engine::Engine(const Plan& p, const std::vector<dd>& H, int Jg, hipStream_t s) : p_(p), dd_(!p.fast), Jg_(Jg) {
+Engine::Engine(const Plan& p, const double* H, long long nH, int Jg, hipStream_t s) : p_(p), dd_(!p.fast), Jg_(Jg) {
     ntask_ = (long long)p.ch * p.nblk;
     k_.init(s);
     fw_.init(p.Nb, dd_, k_, s);
     iv_.init(p.Mb, dd_, k_, s);
     // H in the element format of the engine: (hi, lo) pairs or FP64
-    std::vector<double> hv(H.size() * (dd_ ? 2 : 1));
-    for (size_t i = 0; i < H.size(); ++i) {
-        if (dd_) { hv[2 * i] = H[i].hi; hv[2 * i + 1] = H[i].lo; }
-        else hv[i] = H[i].hi;
-    }
-    H_.upload(hv.data(), hv.size() * sizeof(double), s);
+    H_.alloc((size_t)nH * (dd_ ? 16 : 8));
+    if (dd_) hip_check(hipMemcpyAsync(H_.d(), H, (size_t)nH * 16, hipMemcpyDeviceToDevice, s), "kernel spectrum copy");
+    else hip_check(launch_hi(H, nH, H_.d(), s), "kernel spectrum hi");
     const size_t elt = dd_ ? 32 : 16;
     const long long elems = std::max({(long long)fw_.h + 1, (long long)iv_.h + 1, fw_.c.work_elems(),
                                       iv_.c.work_elems()});   // + 1: the split's X[h]
     const size_t per_task = (size_t)elems * elt;
     ts_ = (long long)(per_task / sizeof(double));
     const bool blue = fw_.c.blue || iv_.c.blue;
-    const size_t budget = (size_t)1 << 30;
+    // tasks per launch: each work buffer gets up to 1/12 of free device memory, between 1 and 4 GiB
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+    const size_t budget = std::clamp<size_t>(free_b / 12, (size_t)1 << 30, (size_t)4 << 30);
     tb_ = (int)std::max<long long>(1, std::min<long long>(ntask_, (long long)(budget / per_task)));
     if (const char* v = std::getenv("KBSRC_BATCH_TASKS")) {   // tuning knob: tasks per launch
         const long long want = std::atoll(v);
@@ -368,6 +372,7 @@ Engine::Engine(const Plan& p, const std::vector<dd>& H, int Jg, hipStream_t s) :
         g_[0].alloc(per_task * tb_);
         g_[1].alloc(per_task * tb_);
     }
+    hip_check(hipStreamSynchronize(s), "engine set-up");   // H and the tables are in place; the caller may free H
 }
 
 void Engine::execute(const double* xx, double* out, hipStream_t s, PassTimes* times) {

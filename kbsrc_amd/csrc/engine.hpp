@@ -105,15 +105,20 @@ struct RunCtx {
 void run_cfft(const CFFTPlan& f, const RunCtx& c, int ld, const double* src, int st, double* dst,
               double* const scr[3], double* t_col, double* t_row);
 
-// The kernel spectrum H (src/design.cpp:40-49) on the GPU: the circularly placed dd kernel through the passes in
-// the reference's scalar-dd form, then a split that keeps Re X / scale.
-std::vector<dd> kernel_spectrum_gpu(const std::vector<dd>& h, int n, double scale, const DeviceConsts& k,
-                                    hipStream_t s);
+// The Kaiser-sinc taps h[0..L] (src/design.cpp:17-37) on the GPU, as (hi, lo) pairs in device memory.
+DevBuf kernel_taps_gpu(long long L, long long a, long long b, double beta, hipStream_t s);
 
-// The overlap-save engine for one plan (src/resampler.cpp:34-88). Jg > 0 selects the general fold.
+// The kernel spectrum H (src/design.cpp:40-49) on the GPU: device taps h[0..Lk] (pairs) placed circularly in
+// length n, through the passes in the reference's scalar-dd form, then a split that keeps Re X / scale. Returns
+// the n/2 + 1 values of H as (hi, lo) pairs in device memory.
+DevBuf kernel_spectrum_gpu(const double* taps, long long Lk, int n, double scale, const DeviceConsts& k,
+                           hipStream_t s);
+
+// The overlap-save engine for one plan (src/resampler.cpp:34-88). H: device, nH (hi, lo) pairs. Jg > 0 selects
+// the general fold.
 class Engine {
 public:
-    Engine(const Plan& p, const std::vector<dd>& H, int Jg, hipStream_t s);
+    Engine(const Plan& p, const double* H, long long nH, int Jg, hipStream_t s);
     // xx: device, ch x plan.xstride padded planar input. out: device, ch x plan.nout. times: optional per-pass timing.
     void execute(const double* xx, double* out, hipStream_t s, PassTimes* times = nullptr);
     long long tasks() const { return ntask_; }

@@ -2,18 +2,14 @@
 #include "host_design.hpp"
 
 #include <algorithm>
-#include <atomic>
 #include <climits>
 #include <cmath>
 #include <numeric>
 #include <stdexcept>
-#include <thread>
 
 #pragma clang fp contract(off)
 
 namespace kbsrc::gpu {
-
-const dd kPi = {3.141592653589793116e+00, 1.224646799147353207e-16};
 
 bool smooth7(long long n) {
     if (n < 1) return false;
@@ -108,44 +104,7 @@ Plan make_plan(long long frames, int channels, int rate_in, int rate_out, bool f
     return p;
 }
 
-// --- twiddles -----------------------------------------------------------------------------------------------
-static void sincos_taylor(dd x, dd& c, dd& s) {   // x in [0, pi/4]
-    const dd x2 = ddmul(x, x);
-    dd ts = x, sum_s = x, tc = mkdd(1.0), sum_c = mkdd(1.0);
-    for (int k = 1; k < 40; ++k) {
-        tc = dddiv1(ddneg(ddmul(tc, x2)), double((2 * k - 1) * (2 * k)));
-        sum_c = add_acc(sum_c, tc);
-        ts = dddiv1(ddneg(ddmul(ts, x2)), double((2 * k) * (2 * k + 1)));
-        sum_s = add_acc(sum_s, ts);
-        if (std::fabs(ts.hi) < 1e-40 && std::fabs(tc.hi) < 1e-40) break;
-    }
-    c = sum_c;
-    s = sum_s;
-}
-
-void cis(long long k, long long N, dd& c, dd& s) {
-    k %= N;
-    if (k < 0) k += N;
-    // 2 pi k / N = (pi/4) * (8k / N): octant index and exact remainder
-    const long long eight = 8 * k, oct = eight / N, rem = eight - oct * N;
-    const bool mirrored = oct & 1;
-    long long quad = oct / 2;
-    const dd quarter_pi = {kPi.hi / 4, kPi.lo / 4};
-    const dd phi = dddiv1(ddmul1(quarter_pi, double(mirrored ? (N - rem) : rem)), double(N));
-    dd cv, sv;
-    sincos_taylor(phi, cv, sv);
-    if (mirrored) {
-        sv = ddneg(sv);
-        quad += 1;
-    }
-    switch (quad & 3) {
-        case 0: c = cv; s = sv; break;
-        case 1: c = ddneg(sv); s = cv; break;
-        case 2: c = ddneg(cv); s = ddneg(sv); break;
-        default: c = sv; s = ddneg(cv); break;
-    }
-}
-
+// --- twiddles (cis itself is in design_math.hpp, shared with the GPU) ----------------------------------------
 CisTable::CisTable(long long n) : N_(n) {
     B_ = (long long)std::ceil(std::sqrt((double)n));
     const long long J = n / B_ + 1;
@@ -162,67 +121,6 @@ void CisTable::get(long long k, dd& c, dd& s) const {
     if (j == 0) { c = bc_[l]; s = bs_[l]; return; }
     c = sub_acc(ddmul(ac_[j], bc_[l]), ddmul(as_[j], bs_[l]));
     s = add_acc(ddmul(ac_[j], bs_[l]), ddmul(as_[j], bc_[l]));
-}
-
-// --- kernel design ------------------------------------------------------------------------------------------
-dd dd_sqrt(dd a) {   // Newton step from the FP64 root (src/dsp/double_double.hpp:94-98)
-    if (a.hi <= 0) return mkdd(0.0);
-    const double y = std::sqrt(a.hi);
-    const dd r = sub_acc(a, ddmul1(mkdd(y), y));
-    return quick2(y, r.hi / (2 * y));
-}
-
-dd bessel_i0(dd x) {   // sum (x^2/4)^k / (k!)^2 (src/dsp/double_double.hpp:100-107)
-    const dd q = dddiv1(ddmul(x, x), 4.0);
-    dd term = mkdd(1.0), sum = mkdd(1.0);
-    for (int k = 1; k < 2000; ++k) {
-        term = dddiv1(ddmul(term, q), double(k) * double(k));
-        sum = add_acc(sum, term);
-        if (term.hi < sum.hi * 1e-34) break;
-    }
-    return sum;
-}
-
-std::vector<dd> kernel_taps(int L, long long a, long long b, double beta) {
-    std::vector<dd> h(L + 1);
-    const dd i0_beta = bessel_i0(mkdd(beta));
-    const dd ratio = dddiv1(mkdd((double)a), (double)b);
-    auto tap = [&](int n) {
-        dd sinc = mkdd(1.0);
-        if (n) {   // sin(pi n a / b) / (pi n a / b), the angle reduced exactly as (n a mod 2b) / 2b turns
-            const long long m = (long long)(((__int128)n * a) % (2 * (__int128)b));
-            dd c, s;
-            cis(m, 2 * b, c, s);
-            const dd angle = dddiv1(ddmul(kPi, ddmul1(mkdd((double)n), (double)a)), (double)b);
-            sinc = dddiv(s, angle);
-        }
-        const dd u = dddiv1(mkdd((double)n), (double)L);
-        const dd win = dddiv(bessel_i0(ddmul(mkdd(beta), dd_sqrt(sub_acc(mkdd(1.0), ddmul(u, u))))), i0_beta);
-        h[n] = ddmul(ddmul(ratio, sinc), win);
-    };
-    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
-    const int nt = (int)std::min<long long>(hw, std::max(1, (L + 1) / 512));
-    std::atomic<int> next(0);
-    auto work = [&]() {
-        for (int c; (c = next.fetch_add(256)) <= L;)
-            for (int n = c; n <= std::min(L, c + 255); ++n) tap(n);
-    };
-    std::vector<std::thread> pool;
-    for (int i = 1; i < nt; ++i) pool.emplace_back(work);
-    work();
-    for (auto& t : pool) t.join();
-    return h;
-}
-
-std::vector<dd> circular_kernel(const std::vector<dd>& h, int n) {
-    std::vector<dd> r(n, mkdd(0.0));
-    const int Lk = (int)h.size() - 1;
-    r[0] = h[0];
-    for (int i = 1; i <= Lk; ++i) {
-        r[i] = h[i];
-        r[n - i] = h[i];
-    }
-    return r;
 }
 
 // --- Burg edge extension ------------------------------------------------------------------------------------

@@ -1,10 +1,11 @@
-// Host-side planning and design of the resampler: block plan, twiddle values, Kaiser taps, Burg extension.
+// Host-side planning of the resampler: block plan, twiddle tables, Burg extension.
 // Everything here is deterministic IEEE FP64 / double-double work that the reference also does on the host.
 #pragma once
 #include <string>
 #include <vector>
 
 #include "dd.hpp"
+#include "design_math.hpp"   // cis, dd_sqrt, bessel_i0, kaiser_tap (host and device)
 
 namespace kbsrc::gpu {
 
@@ -28,9 +29,6 @@ Plan make_plan(long long frames, int channels, int rate_in, int rate_out, bool f
 bool smooth7(long long n);
 long long next_smooth7(long long n);
 
-// e^{+i 2 pi k / N} in dd, exact octant reduction plus a dd Taylor series (src/dsp/double_double.hpp:113-135)
-void cis(long long k, long long N, dd& c, dd& s);
-
 // cis(2 pi k / N) from two short tables and one dd complex product (src/dsp/fft.hpp:29-43)
 class CisTable {
 public:
@@ -41,17 +39,6 @@ private:
     long long N_ = 0, B_ = 1;
     std::vector<dd> ac_, as_, bc_, bs_;
 };
-
-dd bessel_i0(dd x);
-dd dd_sqrt(dd a);
-extern const dd kPi;
-
-// One-sided Kaiser-windowed sinc taps h[0..L], relative cutoff a/b (src/design.cpp:17-37). Threaded over taps;
-// the result does not depend on the thread count.
-std::vector<dd> kernel_taps(int L, long long a, long long b, double beta);
-
-// Circular placement of the symmetric kernel in length n (src/design.cpp:40-44), as dd reals.
-std::vector<dd> circular_kernel(const std::vector<dd>& h, int n);
 
 // Burg LPC prediction of n samples after x[0..len) (src/lpc.cpp:13-41): order 32 trained on the last 4096
 // samples; zeros if the segment is silent, too short or the prediction unstable.

@@ -116,6 +116,14 @@ struct SpecArgs {
     double scale = 1;
 };
 
+// Kaiser-sinc taps h[0..L] on the GPU (src/design.cpp:17-37): ratio = a/b and i0 = I0(beta) as dd, h as
+// (hi, lo) pairs.
+struct TapArgs {
+    long long L = 0, a = 0, b = 1;
+    double beta = 0, ratio_hi = 0, ratio_lo = 0, i0_hi = 0, i0_lo = 0;
+    double* h = nullptr;
+};
+
 size_t pass_lds_bytes(int policy, int n, int G);
 hipError_t launch_pass(int policy, int ld, int st, const PassArgs& a, int threads, hipStream_t s);
 // Compile-time specialised pass for the benchmark transform lengths; false when (policy, n, ld, st) has none.
@@ -129,5 +137,10 @@ hipError_t launch_stage(const StageArgs& a, hipStream_t s);
 // policy: POL_DDA or POL_F64; kind: the LD kind (pre) or ST kind (post)
 hipError_t launch_blue(int policy, int phase, int kind, const BlueArgs& a, hipStream_t s);
 hipError_t launch_spec_split(const SpecArgs& a, hipStream_t s);
+hipError_t launch_taps(const TapArgs& a, hipStream_t s);
+// circular placement of taps h[0..Lk] (pairs) in length n (src/design.cpp:42-43), pairs out
+hipError_t launch_circ(const double* h, long long Lk, long long n, double* r, hipStream_t s);
+// hi parts of n (hi, lo) pairs
+hipError_t launch_hi(const double* pairs, long long n, double* out, hipStream_t s);
 
 }  // namespace kbsrc::gpu

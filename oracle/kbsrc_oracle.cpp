@@ -454,20 +454,22 @@ template <class S, class Sub> struct RFFT {
 
 // ---------------------------------------------------------------------------------------------------------
 // kernel design (design.cpp:17-49)
+// one tap; ib = I0(beta), ab = a/b
+DD kernel_tap(long long n, int L, long long a, long long b, double beta, DD ib, DD ab) {
+    DD s = mk(1.0);
+    if (n) {   // sin(pi n a/b) / (pi n a/b) with n a reduced mod 2b exactly
+        long long m = (long long)(((__int128)n * a) % (2 * (__int128)b));
+        DD c, sn; cis(m, 2 * b, c, sn);
+        s = div(sn, div1(mul(PI_DD, mul1(mk((double)n), (double)a)), (double)b));
+    }
+    DD u = div1(mk((double)n), (double)L);
+    DD w = div(bessel_i0(mul(mk(beta), sqrt_dd(sub_a(mk(1.0), mul(u, u))))), ib);
+    return mul(mul(ab, s), w);
+}
 std::vector<DD> kernel_taps(int L, long long a, long long b, double beta) {
     std::vector<DD> h(L + 1);
     const DD ib = bessel_i0(mk(beta)), ab = div1(mk((double)a), (double)b);
-    for (int n = 0; n <= L; n++) {
-        DD s = mk(1.0);
-        if (n) {   // sin(pi n a/b) / (pi n a/b) with n a reduced mod 2b exactly
-            long long m = (long long)(((__int128)n * a) % (2 * (__int128)b));
-            DD c, sn; cis(m, 2 * b, c, sn);
-            s = div(sn, div1(mul(PI_DD, mul1(mk((double)n), (double)a)), (double)b));
-        }
-        DD u = div1(mk((double)n), (double)L);
-        DD w = div(bessel_i0(mul(mk(beta), sqrt_dd(sub_a(mk(1.0), mul(u, u))))), ib);
-        h[n] = mul(mul(ab, s), w);
-    }
+    for (int n = 0; n <= L; n++) h[n] = kernel_tap(n, L, a, b, beta, ib, ab);
     return h;
 }
 std::vector<DD> kernel_spectrum(const std::vector<DD>& h, int Nlen, double scale) {
@@ -754,6 +756,19 @@ int oracle_kernel_taps(int L, long long a, long long b, double beta, double* v) 
 }
 
 // kernel spectrum of taps (hi, lo pairs, L+1 of them) in length Nlen, divided by scale: Nlen/2+1 (hi, lo) pairs
+// taps h[idx[i]] only, for spot checks of very long kernels
+int oracle_kernel_taps_at(int L, long long a, long long b, double beta, const long long* idx, int count, double* v) {
+    try {
+        const oracle::DD ib = oracle::bessel_i0(oracle::mk(beta)), ab = oracle::div1(oracle::mk((double)a), (double)b);
+        for (int i = 0; i < count; i++) {
+            if (idx[i] < 0 || idx[i] > L) throw std::invalid_argument("tap index out of range");
+            const oracle::DD t = oracle::kernel_tap(idx[i], L, a, b, beta, ib, ab);
+            v[2 * i] = t.hi; v[2 * i + 1] = t.lo;
+        }
+        return 0;
+    } catch (const std::exception& e) { return fail_with(e, 2); }
+}
+
 int oracle_kernel_spectrum(const double* taps, int L, int Nlen, double scale, double* v) {
     try {
         std::vector<oracle::DD> h(L + 1);

@@ -49,6 +49,14 @@ C3_EXACT = "6ed5128888063dca0ec7f8254dba14ac4312a008d69fbe9c6daa95940b5de58e"
 C5_SHORT = {"exact": "cb6ebf41f1876e33961823a63536dbe1576ba05355d50bc5690691b6af408820",
             "fast": "045b10e3c725eec048444ac74117755bf2507286630753a972785fdf34986c77"}
 
+# The same rates on 30 s (300,000,000 frames of c5_full_input()): 31 blocks exact, and the fast plan's own blocks.
+C5_FULL = {"exact": "41ba6929f40b098befd88dc7036a0b42fd9b04912f91ff13cb043e7367612dc0",
+           "fast": "f673ce48cae1a103858663581d33bbbce6ce4e566fda554955d46769b77c45cf"}
+
+
+def c5_full_input():
+    return np.random.default_rng(5).uniform(-0.5, 0.5, 300_000_000)
+
 
 def c5_short_input():
     return np.random.default_rng(5).uniform(-0.5, 0.5, 200000)
@@ -144,10 +152,32 @@ def oracle():
         lib.oracle_cis_table.argtypes = [LL, LL, LL, P]
         lib.oracle_kernel_taps.argtypes = [I, LL, LL, D, P]
         lib.oracle_kernel_spectrum.argtypes = [P, I, I, D, P]
+        lib.oracle_kernel_taps_at.argtypes = [I, LL, LL, D, P, I, P]
         lib.oracle_burg.argtypes = [P, LL, LL, P]
         lib.oracle_cfft.argtypes = [I, I, P, P]
         _oracle = lib
     return _oracle
+
+
+def oracle_taps(L, a, b, beta, idx=None):
+    """Oracle taps as an (n, 2) array of (hi, lo): all of h[0..L], or h[idx] only."""
+    if idx is None:
+        v = np.empty((L + 1, 2))
+        rc = oracle().oracle_kernel_taps(L, a, b, beta, v.ctypes.data)
+    else:
+        idx = np.ascontiguousarray(idx, dtype=np.int64)
+        v = np.empty((len(idx), 2))
+        rc = oracle().oracle_kernel_taps_at(L, a, b, beta, idx.ctypes.data, len(idx), v.ctypes.data)
+    assert rc == 0, oracle().oracle_last_error()
+    return v
+
+
+def oracle_spectrum(taps, n, scale):
+    t = np.ascontiguousarray(taps, dtype=np.float64).reshape(-1, 2)
+    v = np.empty((n // 2 + 1, 2))
+    rc = oracle().oracle_kernel_spectrum(t.ctypes.data, len(t) - 1, n, scale, v.ctypes.data)
+    assert rc == 0, oracle().oracle_last_error()
+    return v
 
 
 ORACLE_PLAN_FIELDS = ["P", "Q", "nout", "nb", "marg", "Nb", "Mb", "hop", "nblk", "o", "tot", "base", "pre", "E", "L",

@@ -167,6 +167,17 @@ def test_long_subfft_extreme_ratio_pin(mode):
     assert hashlib.sha256(y.tobytes()).hexdigest() == H.C5_SHORT[mode]
 
 
+def test_extreme_ratio_full_length_pins():
+    """10 MHz -> 48 kHz, 30 s mono, --len 0.5 (the 5,000,001-tap design on the GPU), both modes."""
+    import hashlib
+    x = H.c5_full_input().reshape(-1, 1)
+    for mode in ("exact", "fast"):
+        say(f"10M->48k 30 s len 0.5 {mode}")
+        y = gpu().resample(x, 10000000, 48000, mode, length=0.5)
+        assert y.shape == (1, 1440000)
+        assert hashlib.sha256(y.tobytes()).hexdigest() == H.C5_FULL[mode]
+
+
 GLOBAL_CASES = [c for c in CASES if c[3] >= 1000 and c[2] <= 2][::2]
 
 

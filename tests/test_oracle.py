@@ -122,6 +122,14 @@ def test_oracle_kernel_taps_closed_form():
     assert v[0] == float(mpmath.mpf(a) / b) or abs(_dd(v[:2]) - r) < mpmath.mpf(2) ** -104
 
 
+def test_oracle_sampled_taps_equal_full_taps():
+    """The spot-check export (single taps) gives the same bits as the full tap vector."""
+    for L, a, b, beta in [(3000, 11009, 24000, 40.0), (2000, 999999937, 1000000007, 200.0), (7, 2, 3, 5.0)]:
+        full = H.oracle_taps(L, a, b, beta)
+        idx = np.unique(np.concatenate([[0, 1, L], np.random.default_rng(L).integers(0, L + 1, 200)]))
+        assert H.oracle_taps(L, a, b, beta, idx).tobytes() == full[idx].tobytes()
+
+
 @pytest.mark.parametrize("n,kind", [(60, 0), (105, 0), (128, 0), (2100, 0), (2100, 2), (2401, 0), (96, 1),
                                     (2100, 1), (98, 2)])
 def test_oracle_cfft_against_dft(n, kind):
