@@ -75,6 +75,14 @@ int kbsrc_plan_query(const kbsrc_plan* plan, long long* values);
  * channels x xstride doubles (channel c at d_padded + c * xstride), with the Burg edge extensions. stream: a hipStream_t, or NULL for the default. */
 int kbsrc_plan_stage(kbsrc_plan* plan, const double* interleaved, double* d_padded, void* stream);
 
+/* Staging from interleaved input already in device memory (frames x channels doubles): the same result as
+ * kbsrc_plan_stage. Only the edge frames the Burg recursion trains on are read back to the host. Lets a caller
+ * overlap the input upload with kbsrc_plan_create, whose design runs on a non-blocking stream of its own. */
+int kbsrc_plan_stage_device(kbsrc_plan* plan, const double* d_interleaved, double* d_padded, void* stream);
+
+/* kbsrc::Stats of the plan (include/kbsrc/kbsrc.hpp:23-31), process_seconds 0. */
+int kbsrc_plan_stats(const kbsrc_plan* plan, kbsrc_stats* stats);
+
 /* The engine (src/resampler.cpp:34-88): every (channel, block) task, device to device. d_padded as staged;
  * d_out: channels x nout doubles, planar. Asynchronous on stream. */
 int kbsrc_plan_execute(kbsrc_plan* plan, const double* d_padded, double* d_out, void* stream);
@@ -95,6 +103,19 @@ int kbsrc_kernel_taps(int L, long long a, long long b, double beta, double* out)
 /* kernel_spectrum (src/design.hpp:17-18, src/design.cpp:40-49): taps h[0..Lk] as (hi, lo) pairs, placed
  * circularly in even length n > Lk; out: the n/2 + 1 values Re X[k] / scale as (hi, lo) pairs. */
 int kbsrc_kernel_spectrum(const double* taps, int Lk, int n, double scale, double* out);
+
+/* ---- Output packing on the GPU, for streamed WAV writing (src/io/wav.cpp:52-86,235-250, without dither):
+ * frames [frame0, frame0 + frames) of the planar device output (channels x nout doubles) -> interleaved
+ * little-endian sample bytes at d_bytes (device, 8-byte aligned), frames x channels x bytes-per-sample. Integer
+ * formats round to nearest even, clip and add the number of clipped samples to *d_clipped (device); NaN -> 0.
+ * Asynchronous on stream. */
+#define KBSRC_PACK_F64 0
+#define KBSRC_PACK_F32 1
+#define KBSRC_PACK_S32 2
+#define KBSRC_PACK_S24 3
+#define KBSRC_PACK_S16 4
+int kbsrc_pack(const double* d_planar, int channels, long long nout, long long frame0, long long frames, int format,
+               void* d_bytes, unsigned long long* d_clipped, void* stream);
 
 #ifdef __cplusplus
 }

@@ -19,7 +19,7 @@ except ImportError:
     pass
 
 __all__ = ["resample", "output_frames", "Plan", "Stats", "version", "library_path", "FAST", "EXACT",
-           "KbsrcError", "PLAN_FIELDS", "kernel_taps", "kernel_spectrum"]
+           "KbsrcError", "PLAN_FIELDS", "kernel_taps", "kernel_spectrum", "pack", "PACK_FORMATS"]
 
 FAST, EXACT, LONG_DOUBLE = 0, 1, 2
 OK, EINVAL, ERUNTIME, ENOMEM = 0, 1, 2, 3
@@ -67,6 +67,9 @@ def _load():
         "kbsrc_plan_execute": (I, [P, P, P, P]),
         "kbsrc_plan_execute_timed": (I, [P, P, P, P, ctypes.POINTER(D)]),
         "kbsrc_plan_destroy": (None, [P]),
+        "kbsrc_plan_stage_device": (I, [P, P, P, P]),
+        "kbsrc_plan_stats": (I, [P, ctypes.POINTER(Stats)]),
+        "kbsrc_pack": (I, [P, I, LL, LL, LL, I, P, P, P]),
         "kbsrc_kernel_taps": (I, [I, LL, LL, D, P]),
         "kbsrc_kernel_spectrum": (I, [P, I, I, D, P]),
     }
@@ -149,6 +152,16 @@ def kernel_spectrum(taps, n: int, scale: float) -> np.ndarray:
     return out
 
 
+PACK_FORMATS = {"f64": 0, "f32": 1, "s32": 2, "s24": 3, "s16": 4}
+
+
+def pack(d_planar: int, channels: int, nout: int, frame0: int, frames: int, fmt: str, d_bytes: int, d_clipped: int,
+         stream: int = 0):
+    """Output packing on the GPU (WAV sample bytes, interleaved, no dither); device pointers as integers."""
+    _check(_lib.kbsrc_pack(ctypes.c_void_p(d_planar), channels, nout, frame0, frames, PACK_FORMATS[fmt],
+                           ctypes.c_void_p(d_bytes), ctypes.c_void_p(d_clipped), ctypes.c_void_p(stream)))
+
+
 class Plan:
     """The engine seam: plan/design once, then stage and execute on device buffers (integer device pointers,
     e.g. ``torch.Tensor.data_ptr()``), on a HIP stream handle (0 = default stream)."""
@@ -166,6 +179,16 @@ class Plan:
     def stage(self, interleaved, d_padded: int, stream: int = 0):
         a = np.ascontiguousarray(np.asarray(interleaved, dtype=np.float64))
         _check(_lib.kbsrc_plan_stage(self._h, a.ctypes.data, ctypes.c_void_p(d_padded), ctypes.c_void_p(stream)))
+
+    def stage_device(self, d_interleaved: int, d_padded: int, stream: int = 0):
+        """Staging from interleaved input already in device memory (frames x channels float64)."""
+        _check(_lib.kbsrc_plan_stage_device(self._h, ctypes.c_void_p(d_interleaved), ctypes.c_void_p(d_padded),
+                                            ctypes.c_void_p(stream)))
+
+    def stats(self) -> Stats:
+        st = Stats()
+        _check(_lib.kbsrc_plan_stats(self._h, ctypes.byref(st)))
+        return st
 
     def execute(self, d_padded: int, d_out: int, stream: int = 0):
         _check(_lib.kbsrc_plan_execute(self._h, ctypes.c_void_p(d_padded), ctypes.c_void_p(d_out),

@@ -5,6 +5,7 @@
 #include <cmath>
 #include <climits>
 #include <numeric>
+#include <cstdint>
 #include <cstring>
 #include <memory>
 #include <new>
@@ -133,28 +134,54 @@ std::unique_ptr<kbsrc_plan> make(long long frames, int channels, int rate_in, in
     return pl;
 }
 
-void stage(kbsrc_plan* pl, const double* in, double* d_padded, hipStream_t s) {
+// Staging (src/resampler.cpp:168-186) from the interleaved input in device memory (d_in) and its first and last
+// edge_frames() frames on the host: Burg extensions on the host, then deinterleave, pad and place on the GPU.
+void stage_dev(kbsrc_plan* pl, const double* d_in, const double* first, const double* last, double* d_padded,
+               hipStream_t s) {
     const Plan& p = pl->p;
     const long long ext = p.E + 32;
     std::vector<double> head((size_t)p.ch * ext, 0.0), tail((size_t)p.ch * ext, 0.0);
     if (pl->opt.lpc_edges) {
         for (int c = 0; c < p.ch; ++c) {
             std::vector<double> hd, tl;
-            edge_extensions(p, in, c, hd, tl);
+            edge_extensions(p, first, last, c, hd, tl);
             std::memcpy(head.data() + (size_t)c * ext, hd.data(), sizeof(double) * ext);
             std::memcpy(tail.data() + (size_t)c * ext, tl.data(), sizeof(double) * ext);
         }
     }
-    DevBuf din, dh, dt;
-    if (p.nin > 0) din.upload(in, sizeof(double) * (size_t)(p.nin * p.ch), s);
-    else din.alloc(sizeof(double));
+    DevBuf dh, dt;
     dh.upload(head.data(), sizeof(double) * head.size(), s);
     dt.upload(tail.data(), sizeof(double) * tail.size(), s);
     StageArgs a;
-    a.in = din.d(); a.head = dh.d(); a.tail = dt.d(); a.xx = d_padded;
+    a.in = d_in; a.head = dh.d(); a.tail = dt.d(); a.xx = d_padded;
     a.nin = p.nin; a.tot = p.tot; a.stride = p.xstride; a.off = p.base + p.pre; a.ext = ext; a.ch = p.ch;
     hip_check(launch_stage(a, s), "staging kernel");
     hip_check(hipStreamSynchronize(s), "staging sync");
+}
+
+// Staging from host interleaved input: one upload, then as above.
+void stage(kbsrc_plan* pl, const double* in, double* d_padded, hipStream_t s) {
+    const Plan& p = pl->p;
+    DevBuf din;
+    if (p.nin > 0) din.upload(in, sizeof(double) * (size_t)(p.nin * p.ch), s);
+    else din.alloc(sizeof(double));
+    const long long m = edge_frames(p);
+    stage_dev(pl, din.d(), in, in + (p.nin - m) * p.ch, d_padded, s);
+}
+
+// Staging from device interleaved input: the edge frames come down to the host for the Burg recursion.
+void stage_device(kbsrc_plan* pl, const double* d_in, double* d_padded, hipStream_t s) {
+    const Plan& p = pl->p;
+    const long long m = edge_frames(p);
+    std::vector<double> first((size_t)std::max(1LL, m * p.ch)), last((size_t)std::max(1LL, m * p.ch));
+    if (m > 0 && pl->opt.lpc_edges) {
+        const size_t bytes = sizeof(double) * (size_t)(m * p.ch);
+        hip_check(hipMemcpyAsync(first.data(), d_in, bytes, hipMemcpyDeviceToHost, s), "edge download");
+        hip_check(hipMemcpyAsync(last.data(), d_in + (p.nin - m) * p.ch, bytes, hipMemcpyDeviceToHost, s),
+                  "edge download");
+        hip_check(hipStreamSynchronize(s), "edge sync");
+    }
+    stage_dev(pl, d_in, first.data(), last.data(), d_padded, s);
 }
 
 void fill_stats(const kbsrc_plan& pl, double process_seconds, kbsrc_stats* st) {
@@ -246,7 +273,19 @@ int kbsrc_plan_create(long long frames, int channels, int rate_in, int rate_out,
         if (!plan) throw std::invalid_argument("bad input buffer");
         const kbsrc_options o = opt ? *opt : defaults();
         NearestRounding rn(o.precision != KBSRC_FAST);
-        *plan = make(frames, channels, rate_in, rate_out, o, nullptr).release();
+        check_options(o);   // argument errors before any device call
+        (void)make_plan(frames, channels, rate_in, rate_out, o.precision == KBSRC_FAST, o.length, o.beta, o.block);
+        // the design runs on a stream of its own that does not wait for the legacy default stream, so a caller
+        // may copy input on other streams meanwhile
+        hipStream_t s = nullptr;
+        hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream");
+        struct StreamGuard {
+            hipStream_t s;
+            ~StreamGuard() { (void)hipStreamDestroy(s); }
+        } sg{s};
+        auto pl = make(frames, channels, rate_in, rate_out, o, s);
+        hip_check(hipStreamSynchronize(s), "plan sync");
+        *plan = pl.release();
     });
 }
 
@@ -259,6 +298,36 @@ int kbsrc_plan_query(const kbsrc_plan* pl, long long* v) {
                                                 pl->eng ? pl->eng->tasks() : 0, pl->eng ? pl->eng->batch() : 0,
                                                 p.xstride};
         std::memcpy(v, a, sizeof a);
+    });
+}
+
+int kbsrc_plan_stage_device(kbsrc_plan* pl, const double* d_interleaved, double* d_padded, void* stream) {
+    return guarded([&] {
+        if (!pl || !d_padded || (pl->p.nin > 0 && !d_interleaved)) throw std::invalid_argument("bad input buffer");
+        if (!pl->eng) throw std::invalid_argument("plan has no engine (equal rates)");
+        NearestRounding rn(pl->opt.precision != KBSRC_FAST);
+        stage_device(pl, d_interleaved, d_padded, static_cast<hipStream_t>(stream));
+    });
+}
+
+int kbsrc_plan_stats(const kbsrc_plan* pl, kbsrc_stats* st) {
+    return guarded([&] {
+        if (!pl || !st) throw std::invalid_argument("bad input buffer");
+        fill_stats(*pl, 0, st);
+    });
+}
+
+int kbsrc_pack(const double* d_planar, int channels, long long nout, long long frame0, long long frames, int format,
+               void* d_bytes, unsigned long long* d_clipped, void* stream) {
+    return guarded([&] {
+        if (channels < 1 || nout < 0 || frame0 < 0 || frames < 0 || frame0 > nout || frames > nout - frame0 ||
+            pack_bytes(format) == 0 || (frames > 0 && (!d_planar || !d_bytes || !d_clipped)) ||
+            (reinterpret_cast<uintptr_t>(d_bytes) & 7))
+            throw std::invalid_argument("bad input buffer");
+        PackArgs a;
+        a.y = d_planar; a.nout = nout; a.frame0 = frame0; a.frames = frames; a.ch = channels; a.format = format;
+        a.out = static_cast<unsigned char*>(d_bytes); a.clipped = d_clipped;
+        hip_check(launch_pack(a, static_cast<hipStream_t>(stream)), "pack kernel");
     });
 }
 

@@ -188,15 +188,21 @@ void edge_fade(double fd[32]) {
 
 void edge_extensions(const Plan& p, const double* in, int channel, std::vector<double>& head,
                      std::vector<double>& tail) {
-    const long long ext = p.E + 32, nin = p.nin, ch = p.ch;
-    const long long m = std::min<long long>(nin, 4096);
+    const long long m = edge_frames(p);
+    edge_extensions(p, in, in + (p.nin - m) * p.ch, channel, head, tail);
+}
+
+void edge_extensions(const Plan& p, const double* first, const double* last_frames, int channel,
+                     std::vector<double>& head, std::vector<double>& tail) {
+    const long long ext = p.E + 32, ch = p.ch;
+    const long long m = edge_frames(p);
     double fd[32];
     edge_fade(fd);
     // tail: the last m samples predict forward; head: the first m samples reversed predict backward
     std::vector<double> last(m), first_rev(m);
     for (long long i = 0; i < m; ++i) {
-        last[i] = in[(nin - m + i) * ch + channel];
-        first_rev[i] = in[(m - 1 - i) * ch + channel];
+        last[i] = last_frames[i * ch + channel];
+        first_rev[i] = first[(m - 1 - i) * ch + channel];
     }
     tail = burg_extrap(last.data(), m, ext);
     head = burg_extrap(first_rev.data(), m, ext);

@@ -51,5 +51,10 @@ void edge_fade(double fd[32]);
 // head[i] is the sample at position -1-i before the signal; tail[i] the sample at nin + i.
 void edge_extensions(const Plan& p, const double* interleaved, int channel, std::vector<double>& head,
                      std::vector<double>& tail);
+// The same from the first and the last edge_frames(p) interleaved frames only.
+void edge_extensions(const Plan& p, const double* first, const double* last, int channel, std::vector<double>& head,
+                     std::vector<double>& tail);
+// Frames each edge's prediction trains on: min(frames, 4096) (src/lpc.cpp:13-41, src/resampler.cpp:176-185).
+inline long long edge_frames(const Plan& p) { return p.nin < 4096 ? p.nin : 4096; }
 
 }  // namespace kbsrc::gpu

@@ -124,6 +124,18 @@ struct TapArgs {
     double* h = nullptr;
 };
 
+// Output packing (src/io/wav.cpp:52-86,235-250): frames [frame0, frame0 + frames) of the planar output y (channel
+// stride nout) -> interleaved sample bytes at out (bps bytes each, set by the launcher).
+enum PackFormat { PACK_F64 = 0, PACK_F32 = 1, PACK_S32 = 2, PACK_S24 = 3, PACK_S16 = 4 };
+struct PackArgs {
+    const double* y = nullptr;
+    long long nout = 0, frame0 = 0, frames = 0;
+    int ch = 1, format = PACK_F64, bps = 8;
+    unsigned char* out = nullptr;
+    unsigned long long* clipped = nullptr;   // incremented once per clipped sample
+};
+int pack_bytes(int format);   // bytes per sample, 0 for an unknown format
+
 size_t pass_lds_bytes(int policy, int n, int G);
 hipError_t launch_pass(int policy, int ld, int st, const PassArgs& a, int threads, hipStream_t s);
 // Compile-time specialised pass for the benchmark transform lengths; false when (policy, n, ld, st) has none.
@@ -142,5 +154,6 @@ hipError_t launch_taps(const TapArgs& a, hipStream_t s);
 hipError_t launch_circ(const double* h, long long Lk, long long n, double* r, hipStream_t s);
 // hi parts of n (hi, lo) pairs
 hipError_t launch_hi(const double* pairs, long long n, double* out, hipStream_t s);
+hipError_t launch_pack(PackArgs a, hipStream_t s);
 
 }  // namespace kbsrc::gpu

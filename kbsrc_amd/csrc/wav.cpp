@@ -54,7 +54,7 @@ template <int B> void quantise(uint8_t* p, double y, bool dither, std::mt19937_6
 
 }  // namespace
 
-Audio read_wav(const char* path) {
+WavReader::WavReader(const char* path) {
     FileCloser fc{fopen(path, "rb")};
     FILE* f = fc.f;
     if (!f) fail("cannot open input");
@@ -101,25 +101,34 @@ Audio read_wav(const char* path) {
     const int bps = align / ch;
     if (tag == 1 && (bps < 1 || bps > 4 || (bits + 7) / 8 != bps)) fail("unsupported PCM width");
     if (tag == 3 && bits != 8 * bps) fail("IEEE float sample size does not match the block alignment");
-    Audio a;
-    a.rate = (int)rate;
-    a.ch = ch;
-    a.fmt_bits = bits;
-    a.fmt_float = tag == 3;
-    a.frames = data_len / align;
-    const size_t n = (size_t)a.frames * ch;
-    a.d.resize(n);
+    rate_ = (int)rate;
+    ch_ = ch;
+    tag_ = tag;
+    bits_ = bits;
+    bps_ = bps;
+    frames_ = data_len / align;
     fseeko(f, data_pos, SEEK_SET);
-    const size_t chunk = (size_t)1 << 20;
-    std::vector<uint8_t> buf(chunk * bps);
+    buf_.resize(((size_t)1 << 20) * bps);
+    f_ = f;
+    fc.f = nullptr;
+}
+
+WavReader::~WavReader() {
+    if (f_) fclose(static_cast<FILE*>(f_));
+}
+
+void WavReader::read(double* dst, long long frames) {
+    FILE* f = static_cast<FILE*>(f_);
+    if (frames < 0 || frames > frames_ - done_) fail("read error");
+    const size_t n = (size_t)frames * ch_, chunk = (size_t)1 << 20, bps = bps_;
     for (size_t k0 = 0; k0 < n; k0 += chunk) {
         const size_t m = std::min(chunk, n - k0);
-        if (fread(buf.data(), (size_t)bps, m, f) != m) fail("read error");
-        const uint8_t* s = buf.data();
-        double* o = a.d.data() + k0;
-        if (tag == 3 && bits == 64) {
+        if (fread(buf_.data(), bps, m, f) != m) fail("read error");
+        const uint8_t* s = buf_.data();
+        double* o = dst + k0;
+        if (tag_ == 3 && bits_ == 64) {
             memcpy(o, s, 8 * m);
-        } else if (tag == 3) {
+        } else if (tag_ == 3) {
             for (size_t k = 0; k < m; ++k) {
                 float v;
                 memcpy(&v, s + 4 * k, 4);
@@ -143,6 +152,17 @@ Audio read_wav(const char* path) {
             }
         }
     }
+    done_ += frames;
+}
+
+Audio read_wav(const char* path) {
+    WavReader r(path);
+    Audio a;
+    a.rate = r.rate();
+    a.ch = r.channels();
+    a.frames = r.frames();
+    a.d.resize((size_t)a.frames * a.ch);
+    r.read(a.d.data(), a.frames);
     return a;
 }
 
@@ -152,8 +172,7 @@ void check_wav_output(int ch, int rate, const std::string& fmt) {
     if (rate < 1 || rate * frame > 0xFFFFFFFFLL) fail("sample rate too high for a WAV file with this many channels");
 }
 
-void write_wav(const char* path, const std::vector<std::vector<double>>& y, int ch, long long frames, int rate,
-               const std::string& fmt, bool dither, long long& clipped) {
+WavWriter::WavWriter(const char* path, int ch, long long frames, int rate, const std::string& fmt) {
     check_wav_output(ch, rate, fmt);
     const bool is_float = fmt == "f64" || fmt == "f32";
     const int bits = fmt == "f64" ? 64 : fmt == "f32" ? 32 : fmt == "s32" ? 32 : fmt == "s24" ? 24 : 16;
@@ -182,13 +201,38 @@ void write_wav(const char* path, const std::vector<std::vector<double>>& y, int 
     if (is_float) { tag4("fact"); put32(4); put32((uint32_t)frames); }
     tag4("data"); put32((uint32_t)data_bytes);
     FileCloser fc{fopen(path, "wb")};
-    FILE* f = fc.f;
-    if (!f) fail("cannot open output");
-    if (fwrite(h.data(), 1, h.size(), f) != h.size()) fail("write error");
+    if (!fc.f) fail("cannot open output");
+    if (fwrite(h.data(), 1, h.size(), fc.f) != h.size()) fail("write error");
+    align_ = (size_t)align;
+    data_bytes_ = data_bytes;
+    f_ = fc.f;
+    fc.f = nullptr;
+}
+
+WavWriter::~WavWriter() {
+    if (f_) fclose(static_cast<FILE*>(f_));
+}
+
+void WavWriter::write(const void* bytes, size_t n) {
+    if (n && fwrite(bytes, 1, n, static_cast<FILE*>(f_)) != n) fail("write error");
+}
+
+void WavWriter::finish() {
+    FILE* f = static_cast<FILE*>(f_);
+    if (data_bytes_ & 1) fputc(0, f);
+    f_ = nullptr;
+    if (fclose(f)) fail("write error");
+}
+
+void write_wav(const char* path, const std::vector<std::vector<double>>& y, int ch, long long frames, int rate,
+               const std::string& fmt, bool dither, long long& clipped) {
+    WavWriter w(path, ch, frames, rate, fmt);
+    const size_t align = w.frame_bytes();
+    const int bps = (int)(align / ch);
     std::mt19937_64 rng(12345);
     std::uniform_real_distribution<double> U(-0.5, 0.5);
     clipped = 0;
-    const long long chunk = std::max(1, std::min(1 << 16, (4 << 20) / align));
+    const long long chunk = std::max(1, std::min(1 << 16, (int)((4 << 20) / align)));
     std::vector<uint8_t> buf((size_t)chunk * align);
     for (long long f0 = 0; f0 < frames; f0 += chunk) {
         const long long nf = std::min(chunk, frames - f0);
@@ -202,12 +246,9 @@ void write_wav(const char* path, const std::vector<std::vector<double>>& y, int 
                 else if (bps == 3) quantise<3>(p, x, dither, rng, U, clipped);
                 else quantise<4>(p, x, dither, rng, U, clipped);
             }
-        const size_t nbytes = (size_t)(p - buf.data());
-        if (fwrite(buf.data(), 1, nbytes, f) != nbytes) fail("write error");
+        w.write(buf.data(), (size_t)(p - buf.data()));
     }
-    if (data_bytes & 1) fputc(0, f);
-    fc.f = nullptr;
-    if (fclose(f)) fail("write error");
+    w.finish();
 }
 
 }  // namespace kbsrc::io

@@ -19,6 +19,44 @@ struct Audio {
 // Throws std::runtime_error with the reference's messages.
 Audio read_wav(const char* path);
 
+// The same decoder, streamed: the constructor parses the header (same checks and messages), read() decodes the
+// next frames into interleaved f64 in chunks of 1 Mi samples (src/io/wav.cpp:150-170).
+class WavReader {
+public:
+    explicit WavReader(const char* path);
+    ~WavReader();
+    WavReader(const WavReader&) = delete;
+    WavReader& operator=(const WavReader&) = delete;
+    int rate() const { return rate_; }
+    int channels() const { return ch_; }
+    long long frames() const { return frames_; }
+    void read(double* dst, long long frames);
+
+private:
+    void* f_ = nullptr;
+    int rate_ = 0, ch_ = 0, tag_ = 0, bits_ = 0, bps_ = 0;
+    long long frames_ = 0, done_ = 0;
+    std::vector<unsigned char> buf_;
+};
+
+// Streamed writing: the header for frames x ch samples in this format, then data written by the caller.
+// The constructor checks the format and writes the header; write() appends data; finish() pads an odd data
+// length and closes the file.
+class WavWriter {
+public:
+    WavWriter(const char* path, int ch, long long frames, int rate, const std::string& format);
+    ~WavWriter();
+    WavWriter(const WavWriter&) = delete;
+    WavWriter& operator=(const WavWriter&) = delete;
+    void write(const void* bytes, size_t n);
+    void finish();
+    size_t frame_bytes() const { return align_; }
+
+private:
+    void* f_ = nullptr;
+    size_t align_ = 0, data_bytes_ = 0;
+};
+
 // Fails (std::runtime_error) when a header in this format cannot describe ch channels at rate.
 void check_wav_output(int ch, int rate, const std::string& format);
 

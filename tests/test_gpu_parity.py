@@ -135,16 +135,18 @@ def test_c2_full_size_pins(mode):
 
 
 def test_c2_quantised_outputs_bit_exact(tmp_path):
-    """Undithered s24/s16/s32 through the CLI (host quantisation of the GPU output) match the reference's bytes."""
-    say("c2 CLI s24/s16/s32")
+    """The benchmark workload through the streamed CLI (decode and upload behind the GPU design, output packed and
+    quantised on the GPU, no dither): f64, s24, s16 and s32 files match the reference's bytes."""
+    say("c2 CLI f64/s24/s16/s32")
     src = tmp_path / "c2.wav"
     with open(src, "wb") as f:
         f.write(H.wav_f64(H.c2_input(), 2, 96000))
-    for fmt in ("s24", "s16", "s32"):
+    for fmt in ("f64", "s24", "s16", "s32"):
         out = tmp_path / f"c2_{fmt}.wav"
-        r = subprocess.run([H.GPU_CLI, "--rate", "44100", "--format", fmt, str(src), str(out)], capture_output=True,
-                           text=True, timeout=110)
+        r = subprocess.run([H.GPU_CLI, "--rate", "44100", "--format", fmt, "-v", str(src), str(out)],
+                           capture_output=True, text=True, timeout=110)
         assert r.returncode == 0, r.stderr
+        say("  " + fmt + ": " + r.stderr.strip().splitlines()[-1])
         assert H.sha(out.read_bytes()) == H.C2_PINS[f"exact_{fmt}"], fmt
         os.remove(out)
 
