@@ -50,6 +50,13 @@ KB_HD void cis(long long k, long long N, dd& c, dd& s) {
     }
 }
 
+// cis(2 pi (jB + l) / N) from the table values cis(2 pi jB / N) = (ac, as) and cis(2 pi l / N) = (bc, bs): one dd
+// complex product (src/dsp/fft.hpp:36-42)
+KB_HD void cis_compose(dd ac, dd as, dd bc, dd bs, dd& c, dd& s) {
+    c = sub_acc(ddmul(ac, bc), ddmul(as, bs));
+    s = add_acc(ddmul(ac, bs), ddmul(as, bc));
+}
+
 // sqrt of a dd: one Newton step from the FP64 root (src/dsp/double_double.hpp:94-98)
 KB_HD dd dd_sqrt(dd a) {
     if (a.hi <= 0) return mkdd(0.0);

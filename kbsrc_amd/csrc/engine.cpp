@@ -130,6 +130,26 @@ void fill_sub(PassArgs& a, const SubFFT& sf, const DeviceConsts& k, bool as_dd, 
 
 }  // namespace
 
+namespace {
+
+// Long twiddle table of CisTable(N) on the GPU: the two short tables go up, every entry is composed there.
+void device_cis_table(DevBuf& out, long long N, long long count, int mode, long long N2, bool as_dd, hipStream_t s) {
+    CisTable ct(N);
+    std::vector<double> a, b;
+    ct.export_tables(a, b);
+    DevBuf da, db;
+    da.upload(a.data(), a.size() * sizeof(double), s);
+    db.upload(b.data(), b.size() * sizeof(double), s);
+    out.alloc((size_t)count * (as_dd ? 32 : 16));
+    TableArgs t;
+    t.a = da.d(); t.b = db.d(); t.N = N; t.B = ct.block(); t.count = count; t.N2 = N2; t.mode = mode;
+    t.as_dd = as_dd; t.out = out.d();
+    hip_check(launch_cis_table(t, s), "twiddle table");
+    hip_check(hipStreamSynchronize(s), "twiddle table");   // da, db are freed on return
+}
+
+}  // namespace
+
 void DeviceConsts::init(hipStream_t s) {
     std::vector<dd> re(21, mkdd(0.0)), im(21, mkdd(0.0));
     const int rr[3] = {3, 5, 7};
@@ -148,24 +168,15 @@ void CFFTPlan::init(int n_, bool as_dd_, const DeviceConsts& k, hipStream_t s) {
         m = (int)next_smooth7(2LL * n - 1);
         inner = std::make_unique<CFFTPlan>();
         inner->init(m, as_dd, k, s);
-        std::vector<dd> cr(n), ci(n), vr(m, mkdd(0.0)), vi(m, mkdd(0.0));
-        for (long long t = 0; t < n; ++t) {
-            const long long e = (long long)((__int128)t * t % (2LL * n));
-            cis(e, 2LL * n, cr[t], ci[t]);
-            vr[t] = cr[t];
-            vi[t] = ci[t];
-            if (t) { vr[m - t] = cr[t]; vi[m - t] = ci[t]; }
-        }
-        const std::vector<double> bv = pack_cx(cr, ci, as_dd);
-        bc.upload(bv.data(), bv.size() * sizeof(double), s);
-        // the chirp sequence's spectrum in the reference's scalar-dd form, divided by m (src/dsp/fft.hpp:236-238)
+        // the chirp and the chirp sequence v (zero between n and m - n) on the GPU
         CFFTPlan vp;
         vp.init(m, true, k, s);
-        const std::vector<double> vv = pack_cx(vr, vi, true);
         const long long ts = 4LL * vp.work_elems();
-        DevBuf src, dst, s0(ts * 8), s1(ts * 8), s2(ts * 8);
-        src.upload(vv.data(), vv.size() * sizeof(double), s);
-        dst.alloc(ts * 8);
+        DevBuf src(ts * 8), dst(ts * 8), s0(ts * 8), s1(ts * 8), s2(ts * 8);
+        bc.alloc((size_t)n * (as_dd ? 32 : 16));
+        hip_check(hipMemsetAsync(src.d(), 0, (size_t)m * 32, s), "chirp clear");
+        hip_check(launch_chirp(n, m, as_dd, bc.d(), src.d(), s), "chirp");
+        // the chirp sequence's spectrum in the reference's scalar-dd form, divided by m (src/dsp/fft.hpp:236-238)
         RunCtx c;
         c.ts = ts; c.k = &k; c.dd = true; c.lane = false; c.s = s;
         c.proto.ntask = 1;
@@ -179,16 +190,9 @@ void CFFTPlan::init(int n_, bool as_dd_, const DeviceConsts& k, hipStream_t s) {
         double* const scr[3] = {s0.d(), s1.d(), s2.d()};
         double tc = 0, tr = 0;
         run_cfft(vp, c, LD_CPLX, src.d(), ST_CPLX, dst.d(), scr, &tc, &tr);
-        std::vector<double> V(4 * (size_t)m);
-        hip_check(hipMemcpyAsync(V.data(), dst.d(), V.size() * sizeof(double), hipMemcpyDeviceToHost, s), "chirp copy");
-        hip_check(hipStreamSynchronize(s), "chirp sync");
-        std::vector<dd> fr(m), fi(m);
-        for (int q = 0; q < m; ++q) {
-            fr[q] = dddiv1(dd{V[4 * q], V[4 * q + 2]}, (double)m);
-            fi[q] = dddiv1(dd{V[4 * q + 1], V[4 * q + 3]}, (double)m);
-        }
-        const std::vector<double> fv = pack_cx(fr, fi, as_dd);
-        vf.upload(fv.data(), fv.size() * sizeof(double), s);
+        vf.alloc((size_t)m * (as_dd ? 32 : 16));
+        hip_check(launch_div_m(dst.d(), m, as_dd, vf.d(), s), "chirp spectrum");
+        hip_check(hipStreamSynchronize(s), "chirp sync");   // the work buffers are freed on return
         return;
     }
     four = n > 2048;
@@ -206,12 +210,7 @@ void CFFTPlan::init(int n_, bool as_dd_, const DeviceConsts& k, hipStream_t s) {
     }
     s1.init(N1, as_dd, s);
     s2.init(N2, as_dd, s);
-    CisTable ct(n);
-    std::vector<dd> re((size_t)n), im((size_t)n);
-    for (int k1 = 0; k1 < N1; ++k1)
-        for (int n2 = 0; n2 < N2; ++n2) ct.get(-(long long)k1 * n2, re[(size_t)k1 * N2 + n2], im[(size_t)k1 * N2 + n2]);
-    const std::vector<double> v = pack_cx(re, im, as_dd);
-    tw4.upload(v.data(), v.size() * sizeof(double), s);
+    device_cis_table(tw4, n, n, TAB_FOUR, N2, as_dd, s);   // w_n^{-k1 n2} at k1*N2 + n2
 }
 
 void RFFTPlan::init(int N_, bool as_dd, const DeviceConsts& k, hipStream_t s) {
@@ -219,11 +218,7 @@ void RFFTPlan::init(int N_, bool as_dd, const DeviceConsts& k, hipStream_t s) {
     h = N / 2;
     if (N % 2) throw std::runtime_error("RFFT length must be even");
     c.init(h, as_dd, k, s);
-    CisTable ct(N);
-    std::vector<dd> re(h + 1), im(h + 1);
-    for (int q = 0; q <= h; ++q) ct.get(-q, re[q], im[q]);
-    const std::vector<double> v = pack_cx(re, im, as_dd);
-    w.upload(v.data(), v.size() * sizeof(double), s);
+    device_cis_table(w, N, (long long)h + 1, TAB_LINEAR, 1, as_dd, s);   // w[q] = e^{-2 pi i q / N}
 }
 
 bool CFFTPlan::needs_global() const {

@@ -136,6 +136,19 @@ struct PackArgs {
 };
 int pack_bytes(int format);   // bytes per sample, 0 for an unknown format
 
+// Twiddle tables on the GPU from a CisTable's two short tables (a: cis(jB, N), b: cis(l, N), 4 doubles each):
+// mode TAB_FOUR: element k1*N2 + n2 = cis(-k1 n2, N) for count = N entries (the four-step table); mode TAB_LINEAR:
+// element i = cis(-i, N), i < count (the real-FFT split table). Elements in the plan's format (as_dd: {re.hi,
+// im.hi, re.lo, im.lo}, else {re, im}).
+enum TableMode { TAB_FOUR = 0, TAB_LINEAR = 1 };
+struct TableArgs {
+    const double* a = nullptr;
+    const double* b = nullptr;
+    long long N = 1, B = 1, count = 0, N2 = 1;
+    int mode = TAB_LINEAR, as_dd = 1;
+    double* out = nullptr;
+};
+
 size_t pass_lds_bytes(int policy, int n, int G);
 hipError_t launch_pass(int policy, int ld, int st, const PassArgs& a, int threads, hipStream_t s);
 // Compile-time specialised pass for the benchmark transform lengths; false when (policy, n, ld, st) has none.
@@ -155,5 +168,11 @@ hipError_t launch_circ(const double* h, long long Lk, long long n, double* r, hi
 // hi parts of n (hi, lo) pairs
 hipError_t launch_hi(const double* pairs, long long n, double* out, hipStream_t s);
 hipError_t launch_pack(PackArgs a, hipStream_t s);
+hipError_t launch_cis_table(const TableArgs& a, hipStream_t s);
+// Bluestein chirp b[t] = cis(t^2 mod 2n, 2n), t < n, into bc (plan format) and the chirp sequence v (dd format,
+// m elements, zeroed by the caller): v[t] = v[m - t] = b[t] (src/dsp/fft.hpp:231-236)
+hipError_t launch_chirp(long long n, long long m, int as_dd, double* bc, double* v, hipStream_t s);
+// out[q] = V[q] / m per dd component (V dd format, m elements), out in the plan's format (src/dsp/fft.hpp:237-238)
+hipError_t launch_div_m(const double* V, long long m, int as_dd, double* out, hipStream_t s);
 
 }  // namespace kbsrc::gpu
