@@ -93,6 +93,34 @@ int kbsrc_plan_execute_timed(kbsrc_plan* plan, const double* d_padded, double* d
 
 void kbsrc_plan_destroy(kbsrc_plan* plan);
 
+/* ---- Batches: many jobs (files, channel counts, rate pairs, lengths) in one device input buffer and one device
+ * output buffer, with one set of options. Each job's output is bit-identical to its own kbsrc_resample. Jobs whose
+ * plans share a shape run as one launch per pass over all their (job, channel, block) tasks; no counterpart in the
+ * reference (its callers loop over files). */
+typedef struct kbsrc_job {
+    long long frames;
+    int channels, rate_in, rate_out;
+} kbsrc_job;
+typedef struct kbsrc_batch kbsrc_batch;
+
+/* Plans every job (the reference's rejections, src/resampler.cpp:94-155) and designs one kernel per shape. */
+int kbsrc_batch_create(int njobs, const kbsrc_job* jobs, const kbsrc_options* opt, kbsrc_batch** batch);
+
+/* job >= 0: in_offset, in_doubles, out_offset, out_doubles, nout, group (-1 for an equal-rate copy); offsets in
+ * doubles into the batch's device buffers; job's output is planar, channel c at out_offset + c * nout.
+ * job == -1: 0, input buffer doubles, 0, output buffer doubles, jobs, groups. */
+#define KBSRC_BATCH_FIELDS 6
+int kbsrc_batch_query(const kbsrc_batch* batch, int job, long long* values);
+
+/* Stages job's interleaved input (host, or device memory when on_device) into the batch input buffer d_in. */
+int kbsrc_batch_stage(kbsrc_batch* batch, int job, const double* interleaved, int on_device, double* d_in,
+                      void* stream);
+
+/* Every job, device to device, asynchronous on stream (the groups fan out over the batch's own streams). */
+int kbsrc_batch_execute(kbsrc_batch* batch, const double* d_in, double* d_out, void* stream);
+
+void kbsrc_batch_destroy(kbsrc_batch* batch);
+
 /* ---- The kernel design on the GPU (host buffers in and out; the plan calls do the same device to device). */
 
 /* kernel_taps (src/design.hpp:13-15, src/design.cpp:17-37): one-sided Kaiser-sinc taps h[0..L], relative cutoff

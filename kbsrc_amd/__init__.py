@@ -19,7 +19,8 @@ except ImportError:
     pass
 
 __all__ = ["resample", "output_frames", "Plan", "Stats", "version", "library_path", "FAST", "EXACT",
-           "KbsrcError", "PLAN_FIELDS", "kernel_taps", "kernel_spectrum", "pack", "PACK_FORMATS"]
+           "KbsrcError", "PLAN_FIELDS", "kernel_taps", "kernel_spectrum", "pack", "PACK_FORMATS", "Batch",
+           "resample_batch"]
 
 FAST, EXACT, LONG_DOUBLE = 0, 1, 2
 OK, EINVAL, ERUNTIME, ENOMEM = 0, 1, 2, 3
@@ -70,6 +71,11 @@ def _load():
         "kbsrc_plan_stage_device": (I, [P, P, P, P]),
         "kbsrc_plan_stats": (I, [P, ctypes.POINTER(Stats)]),
         "kbsrc_pack": (I, [P, I, LL, LL, LL, I, P, P, P]),
+        "kbsrc_batch_create": (I, [I, P, ctypes.POINTER(_Options), ctypes.POINTER(P)]),
+        "kbsrc_batch_query": (I, [P, I, ctypes.POINTER(LL)]),
+        "kbsrc_batch_stage": (I, [P, I, P, I, P, P]),
+        "kbsrc_batch_execute": (I, [P, P, P, P]),
+        "kbsrc_batch_destroy": (None, [P]),
         "kbsrc_kernel_taps": (I, [I, LL, LL, D, P]),
         "kbsrc_kernel_spectrum": (I, [P, I, I, D, P]),
     }
@@ -160,6 +166,79 @@ def pack(d_planar: int, channels: int, nout: int, frame0: int, frames: int, fmt:
     """Output packing on the GPU (WAV sample bytes, interleaved, no dither); device pointers as integers."""
     _check(_lib.kbsrc_pack(ctypes.c_void_p(d_planar), channels, nout, frame0, frames, PACK_FORMATS[fmt],
                            ctypes.c_void_p(d_bytes), ctypes.c_void_p(d_clipped), ctypes.c_void_p(stream)))
+
+
+class _Job(ctypes.Structure):
+    _fields_ = [("frames", ctypes.c_longlong), ("channels", ctypes.c_int), ("rate_in", ctypes.c_int),
+                ("rate_out", ctypes.c_int)]
+
+
+class Batch:
+    """Many jobs in one pair of device buffers (kbsrc_batch_*): jobs = [(frames, channels, rate_in, rate_out)],
+    one set of options. ``layout[j]`` = dict(in_offset, in_doubles, out_offset, out_doubles, nout, group) in
+    doubles; ``in_doubles`` / ``out_doubles`` the buffer sizes."""
+
+    def __init__(self, jobs, precision="exact", length=0.2, beta=40.0, block=0, lpc_edges=True):
+        arr = (_Job * len(jobs))(*[_Job(int(f), int(c), int(a), int(b)) for f, c, a, b in jobs])
+        self._h = ctypes.c_void_p()
+        o = _options(precision, length, beta, block, lpc_edges)
+        _check(_lib.kbsrc_batch_create(len(jobs), arr, ctypes.byref(o), ctypes.byref(self._h)))
+        v = (ctypes.c_longlong * 6)()
+        _check(_lib.kbsrc_batch_query(self._h, -1, v))
+        self.in_doubles, self.out_doubles, self.groups = v[1], v[3], v[5]
+        self.layout = []
+        keys = ["in_offset", "in_doubles", "out_offset", "out_doubles", "nout", "group"]
+        for j in range(len(jobs)):
+            _check(_lib.kbsrc_batch_query(self._h, j, v))
+            self.layout.append(dict(zip(keys, list(v))))
+
+    def stage(self, job: int, interleaved, d_in: int, stream: int = 0):
+        """Host interleaved (frames, channels) float64 input of one job into the batch input buffer."""
+        a = np.ascontiguousarray(np.asarray(interleaved, dtype=np.float64))
+        _check(_lib.kbsrc_batch_stage(self._h, job, a.ctypes.data if a.size else None, 0, ctypes.c_void_p(d_in),
+                                      ctypes.c_void_p(stream)))
+
+    def stage_device(self, job: int, d_interleaved: int, d_in: int, stream: int = 0):
+        _check(_lib.kbsrc_batch_stage(self._h, job, ctypes.c_void_p(d_interleaved), 1, ctypes.c_void_p(d_in),
+                                      ctypes.c_void_p(stream)))
+
+    def execute(self, d_in: int, d_out: int, stream: int = 0):
+        _check(_lib.kbsrc_batch_execute(self._h, ctypes.c_void_p(d_in), ctypes.c_void_p(d_out),
+                                        ctypes.c_void_p(stream)))
+
+    def close(self):
+        if self._h:
+            _lib.kbsrc_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def resample_batch(signals, rates_in, rates_out, precision="exact", length=0.2, beta=40.0, block=0,
+                   lpc_edges=True):
+    """Resample several signals ((frames, channels) float64 each) in one batch on the GPU; returns a list of
+    (channels, nout) arrays. Device buffers come from torch."""
+    import torch
+    xs = [np.asarray(x, dtype=np.float64) for x in signals]
+    xs = [x[:, None] if x.ndim == 1 else x for x in xs]
+    b = Batch([(x.shape[0], x.shape[1], ri, ro) for x, ri, ro in zip(xs, rates_in, rates_out)], precision, length,
+              beta, block, lpc_edges)
+    d_in = torch.zeros(max(1, b.in_doubles), dtype=torch.float64, device="cuda")
+    d_out = torch.zeros(max(1, b.out_doubles), dtype=torch.float64, device="cuda")
+    for j, x in enumerate(xs):
+        b.stage(j, x, d_in.data_ptr())
+    b.execute(d_in.data_ptr(), d_out.data_ptr())
+    torch.cuda.synchronize()
+    host = d_out.cpu().numpy()
+    out = []
+    for x, L in zip(xs, b.layout):
+        out.append(host[L["out_offset"]:L["out_offset"] + L["out_doubles"]].reshape(x.shape[1], L["nout"]).copy())
+    b.close()
+    return out
 
 
 class Plan:

@@ -92,7 +92,69 @@ struct kbsrc_plan {
     double design_seconds = 0;
 };
 
+// Many jobs (files, channel counts, rate pairs, lengths) in one set of device buffers. Jobs whose plans share a
+// shape (rates and block size, hence transform lengths, kernel spectrum and tables) form a group: one engine, one
+// launch per pass over all the group's (job, channel, block) tasks through a task table. Groups run concurrently
+// on a few streams. Equal-rate jobs are copies.
+struct kbsrc_batch {
+    struct Job {
+        Plan p;
+        long long in_off = 0, in_len = 0, out_off = 0, out_len = 0;
+        int group = -1;
+    };
+    struct Group {
+        Plan p;
+        std::vector<int> jobs;
+        long long ntask = 0;
+        DevBuf tdesc;
+        std::unique_ptr<Engine> eng;
+    };
+    kbsrc_options opt;
+    std::vector<Job> jobs;
+    std::vector<Group> groups;
+    long long in_total = 0, out_total = 0;
+    double design_seconds = 0;
+    std::vector<hipStream_t> streams;
+    std::vector<hipEvent_t> events;   // one per stream, plus the fork event
+    ~kbsrc_batch() {
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        for (hipStream_t q : streams) (void)hipStreamDestroy(q);
+    }
+};
+
 namespace {
+
+// The kernel design for a plan on the GPU (src/resampler.cpp:187-206): taps, then the spectrum H as (hi, lo) pairs.
+struct Design {
+    DevBuf H;
+    long long nH = 0;
+    int Jg = 0;
+};
+Design design_kernel(const Plan& p, const kbsrc_options& o, const DeviceConsts& k, hipStream_t s) {
+    Design d;
+    if (p.gen) {
+        // near-unity ratios: the kernel on a finer grid Nk, its true response folded modulo Mb in the engine
+        // (src/resampler.cpp:190-202)
+        long long Nk = (long long)std::max(p.Nb, p.Mb) * 5 / 4;
+        Nk += Nk & 1;
+        while (!smooth7(Nk / 2)) Nk += 2;
+        const __int128 A = (__int128)p.c2 * p.Nb, B = (__int128)Nk * p.fi;
+        const long long g2 = (long long)std::gcd((unsigned long long)p.c2 * (unsigned long long)p.Nb,
+                                                 (unsigned long long)Nk * (unsigned long long)p.fi);
+        const double fk2 = (double)Nk * p.fi / p.Nb;
+        const long long L2 = (long long)(int)std::floor(o.length * fk2);
+        const DevBuf h2 = kernel_taps_gpu(L2, (long long)(A / g2), (long long)(B / g2), o.beta, s);
+        d.Jg = (int)std::min<long long>(Nk / 2, (long long)std::ceil((p.mn / 2.0 + 4 * p.fml) * p.Nb / p.fi));
+        d.H = kernel_spectrum_gpu(h2.d(), L2, (int)Nk, (double)p.Nb, k, s);
+        d.nH = Nk / 2 + 1;
+    } else {
+        const DevBuf h = kernel_taps_gpu(p.Lk, p.c2 / p.gc, p.fk / p.gc, o.beta, s);
+        const int n = p.up ? p.Mb : p.Nb;
+        d.H = kernel_spectrum_gpu(h.d(), p.Lk, n, (double)p.Nb, k, s);
+        d.nH = n / 2 + 1;
+    }
+    return d;
+}
 
 // Planning (host) and design (taps and spectrum on the GPU) (src/resampler.cpp:92-167,187-206).
 std::unique_ptr<kbsrc_plan> make(long long frames, int channels, int rate_in, int rate_out, const kbsrc_options& o,
@@ -106,42 +168,20 @@ std::unique_ptr<kbsrc_plan> make(long long frames, int channels, int rate_in, in
     if (p.equal) return pl;
     DeviceConsts k;
     k.init(s);
-    DevBuf H;   // kernel spectrum, (hi, lo) pairs
-    long long nH = 0;
-    if (p.gen) {
-        // near-unity ratios: the kernel on a finer grid Nk, its true response folded modulo Mb in the engine
-        // (src/resampler.cpp:190-202)
-        long long Nk = (long long)std::max(p.Nb, p.Mb) * 5 / 4;
-        Nk += Nk & 1;
-        while (!smooth7(Nk / 2)) Nk += 2;
-        const __int128 A = (__int128)p.c2 * p.Nb, B = (__int128)Nk * p.fi;
-        const long long g2 = (long long)std::gcd((unsigned long long)p.c2 * (unsigned long long)p.Nb,
-                                                 (unsigned long long)Nk * (unsigned long long)p.fi);
-        const double fk2 = (double)Nk * p.fi / p.Nb;
-        const long long L2 = (long long)(int)std::floor(o.length * fk2);
-        const DevBuf h2 = kernel_taps_gpu(L2, (long long)(A / g2), (long long)(B / g2), o.beta, s);
-        pl->Jg = (int)std::min<long long>(Nk / 2, (long long)std::ceil((p.mn / 2.0 + 4 * p.fml) * p.Nb / p.fi));
-        H = kernel_spectrum_gpu(h2.d(), L2, (int)Nk, (double)p.Nb, k, s);
-        nH = Nk / 2 + 1;
-    } else {
-        const DevBuf h = kernel_taps_gpu(p.Lk, p.c2 / p.gc, p.fk / p.gc, o.beta, s);
-        const int n = p.up ? p.Mb : p.Nb;
-        H = kernel_spectrum_gpu(h.d(), p.Lk, n, (double)p.Nb, k, s);
-        nH = n / 2 + 1;
-    }
-    pl->eng = std::make_unique<Engine>(p, H.d(), nH, pl->Jg, s);   // keeps its own copy of H
+    const Design d = design_kernel(p, o, k, s);
+    pl->Jg = d.Jg;
+    pl->eng = std::make_unique<Engine>(p, d.H.d(), d.nH, d.Jg, s);   // keeps its own copy of H
     pl->design_seconds = now() - t0;
     return pl;
 }
 
 // Staging (src/resampler.cpp:168-186) from the interleaved input in device memory (d_in) and its first and last
 // edge_frames() frames on the host: Burg extensions on the host, then deinterleave, pad and place on the GPU.
-void stage_dev(kbsrc_plan* pl, const double* d_in, const double* first, const double* last, double* d_padded,
+void stage_dev(const Plan& p, bool lpc, const double* d_in, const double* first, const double* last, double* d_padded,
                hipStream_t s) {
-    const Plan& p = pl->p;
     const long long ext = p.E + 32;
     std::vector<double> head((size_t)p.ch * ext, 0.0), tail((size_t)p.ch * ext, 0.0);
-    if (pl->opt.lpc_edges) {
+    if (lpc) {
         for (int c = 0; c < p.ch; ++c) {
             std::vector<double> hd, tl;
             edge_extensions(p, first, last, c, hd, tl);
@@ -160,28 +200,26 @@ void stage_dev(kbsrc_plan* pl, const double* d_in, const double* first, const do
 }
 
 // Staging from host interleaved input: one upload, then as above.
-void stage(kbsrc_plan* pl, const double* in, double* d_padded, hipStream_t s) {
-    const Plan& p = pl->p;
+void stage(const Plan& p, bool lpc, const double* in, double* d_padded, hipStream_t s) {
     DevBuf din;
     if (p.nin > 0) din.upload(in, sizeof(double) * (size_t)(p.nin * p.ch), s);
     else din.alloc(sizeof(double));
     const long long m = edge_frames(p);
-    stage_dev(pl, din.d(), in, in + (p.nin - m) * p.ch, d_padded, s);
+    stage_dev(p, lpc, din.d(), in, in + (p.nin - m) * p.ch, d_padded, s);
 }
 
 // Staging from device interleaved input: the edge frames come down to the host for the Burg recursion.
-void stage_device(kbsrc_plan* pl, const double* d_in, double* d_padded, hipStream_t s) {
-    const Plan& p = pl->p;
+void stage_device(const Plan& p, bool lpc, const double* d_in, double* d_padded, hipStream_t s) {
     const long long m = edge_frames(p);
     std::vector<double> first((size_t)std::max(1LL, m * p.ch)), last((size_t)std::max(1LL, m * p.ch));
-    if (m > 0 && pl->opt.lpc_edges) {
+    if (m > 0 && lpc) {
         const size_t bytes = sizeof(double) * (size_t)(m * p.ch);
         hip_check(hipMemcpyAsync(first.data(), d_in, bytes, hipMemcpyDeviceToHost, s), "edge download");
         hip_check(hipMemcpyAsync(last.data(), d_in + (p.nin - m) * p.ch, bytes, hipMemcpyDeviceToHost, s),
                   "edge download");
         hip_check(hipStreamSynchronize(s), "edge sync");
     }
-    stage_dev(pl, d_in, first.data(), last.data(), d_padded, s);
+    stage_dev(p, lpc, d_in, first.data(), last.data(), d_padded, s);
 }
 
 void fill_stats(const kbsrc_plan& pl, double process_seconds, kbsrc_stats* st) {
@@ -231,7 +269,7 @@ void resample_host(const double* in, long long frames, int channels, int rate_in
     const Plan& p = pl->p;
     const double t1 = now();
     DevBuf xx((size_t)p.ch * p.xstride * sizeof(double)), dout((size_t)std::max(1LL, p.ch * p.nout) * sizeof(double));
-    stage(pl.get(), in, xx.d(), s);
+    stage(p, o.lpc_edges, in, xx.d(), s);
     pl->design_seconds += now() - t1;
     const double t2 = now();
     pl->eng->execute(xx.d(), dout.d(), s);
@@ -306,7 +344,7 @@ int kbsrc_plan_stage_device(kbsrc_plan* pl, const double* d_interleaved, double*
         if (!pl || !d_padded || (pl->p.nin > 0 && !d_interleaved)) throw std::invalid_argument("bad input buffer");
         if (!pl->eng) throw std::invalid_argument("plan has no engine (equal rates)");
         NearestRounding rn(pl->opt.precision != KBSRC_FAST);
-        stage_device(pl, d_interleaved, d_padded, static_cast<hipStream_t>(stream));
+        stage_device(pl->p, pl->opt.lpc_edges, d_interleaved, d_padded, static_cast<hipStream_t>(stream));
     });
 }
 
@@ -336,7 +374,7 @@ int kbsrc_plan_stage(kbsrc_plan* pl, const double* in, double* d_padded, void* s
         if (!pl || !d_padded || (pl->p.nin > 0 && !in)) throw std::invalid_argument("bad input buffer");
         if (!pl->eng) throw std::invalid_argument("plan has no engine (equal rates)");
         NearestRounding rn(pl->opt.precision != KBSRC_FAST);
-        stage(pl, in, d_padded, static_cast<hipStream_t>(stream));
+        stage(pl->p, pl->opt.lpc_edges, in, d_padded, static_cast<hipStream_t>(stream));
     });
 }
 
@@ -357,6 +395,152 @@ int kbsrc_plan_execute_timed(kbsrc_plan* pl, const double* d_padded, double* d_o
 }
 
 void kbsrc_plan_destroy(kbsrc_plan* pl) { delete pl; }
+
+int kbsrc_batch_create(int njobs, const kbsrc_job* jobs, const kbsrc_options* opt, kbsrc_batch** batch) {
+    return guarded([&] {
+        if (njobs < 1 || !jobs || !batch) throw std::invalid_argument("bad input buffer");
+        const kbsrc_options o = opt ? *opt : defaults();
+        check_options(o);
+        NearestRounding rn(o.precision != KBSRC_FAST);
+        const double t0 = now();
+        auto b = std::make_unique<kbsrc_batch>();
+        b->opt = o;
+        b->jobs.resize(njobs);
+        auto align = [](long long v) { return (v + 31) / 32 * 32; };
+        std::vector<long long> keys;   // (rate_in, rate_out, nb) of each group
+        for (int j = 0; j < njobs; ++j) {
+            const kbsrc_job& J = jobs[j];
+            if (J.channels < 1 || J.frames < 0) throw std::invalid_argument("bad input buffer");
+            kbsrc_batch::Job& job = b->jobs[j];
+            job.p = make_plan(J.frames, J.channels, J.rate_in, J.rate_out, o.precision == KBSRC_FAST, o.length, o.beta,
+                              o.block);
+            const Plan& p = job.p;
+            job.in_off = b->in_total;
+            job.in_len = p.equal ? p.nin * p.ch : p.ch * p.xstride;
+            b->in_total += align(job.in_len);
+            job.out_off = b->out_total;
+            job.out_len = p.ch * p.nout;
+            b->out_total += align(job.out_len);
+            if (p.equal) continue;
+            int g = 0;
+            for (; g < (int)b->groups.size(); ++g) {
+                const Plan& q = b->groups[g].p;
+                if (q.fi == p.fi && q.fo == p.fo && q.nb == p.nb) break;
+            }
+            if (g == (int)b->groups.size()) {
+                b->groups.emplace_back();
+                b->groups.back().p = p;
+            }
+            job.group = g;
+            b->groups[g].jobs.push_back(j);
+            b->groups[g].ntask += (long long)p.ch * p.nblk;
+        }
+        const int nstreams = (int)std::min<size_t>(4, std::max<size_t>(1, b->groups.size()));
+        for (int i = 0; i < nstreams; ++i) {
+            hipStream_t q = nullptr;
+            hip_check(hipStreamCreateWithFlags(&q, hipStreamNonBlocking), "stream");
+            b->streams.push_back(q);
+        }
+        for (int i = 0; i <= nstreams; ++i) {
+            hipEvent_t e = nullptr;
+            hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+            b->events.push_back(e);
+        }
+        hipStream_t s = b->streams[0];
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+        const size_t per_group = std::clamp<size_t>(free_b / (16 * std::max<size_t>(1, b->groups.size())),
+                                                    (size_t)256 << 20, (size_t)4 << 30);
+        DeviceConsts k;
+        k.init(s);
+        for (kbsrc_batch::Group& G : b->groups) {
+            // task table: (input offset of the block, output offset of the channel, block, channel length)
+            std::vector<long long> t;
+            t.reserve(4 * G.ntask);
+            for (int j : G.jobs) {
+                const kbsrc_batch::Job& job = b->jobs[j];
+                const Plan& p = job.p;
+                for (int c = 0; c < p.ch; ++c)
+                    for (long long blk = 0; blk < p.nblk; ++blk) {
+                        t.push_back(job.in_off + c * p.xstride + blk * (long long)p.hop * p.Q);
+                        t.push_back(job.out_off + c * p.nout);
+                        t.push_back(blk);
+                        t.push_back(p.nout);
+                    }
+            }
+            G.tdesc.upload(t.data(), t.size() * sizeof(long long), s);
+            const Design d = design_kernel(G.p, o, k, s);
+            G.eng = std::make_unique<Engine>(G.p, d.H.d(), d.nH, d.Jg, s, G.ntask,
+                                             reinterpret_cast<const long long*>(G.tdesc.d()), per_group);
+        }
+        b->design_seconds = now() - t0;
+        *batch = b.release();
+    });
+}
+
+int kbsrc_batch_query(const kbsrc_batch* b, int job, long long* v) {
+    return guarded([&] {
+        if (!b || !v || job < -1 || job >= (int)b->jobs.size()) throw std::invalid_argument("bad input buffer");
+        if (job < 0) {
+            v[0] = 0; v[1] = b->in_total; v[2] = 0; v[3] = b->out_total; v[4] = (long long)b->jobs.size();
+            v[5] = (long long)b->groups.size();
+            return;
+        }
+        const kbsrc_batch::Job& J = b->jobs[job];
+        v[0] = J.in_off; v[1] = J.in_len; v[2] = J.out_off; v[3] = J.out_len; v[4] = J.p.nout; v[5] = J.group;
+    });
+}
+
+int kbsrc_batch_stage(kbsrc_batch* b, int job, const double* interleaved, int on_device, double* d_in, void* stream) {
+    return guarded([&] {
+        if (!b || !d_in || job < 0 || job >= (int)b->jobs.size()) throw std::invalid_argument("bad input buffer");
+        const kbsrc_batch::Job& J = b->jobs[job];
+        const Plan& p = J.p;
+        if (p.nin > 0 && !interleaved) throw std::invalid_argument("bad input buffer");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        NearestRounding rn(b->opt.precision != KBSRC_FAST);
+        if (p.equal) {   // a planar copy: channel c at in_off + c * frames
+            DevBuf tmp;
+            const double* src = interleaved;
+            if (!on_device && p.nin > 0) {
+                tmp.upload(interleaved, sizeof(double) * (size_t)(p.nin * p.ch), s);
+                src = tmp.d();
+            }
+            if (p.nin > 0) {
+                StageArgs a;
+                a.in = src; a.head = src; a.tail = src; a.xx = d_in + J.in_off;
+                a.nin = p.nin; a.tot = p.nin; a.stride = p.nin; a.off = 0; a.ext = 0; a.ch = p.ch;
+                hip_check(launch_stage(a, s), "staging kernel");
+                hip_check(hipStreamSynchronize(s), "staging sync");
+            }
+            return;
+        }
+        if (on_device) stage_device(p, b->opt.lpc_edges, interleaved, d_in + J.in_off, s);
+        else stage(p, b->opt.lpc_edges, interleaved, d_in + J.in_off, s);
+    });
+}
+
+int kbsrc_batch_execute(kbsrc_batch* b, const double* d_in, double* d_out, void* stream) {
+    return guarded([&] {
+        if (!b || !d_in || !d_out) throw std::invalid_argument("bad input buffer");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        const int ns = (int)b->streams.size();
+        hip_check(hipEventRecord(b->events[ns], s), "batch fork");
+        for (int i = 0; i < ns; ++i) hip_check(hipStreamWaitEvent(b->streams[i], b->events[ns], 0), "batch fork");
+        for (size_t g = 0; g < b->groups.size(); ++g)   // groups round-robin over the batch's streams
+            b->groups[g].eng->execute(d_in, d_out, b->streams[g % ns]);
+        for (const kbsrc_batch::Job& J : b->jobs)
+            if (J.p.equal && J.out_len > 0)
+                hip_check(hipMemcpyAsync(d_out + J.out_off, d_in + J.in_off, sizeof(double) * J.out_len,
+                                         hipMemcpyDeviceToDevice, b->streams[0]), "copy");
+        for (int i = 0; i < ns; ++i) {
+            hip_check(hipEventRecord(b->events[i], b->streams[i]), "batch join");
+            hip_check(hipStreamWaitEvent(s, b->events[i], 0), "batch join");
+        }
+    });
+}
+
+void kbsrc_batch_destroy(kbsrc_batch* b) { delete b; }
 
 int kbsrc_kernel_taps(int L, long long a, long long b, double beta, double* out) {
     return guarded([&] {

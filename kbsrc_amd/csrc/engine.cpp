@@ -262,7 +262,7 @@ void run_cfft(const CFFTPlan& f, const RunCtx& c, int ld, const double* src, int
     if (f.blue) {   // X_k = conj(b_k) sum_j x_j conj(b_j) b_{k-j}, through the inner FFT twice (src/dsp/fft.hpp:268-278)
         BlueArgs b;
         b.n = f.n; b.m = f.m; b.ntask = c.proto.ntask; b.task0 = c.proto.task0; b.nblk = c.proto.nblk;
-        b.in_cstride = c.proto.in_cstride; b.in_bstride = c.proto.in_bstride;
+        b.in_cstride = c.proto.in_cstride; b.in_bstride = c.proto.in_bstride; b.tdesc = c.proto.tdesc;
         b.bc = f.bc.d(); b.vf = f.vf.d();
         b.out = c.proto.out; b.nout = c.proto.nout; b.o = c.proto.o; b.margP = c.proto.margP; b.hopP = c.proto.hopP;
         const int pol = pol_one;
@@ -338,8 +338,10 @@ DevBuf kernel_spectrum_gpu(const double* taps, long long Lk, int n, double scale
     return dH;
 }
 
-Engine::Engine(const Plan& p, const double* H, long long nH, int Jg, hipStream_t s) : p_(p), dd_(!p.fast), Jg_(Jg) {
-    ntask_ = (long long)p.ch * p.nblk;
+Engine::Engine(const Plan& p, const double* H, long long nH, int Jg, hipStream_t s, long long ntask,
+               const long long* tdesc, size_t buffer_budget)
+    : p_(p), dd_(!p.fast), Jg_(Jg), tdesc_(tdesc) {
+    ntask_ = ntask > 0 ? ntask : (long long)p.ch * p.nblk;
     k_.init(s);
     fw_.init(p.Nb, dd_, k_, s);
     iv_.init(p.Mb, dd_, k_, s);
@@ -353,10 +355,12 @@ Engine::Engine(const Plan& p, const double* H, long long nH, int Jg, hipStream_t
     const size_t per_task = (size_t)elems * elt;
     ts_ = (long long)(per_task / sizeof(double));
     const bool blue = fw_.c.blue || iv_.c.blue;
-    // tasks per launch: each work buffer gets up to 1/12 of free device memory, between 1 and 4 GiB
+    // tasks per launch: each work buffer gets up to 1/12 of free device memory, between 1 and 4 GiB (or the
+    // caller's budget, for engines that share the device with others)
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
-    const size_t budget = std::clamp<size_t>(free_b / 12, (size_t)1 << 30, (size_t)4 << 30);
+    const size_t budget = buffer_budget ? buffer_budget
+                                        : std::clamp<size_t>(free_b / 12, (size_t)1 << 30, (size_t)4 << 30);
     tb_ = (int)std::max<long long>(1, std::min<long long>(ntask_, (long long)(budget / per_task)));
     if (const char* v = std::getenv("KBSRC_BATCH_TASKS")) {   // tuning knob: tasks per launch
         const long long want = std::atoll(v);
@@ -380,6 +384,7 @@ void Engine::execute(const double* xx, double* out, hipStream_t s, PassTimes* ti
     PassArgs& a = c.proto;
     a.gscr0 = g_[0].d();
     a.gscr1 = g_[1].d();
+    a.tdesc = tdesc_;
     a.nblk = p_.nblk;
     a.in_cstride = p_.xstride;
     a.in_bstride = (long long)p_.hop * p_.Q;

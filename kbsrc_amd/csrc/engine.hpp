@@ -118,7 +118,11 @@ DevBuf kernel_spectrum_gpu(const double* taps, long long Lk, int n, double scale
 // the general fold.
 class Engine {
 public:
-    Engine(const Plan& p, const double* H, long long nH, int Jg, hipStream_t s);
+    // ntask > 0 with tdesc (device, 4 values per task, see task_io): the engine runs that many tasks of several
+    // jobs that share p's shape (a batch) instead of p's own channels x blocks. buffer_budget > 0: bytes per work
+    // buffer (tasks per launch follow from it).
+    Engine(const Plan& p, const double* H, long long nH, int Jg, hipStream_t s, long long ntask = 0,
+           const long long* tdesc = nullptr, size_t buffer_budget = 0);
     // xx: device, ch x plan.xstride padded planar input. out: device, ch x plan.nout. times: optional per-pass timing.
     void execute(const double* xx, double* out, hipStream_t s, PassTimes* times = nullptr);
     long long tasks() const { return ntask_; }
@@ -130,6 +134,7 @@ private:
     bool dd_;
     int Jg_ = 0;
     long long ntask_ = 0, ts_ = 0;   // ts_: doubles per task in the work buffers
+    const long long* tdesc_ = nullptr;
     int tb_ = 1;
     DeviceConsts k_;
     RFFTPlan fw_, iv_;

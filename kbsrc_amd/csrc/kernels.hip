@@ -124,8 +124,7 @@ __global__ __launch_bounds__(256) void k_subfft(PassArgs a) {
     const int n = a.sd.n;
     const int cap = n * Gs;
     Planes<A> cur{lds, cap}, nxt{lds + kPlanes<A> * cap, cap};
-    const long long gtask = a.task0 + task;
-    const long long ch = gtask / a.nblk, blk = gtask - ch * a.nblk;
+    const TaskIO io = task_io(a.tdesc, a.task0 + task, a.nblk, a.in_cstride, a.in_bstride, a.nout);
 
     // ---- load G sub-FFTs into LDS
     {
@@ -135,7 +134,7 @@ __global__ __launch_bounds__(256) void k_subfft(PassArgs a) {
             const long long i = e * a.lde + (long long)(j0 + g) * a.ldg;
             Cx<A> v;
             if constexpr (LD == LD_REAL_F64) {
-                const double* s = a.src + ch * a.in_cstride + blk * a.in_bstride;
+                const double* s = a.src + io.in_off;
                 v = {A::lift(s[2 * i]), A::lift(s[2 * i + 1])};
             } else if constexpr (LD == LD_REAL_DD) {
                 const double* s = a.src + task * a.src_tstride;
@@ -208,16 +207,16 @@ __global__ __launch_bounds__(256) void k_subfft(PassArgs a) {
                 gstore<A>(a.dst + task * a.dst_tstride, i, v);
             } else {
                 // r[2i] = re, r[2i+1] = -im (src/dsp/fft.hpp:460); keep the block's hop (src/resampler.cpp:80-84)
-                double* o = a.out + ch * a.nout;
+                double* o = a.out + io.out_off;
                 const long long x0 = 2 * i - a.margP;
                 if (x0 >= 0 && x0 < a.hopP) {
-                    const long long d = blk * a.hopP + x0 - a.o;
-                    if (d >= 0 && d < a.nout) o[d] = to_f64<A>(v.re);
+                    const long long d = io.blk * a.hopP + x0 - a.o;
+                    if (d >= 0 && d < io.nout) o[d] = to_f64<A>(v.re);
                 }
                 const long long x1 = x0 + 1;
                 if (x1 >= 0 && x1 < a.hopP) {
-                    const long long d = blk * a.hopP + x1 - a.o;
-                    if (d >= 0 && d < a.nout) o[d] = to_f64<A>(A::neg(v.im));
+                    const long long d = io.blk * a.hopP + x1 - a.o;
+                    if (d >= 0 && d < io.nout) o[d] = to_f64<A>(A::neg(v.im));
                 }
             }
         }
@@ -367,8 +366,8 @@ template <class A, int LD> __global__ __launch_bounds__(256) void k_blue_pre(Blu
     if (j < a.n) {
         Cx<A> x;
         if constexpr (LD == LD_REAL_F64) {
-            const long long gtask = a.task0 + task, ch = gtask / a.nblk, blk = gtask - ch * a.nblk;
-            const double* s = a.src + ch * a.in_cstride + blk * a.in_bstride;
+            const TaskIO io = task_io(a.tdesc, a.task0 + task, a.nblk, a.in_cstride, a.in_bstride, a.nout);
+            const double* s = a.src + io.in_off;
             x = {A::lift(s[2 * j]), A::lift(s[2 * j + 1])};
         } else if constexpr (LD == LD_REAL_DD) {
             const double* s = a.src + task * a.src_tstride;
@@ -395,16 +394,16 @@ template <class A, int ST> __global__ __launch_bounds__(256) void k_blue_post(Bl
     if constexpr (ST == ST_CPLX) {
         gstore<A>(a.dst + task * a.dst_tstride, k, y);
     } else {
-        const long long gtask = a.task0 + task, ch = gtask / a.nblk, blk = gtask - ch * a.nblk;
-        double* o = a.out + ch * a.nout;
+        const TaskIO io = task_io(a.tdesc, a.task0 + task, a.nblk, a.in_cstride, a.in_bstride, a.nout);
+        double* o = a.out + io.out_off;
         const long long x0 = 2LL * k - a.margP, x1 = x0 + 1;
         if (x0 >= 0 && x0 < a.hopP) {
-            const long long d = blk * a.hopP + x0 - a.o;
-            if (d >= 0 && d < a.nout) o[d] = to_f64<A>(y.re);
+            const long long d = io.blk * a.hopP + x0 - a.o;
+            if (d >= 0 && d < io.nout) o[d] = to_f64<A>(y.re);
         }
         if (x1 >= 0 && x1 < a.hopP) {
-            const long long d = blk * a.hopP + x1 - a.o;
-            if (d >= 0 && d < a.nout) o[d] = to_f64<A>(A::neg(y.im));
+            const long long d = io.blk * a.hopP + x1 - a.o;
+            if (d >= 0 && d < io.nout) o[d] = to_f64<A>(A::neg(y.im));
         }
     }
 }

@@ -38,6 +38,20 @@ enum Policy { POL_F64 = 0, POL_DDA = 1, POL_DDS = 2 };
 
 // A batch of equal sub-FFTs, G of them per workgroup, staged through LDS. Element e of sub-FFT j (within a
 // task) is read at complex index e*lde + j*ldg and written at e*ste + j*stg.
+// Where (channel, block) task gtask reads its input block and writes its output samples. One job: channel
+// gtask / nblk, block gtask % nblk, input at channel * in_cstride + block * in_bstride, output channel at
+// channel * nout. A batch of jobs gives a table instead: tdesc[4 gtask + 0..3] = the block's input offset, the
+// channel's output offset (both in doubles), the block index and the channel's output length.
+struct TaskIO {
+    long long in_off, out_off, blk, nout;
+};
+__host__ __device__ inline TaskIO task_io(const long long* tdesc, long long gtask, long long nblk, long long in_cstride,
+                                          long long in_bstride, long long nout) {
+    if (tdesc) return {tdesc[4 * gtask], tdesc[4 * gtask + 1], tdesc[4 * gtask + 2], tdesc[4 * gtask + 3]};
+    const long long ch = gtask / nblk, blk = gtask - ch * nblk;
+    return {ch * in_cstride + blk * in_bstride, ch * nout, blk, nout};
+}
+
 struct PassArgs {
     SubDesc sd;
     const double* tw = nullptr;     // stage twiddles of this sub-FFT length
@@ -49,6 +63,7 @@ struct PassArgs {
     long long src_tstride = 0;      // doubles between tasks (LD_CPLX, LD_REAL_DD)
     long long lde = 1, ldg = 1;
     long long nblk = 1, in_cstride = 0, in_bstride = 0;   // LD_REAL_F64: task t reads in + c*cstride + b*bstride
+    const long long* tdesc = nullptr;   // a batch of jobs: per-task addresses instead (see task_io)
     // store side
     double* dst = nullptr;
     long long dst_tstride = 0;
@@ -86,6 +101,7 @@ struct MidArgs {
 struct BlueArgs {
     int n = 0, m = 0, ntask = 0;
     long long task0 = 0, nblk = 1, in_cstride = 0, in_bstride = 0;
+    const long long* tdesc = nullptr;
     const double* src = nullptr;
     long long src_tstride = 0;
     double* dst = nullptr;

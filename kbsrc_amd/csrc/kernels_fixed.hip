@@ -204,8 +204,7 @@ __global__ __launch_bounds__(T, 4) void k_fixed(PassArgs a) {
     const int wg = xcd_wg();
     const int task = wg / a.ngrp, grp = wg - task * a.ngrp;
     const int j0 = grp * G;
-    const long long gtask = a.task0 + task;
-    const long long ch = gtask / a.nblk, blk = gtask - ch * a.nblk;
+    const TaskIO io = task_io(a.tdesc, a.task0 + task, a.nblk, a.in_cstride, a.in_bstride, a.nout);
     // sub-FFT index of local g, or -1
     auto jof = [&](int g) -> int {
         if constexpr (kPaired) return slot_index<G>(g, grp, a.f.M);
@@ -268,7 +267,7 @@ __global__ __launch_bounds__(T, 4) void k_fixed(PassArgs a) {
             const long long i = e * a.lde + (long long)j * a.ldg;
             Cx<A> v;
             if constexpr (LD == LD_REAL_F64) {
-                const double2 pr = reinterpret_cast<const double2*>(a.src + ch * a.in_cstride + blk * a.in_bstride)[i];
+                const double2 pr = reinterpret_cast<const double2*>(a.src + io.in_off)[i];
                 v = {A::lift(pr.x), A::lift(pr.y)};
             } else {
                 v = gld<A>(a.src + task * a.src_tstride, i);
@@ -312,19 +311,19 @@ __global__ __launch_bounds__(T, 4) void k_fixed(PassArgs a) {
             if constexpr (ST == ST_CPLX) {
                 gst<A>(a.dst + task * a.dst_tstride, i, v);
             } else {
-                double* o = a.out + ch * a.nout;
+                double* o = a.out + io.out_off;
                 const long long x0 = 2 * i - a.margP;
                 if (x0 >= 0 && x0 < a.hopP) {
-                    const long long d = blk * a.hopP + x0 - a.o;
-                    if (d >= 0 && d < a.nout) {
+                    const long long d = io.blk * a.hopP + x0 - a.o;
+                    if (d >= 0 && d < io.nout) {
                         if constexpr (A::kDD) o[d] = v.re.hi + v.re.lo;
                         else o[d] = v.re;
                     }
                 }
                 const long long x1 = x0 + 1;
                 if (x1 >= 0 && x1 < a.hopP) {
-                    const long long d = blk * a.hopP + x1 - a.o;
-                    if (d >= 0 && d < a.nout) {
+                    const long long d = io.blk * a.hopP + x1 - a.o;
+                    if (d >= 0 && d < io.nout) {
                         const Val<A> ni = A::neg(v.im);
                         if constexpr (A::kDD) o[d] = ni.hi + ni.lo;
                         else o[d] = ni;

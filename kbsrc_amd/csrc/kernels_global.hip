@@ -17,8 +17,8 @@ template <class A, int LD> __global__ __launch_bounds__(256) void k_gather(PassA
     const long long i = e * a.lde + j * a.ldg;
     Cx<A> v;
     if constexpr (LD == LD_REAL_F64) {
-        const long long gtask = a.task0 + task, ch = gtask / a.nblk, blk = gtask - ch * a.nblk;
-        const double* s = a.src + ch * a.in_cstride + blk * a.in_bstride;
+        const TaskIO io = task_io(a.tdesc, a.task0 + task, a.nblk, a.in_cstride, a.in_bstride, a.nout);
+        const double* s = a.src + io.in_off;
         v = {A::lift(s[2 * i]), A::lift(s[2 * i + 1])};
     } else if constexpr (LD == LD_REAL_DD) {
         const double* s = a.src + task * a.src_tstride;   // real dd {hi, lo} pairs (FP64: plain reals)
@@ -76,19 +76,19 @@ template <class A, int ST> __global__ __launch_bounds__(256) void k_scatter(Pass
     if constexpr (ST == ST_CPLX) {
         gst<A>(a.dst + task * a.dst_tstride, i, v);
     } else {
-        const long long gtask = a.task0 + task, ch = gtask / a.nblk, blk = gtask - ch * a.nblk;
-        double* o = a.out + ch * a.nout;
+        const TaskIO io = task_io(a.tdesc, a.task0 + task, a.nblk, a.in_cstride, a.in_bstride, a.nout);
+        double* o = a.out + io.out_off;
         const long long x0 = 2 * i - a.margP, x1 = x0 + 1;
         if (x0 >= 0 && x0 < a.hopP) {
-            const long long d = blk * a.hopP + x0 - a.o;
-            if (d >= 0 && d < a.nout) {
+            const long long d = io.blk * a.hopP + x0 - a.o;
+            if (d >= 0 && d < io.nout) {
                 if constexpr (A::kDD) o[d] = v.re.hi + v.re.lo;
                 else o[d] = v.re;
             }
         }
         if (x1 >= 0 && x1 < a.hopP) {
-            const long long d = blk * a.hopP + x1 - a.o;
-            if (d >= 0 && d < a.nout) {
+            const long long d = io.blk * a.hopP + x1 - a.o;
+            if (d >= 0 && d < io.nout) {
                 const Val<A> ni = A::neg(v.im);
                 if constexpr (A::kDD) o[d] = ni.hi + ni.lo;
                 else o[d] = ni;
