@@ -181,7 +181,7 @@ def summarize_rocprof(d, mode):
 
 def committed_traffic(mode, pass_name):
     """HBM bytes per launch of a pass from the committed rocprofv3 summary, or None."""
-    path = os.path.join(ROOT, "profiles", f"passes_c2_{mode}.txt")
+    path = os.path.join(ROOT, "profiles", f"passes_96k_to_44k1_180s_{mode}.txt")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -198,7 +198,7 @@ def cpu_baseline(x, threads, workdir):
     if not os.path.exists(ref):
         return None
     from tests.helpers import wav_f64
-    src, dst = os.path.join(workdir, "c2_in.wav"), os.path.join(workdir, "c2_out.wav")
+    src, dst = os.path.join(workdir, "in.wav"), os.path.join(workdir, "out.wav")
     with open(src, "wb") as f:
         f.write(wav_f64(x, CH, RATE_IN))
     times = []
@@ -231,18 +231,18 @@ def main():
     if args.summarize_rocprof:
         return summarize_rocprof(args.summarize_rocprof, args.mode)
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only (asked for {args.gpus}): the path shards by block ranges with no "
-              "exchange, see DESIGN.md; run with --gpus 1", file=sys.stderr)
+        print(f"bench.py measures one GPU only (asked for {args.gpus}): the path would shard by block ranges with "
+              "no exchange step, and nothing here drives a second GPU; run with --gpus 1", file=sys.stderr)
         return 2
 
     import numpy as np
     import torch
 
     import kbsrc_amd
-    from tests.helpers import c2_input
+    from tests.helpers import input_96k_180s
 
     frames = RATE_IN * SECONDS
-    x = c2_input(frames)
+    x = input_96k_180s(frames)
     t0 = time.perf_counter()
     plan = kbsrc_amd.Plan(frames, CH, RATE_IN, RATE_OUT, args.mode)
     t_design = time.perf_counter() - t0

@@ -363,60 +363,60 @@ template <> struct PolicyOf<POL_DDS> { using type = DDS; };
     Entry{POL, Radices<__VA_ARGS__>::N, LD, ST, G, T,                                                              \
           &launch_fixed_one<PolicyOf<POL>::type, Radices<__VA_ARGS__>, G, T, LD, ST>}
 
-// Transform lengths of the benchmark workloads (c1: 48k->44.1k mono; c2: 96k->44.1k exact and fast; c3: 44.1k->48k).
+// Transform lengths of the benchmark workloads (48k->44.1k mono; 96k->44.1k exact and fast; 44.1k->48k).
 // Several (G, T) variants may follow one another for the same (policy, n, load, store); the first is the default
 // and KBSRC_TUNE="n:k,..." selects the k-th for length n (the sweep that picks the defaults).
 const Entry kEntries[] = {
     // exact (four-step sub-FFTs with the one-TwoSum add)
-    KB_FIXED(POL_DDS, 2, 128, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),    // 315: c2 forward columns
+    KB_FIXED(POL_DDS, 2, 128, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),    // 315: 96k->44.1k forward columns
     KB_FIXED(POL_DDS, 4, 448, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),
     KB_FIXED(POL_DDS, 4, 512, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),
     KB_FIXED(POL_DDS, 7, 512, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),
     KB_FIXED(POL_DDS, 8, 512, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),
-    KB_FIXED(POL_DDS, 8, 512, LD_CPLX, ST_CPLX, 4, 4, 4, 5),        // 320: c2 forward rows
+    KB_FIXED(POL_DDS, 8, 512, LD_CPLX, ST_CPLX, 4, 4, 4, 5),        // 320: 96k->44.1k forward rows
     KB_FIXED(POL_DDS, 4, 320, LD_CPLX, ST_CPLX, 4, 4, 4, 5),
     KB_FIXED(POL_DDS, 4, 256, LD_CPLX, ST_CPLX, 4, 4, 4, 5),
     KB_FIXED(POL_DDS, 4, 512, LD_CPLX, ST_CPLX, 4, 4, 4, 5),
     KB_FIXED(POL_DDS, 2, 128, LD_CPLX, ST_CPLX, 4, 4, 4, 5),
     KB_FIXED(POL_DDS, 6, 384, LD_CPLX, ST_CPLX, 4, 4, 4, 5),
-    KB_FIXED(POL_DDS, 8, 512, LD_CPLX, ST_CPLX, 3, 3, 3, 7),        // 189: c2 inverse columns
+    KB_FIXED(POL_DDS, 8, 512, LD_CPLX, ST_CPLX, 3, 3, 3, 7),        // 189: 96k->44.1k inverse columns
     KB_FIXED(POL_DDS, 5, 192, LD_CPLX, ST_CPLX, 3, 3, 3, 7),
     KB_FIXED(POL_DDS, 7, 320, LD_CPLX, ST_CPLX, 3, 3, 3, 7),
     KB_FIXED(POL_DDS, 7, 512, LD_CPLX, ST_CPLX, 3, 3, 3, 7),
     KB_FIXED(POL_DDS, 4, 256, LD_CPLX, ST_CPLX, 3, 3, 3, 7),
-    KB_FIXED(POL_DDS, 7, 512, LD_CPLX, ST_OUT, 5, 7, 7),            // 245: c2 inverse rows
+    KB_FIXED(POL_DDS, 7, 512, LD_CPLX, ST_OUT, 5, 7, 7),            // 245: 96k->44.1k inverse rows
     KB_FIXED(POL_DDS, 9, 448, LD_CPLX, ST_OUT, 5, 7, 7),
     KB_FIXED(POL_DDS, 9, 512, LD_CPLX, ST_OUT, 5, 7, 7),
     KB_FIXED(POL_DDS, 4, 256, LD_CPLX, ST_OUT, 5, 7, 7),
     KB_FIXED(POL_DDS, 2, 128, LD_CPLX, ST_OUT, 5, 7, 7),
-    KB_FIXED(POL_DDS, 5, 192, LD_REAL_F64, ST_CPLX, 3, 3, 3, 7),    // 189: c3 forward columns
-    KB_FIXED(POL_DDS, 9, 448, LD_CPLX, ST_CPLX, 5, 7, 7),           // 245: c3 forward rows
-    KB_FIXED(POL_DDS, 8, 448, LD_CPLX, ST_CPLX, 4, 4, 2, 7),        // 224: c3 inverse columns
-    KB_FIXED(POL_DDS, 7, 320, LD_CPLX, ST_OUT, 3, 3, 5, 5),         // 225: c3 inverse rows
-    KB_FIXED(POL_DDS, 5, 128, LD_REAL_F64, ST_CPLX, 5, 5, 5),       // 125: c1 forward columns
-    KB_FIXED(POL_DDS, 8, 320, LD_CPLX, ST_CPLX, 4, 4, 2, 5),        // 160: c1 forward rows
-    KB_FIXED(POL_DDS, 5, 128, LD_CPLX, ST_CPLX, 5, 5, 5),           // 125: c1 inverse columns
-    KB_FIXED(POL_DDS, 11, 320, LD_CPLX, ST_OUT, 3, 7, 7),           // 147: c1 inverse rows
+    KB_FIXED(POL_DDS, 5, 192, LD_REAL_F64, ST_CPLX, 3, 3, 3, 7),    // 189: 44.1k->48k forward columns
+    KB_FIXED(POL_DDS, 9, 448, LD_CPLX, ST_CPLX, 5, 7, 7),           // 245: 44.1k->48k forward rows
+    KB_FIXED(POL_DDS, 8, 448, LD_CPLX, ST_CPLX, 4, 4, 2, 7),        // 224: 44.1k->48k inverse columns
+    KB_FIXED(POL_DDS, 7, 320, LD_CPLX, ST_OUT, 3, 3, 5, 5),         // 225: 44.1k->48k inverse rows
+    KB_FIXED(POL_DDS, 5, 128, LD_REAL_F64, ST_CPLX, 5, 5, 5),       // 125: 48k->44.1k forward columns
+    KB_FIXED(POL_DDS, 8, 320, LD_CPLX, ST_CPLX, 4, 4, 2, 5),        // 160: 48k->44.1k forward rows
+    KB_FIXED(POL_DDS, 5, 128, LD_CPLX, ST_CPLX, 5, 5, 5),           // 125: 48k->44.1k inverse columns
+    KB_FIXED(POL_DDS, 11, 320, LD_CPLX, ST_OUT, 3, 7, 7),           // 147: 48k->44.1k inverse rows
     // exact, fused: forward rows with the real-FFT split, inverse columns with x H, bin map and pre-split
-    KB_FIXED(POL_DDS, 4, 320, LD_CPLX, ST_SPLIT, 4, 4, 4, 5),       // 320: c2
-    KB_FIXED(POL_DDS, 6, 192, LD_FOLD, ST_CPLX, 3, 3, 3, 7),        // 189: c2
-    KB_FIXED(POL_DDS, 8, 448, LD_CPLX, ST_SPLIT, 5, 7, 7),          // 245: c3
-    KB_FIXED(POL_DDS, 8, 448, LD_FOLD, ST_CPLX, 4, 4, 2, 7),        // 224: c3
-    KB_FIXED(POL_DDS, 8, 320, LD_CPLX, ST_SPLIT, 4, 4, 2, 5),       // 160: c1
-    KB_FIXED(POL_DDS, 4, 128, LD_FOLD, ST_CPLX, 5, 5, 5),           // 125: c1
+    KB_FIXED(POL_DDS, 4, 320, LD_CPLX, ST_SPLIT, 4, 4, 4, 5),       // 320: 96k->44.1k
+    KB_FIXED(POL_DDS, 6, 192, LD_FOLD, ST_CPLX, 3, 3, 3, 7),        // 189: 96k->44.1k
+    KB_FIXED(POL_DDS, 8, 448, LD_CPLX, ST_SPLIT, 5, 7, 7),          // 245: 44.1k->48k
+    KB_FIXED(POL_DDS, 8, 448, LD_FOLD, ST_CPLX, 4, 4, 2, 7),        // 224: 44.1k->48k
+    KB_FIXED(POL_DDS, 8, 320, LD_CPLX, ST_SPLIT, 4, 4, 2, 5),       // 160: 48k->44.1k
+    KB_FIXED(POL_DDS, 4, 128, LD_FOLD, ST_CPLX, 5, 5, 5),           // 125: 48k->44.1k
     // fast (FP64)
-    KB_FIXED(POL_F64, 8, 256, LD_REAL_F64, ST_CPLX, 4, 2, 5, 5),    // 200: c2 fast forward columns
+    KB_FIXED(POL_F64, 8, 256, LD_REAL_F64, ST_CPLX, 4, 2, 5, 5),    // 200: 96k->44.1k fast forward columns
     KB_FIXED(POL_F64, 6, 128, LD_REAL_F64, ST_CPLX, 4, 2, 5, 5),
     KB_FIXED(POL_F64, 16, 320, LD_REAL_F64, ST_CPLX, 4, 2, 5, 5),
-    KB_FIXED(POL_F64, 8, 256, LD_CPLX, ST_CPLX, 4, 2, 5, 5),        // 200: c2 fast forward rows
+    KB_FIXED(POL_F64, 8, 256, LD_CPLX, ST_CPLX, 4, 2, 5, 5),        // 200: 96k->44.1k fast forward rows
     KB_FIXED(POL_F64, 6, 128, LD_CPLX, ST_CPLX, 4, 2, 5, 5),
     KB_FIXED(POL_F64, 16, 320, LD_CPLX, ST_CPLX, 4, 2, 5, 5),
-    KB_FIXED(POL_F64, 10, 128, LD_CPLX, ST_CPLX, 5, 5, 5),          // 125: c2 fast inverse columns
+    KB_FIXED(POL_F64, 10, 128, LD_CPLX, ST_CPLX, 5, 5, 5),          // 125: 96k->44.1k fast inverse columns
     KB_FIXED(POL_F64, 5, 128, LD_CPLX, ST_CPLX, 5, 5, 5),
-    KB_FIXED(POL_F64, 9, 256, LD_CPLX, ST_OUT, 3, 7, 7),            // 147: c2 fast inverse rows
+    KB_FIXED(POL_F64, 9, 256, LD_CPLX, ST_OUT, 3, 7, 7),            // 147: 96k->44.1k fast inverse rows
     KB_FIXED(POL_F64, 6, 128, LD_CPLX, ST_OUT, 3, 7, 7),
-    KB_FIXED(POL_F64, 6, 128, LD_CPLX, ST_SPLIT, 4, 2, 5, 5),       // 200: c2 fast, fused split
-    KB_FIXED(POL_F64, 10, 128, LD_FOLD, ST_CPLX, 5, 5, 5),          // 125: c2 fast, fused fold
+    KB_FIXED(POL_F64, 6, 128, LD_CPLX, ST_SPLIT, 4, 2, 5, 5),       // 200: 96k->44.1k fast, fused split
+    KB_FIXED(POL_F64, 10, 128, LD_FOLD, ST_CPLX, 5, 5, 5),          // 125: 96k->44.1k fast, fused fold
 };
 #undef KB_FIXED
 

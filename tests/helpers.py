@@ -31,34 +31,34 @@ FRAME1 = (-0.5955104827880859, 0.11259841918945312)
 FRAME1_OUT = ["bfe003180f1a9fbe", "bfd77c39344e9ed3", "3fb8386a3126e979", "3fb1c322eee114b9"]
 
 # Pins measured on the compiled reference for the benchmark workloads (float64 WAV in and out).
-C1_IN = "c4337677f970288e21daeb63dc74cc1605bbc3d5372d06d26ce50776f017fe04"
-C1_PINS = {("exact", True): "25cf220e8557f2b33649a7d7ab8824c8f89708b41d08597fb9e79d9fd5ba33b6",
+IN_48K_1S = "c4337677f970288e21daeb63dc74cc1605bbc3d5372d06d26ce50776f017fe04"
+PINS_48K_1S = {("exact", True): "25cf220e8557f2b33649a7d7ab8824c8f89708b41d08597fb9e79d9fd5ba33b6",
            ("fast", True): "671316329d4984f9fad1286f53b56550bdffbd52de66f1f8f7b90436bf83a251",
            ("exact", False): "46b7159b71385b7c9b373fcb377bdc1d32cc5a03539697f814191a63f109be7b"}
-C2_IN = "f44fdd75bb4cd2fed49617582d5437b96a445d71770e6bd77558fbc59ee9901c"
-C2_PINS = {"exact_f64": "53dbcdc814aba411c04913eec37e65a59f508a7d7b45496e2289aa1d28a7fe2a",
+IN_96K_180S = "f44fdd75bb4cd2fed49617582d5437b96a445d71770e6bd77558fbc59ee9901c"
+PINS_96K_180S = {"exact_f64": "53dbcdc814aba411c04913eec37e65a59f508a7d7b45496e2289aa1d28a7fe2a",
            "fast_f64": "a7da412e086fbafc38026607bd44ab6848a9456e4e37f536c690999a6243fa46",
            "exact_s24": "56c469a411c7b2e45568ad2b38949c8269e0f031a7a5d4cdb04e515eab662c39",
            "exact_s16": "04f5ffe2af3971e0aee6dc4b97089c3aeca307a9ade93895118b000ca091a37f",
            "exact_s32": "06c4edb4e089bb988b91a98fe2aa04e6fdce59b4ce6762d7d6e2a97e13ca0d4f"}
-C3_IN = "4d52e1161c75429ba5ae91dd4eca67135a8cf749c957a1166e3a4ff49ed94166"
-C3_EXACT = "6ed5128888063dca0ec7f8254dba14ac4312a008d69fbe9c6daa95940b5de58e"
+IN_44K_180S = "4d52e1161c75429ba5ae91dd4eca67135a8cf749c957a1166e3a4ff49ed94166"
+PIN_44K_TO_48K_180S = "6ed5128888063dca0ec7f8254dba14ac4312a008d69fbe9c6daa95940b5de58e"
 # 10 MHz -> 48 kHz mono, --len 0.5 (5,000,000-tap kernel, 20,160,000-point block FFT: 3150 x 3200 four-step), on
-# c5_short_input(): sha256 of the raw float64 output samples. Measured on the compiled reference; the oracle gives
+# input_10m_short(): sha256 of the raw float64 output samples. Measured on the compiled reference; the oracle gives
 # the same bits.
-C5_SHORT = {"exact": "cb6ebf41f1876e33961823a63536dbe1576ba05355d50bc5690691b6af408820",
-            "fast": "045b10e3c725eec048444ac74117755bf2507286630753a972785fdf34986c77"}
+PINS_10M_SHORT = {"exact": "cb6ebf41f1876e33961823a63536dbe1576ba05355d50bc5690691b6af408820",
+                  "fast": "045b10e3c725eec048444ac74117755bf2507286630753a972785fdf34986c77"}
 
-# The same rates on 30 s (300,000,000 frames of c5_full_input()): 31 blocks exact, and the fast plan's own blocks.
-C5_FULL = {"exact": "41ba6929f40b098befd88dc7036a0b42fd9b04912f91ff13cb043e7367612dc0",
-           "fast": "f673ce48cae1a103858663581d33bbbce6ce4e566fda554955d46769b77c45cf"}
+# The same rates on 30 s (300,000,000 frames of input_10m_30s()): 31 blocks exact, and the fast plan's own blocks.
+PINS_10M_30S = {"exact": "41ba6929f40b098befd88dc7036a0b42fd9b04912f91ff13cb043e7367612dc0",
+                "fast": "f673ce48cae1a103858663581d33bbbce6ce4e566fda554955d46769b77c45cf"}
 
 
-def c5_full_input():
+def input_10m_30s():
     return np.random.default_rng(5).uniform(-0.5, 0.5, 300_000_000)
 
 
-def c5_short_input():
+def input_10m_short():
     return np.random.default_rng(5).uniform(-0.5, 0.5, 200000)
 
 
@@ -72,11 +72,11 @@ def subnormal97():
     return np.array([math.ldexp(rng.randint(-1024, 1024) / 1024, -1024) for _ in range(194)])
 
 
-def c1_input():
+def input_48k_1s():
     return np.random.default_rng(1).uniform(-0.5, 0.5, 48000)
 
 
-def c2_input(frames=17_280_000):
+def input_96k_180s(frames=17_280_000):
     """Stereo interleaved: channel c from default_rng(c + 1)."""
     x = np.empty((frames, 2))
     x[:, 0] = np.random.default_rng(1).uniform(-0.5, 0.5, frames)
@@ -84,7 +84,7 @@ def c2_input(frames=17_280_000):
     return x
 
 
-def c3_input(frames=44100 * 180):
+def input_44k_180s(frames=44100 * 180):
     x = np.empty((frames, 2))
     x[:, 0] = np.random.default_rng(3).uniform(-0.5, 0.5, frames)
     x[:, 1] = np.random.default_rng(4).uniform(-0.5, 0.5, frames)

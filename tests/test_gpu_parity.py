@@ -39,11 +39,11 @@ def test_frame1_and_subnormal_planar_pins():
 
 
 @pytest.mark.parametrize("mode,lpc", [("exact", True), ("fast", True), ("exact", False)])
-def test_c1_pins(mode, lpc):
-    say(f"c1 48000->44100 {mode} lpc={lpc}")
-    y, st = gpu().resample(H.c1_input(), 48000, 44100, mode, lpc_edges=lpc, return_stats=True)
+def test_48k_to_44k1_pins(mode, lpc):
+    say(f"48k->44.1k mono 1 s {mode} lpc={lpc}")
+    y, st = gpu().resample(H.input_48k_1s(), 48000, 44100, mode, lpc_edges=lpc, return_stats=True)
     assert st.block == 250 and st.p == 147 and st.q == 160 and st.isa == b"gfx950"
-    assert H.wav_sha_planar(y, 44100) == H.C1_PINS[(mode, lpc)]
+    assert H.wav_sha_planar(y, 44100) == H.PINS_48K_1S[(mode, lpc)]
 
 
 CASES = [
@@ -111,11 +111,11 @@ def test_gpu_cli_pins(tmp_path):
         assert H.sha(out.read_bytes()) == want
 
 
-def _c2_plan_run(mode):
+def _run_96k_180s(mode):
     """Full benchmark workload: stereo 96 kHz -> 44.1 kHz, 180 s, through plan/stage/execute on device buffers."""
     import torch
-    x = H.c2_input()
-    assert H.sha(H.wav_f64(x, 2, 96000)) == H.C2_IN
+    x = H.input_96k_180s()
+    assert H.sha(H.wav_f64(x, 2, 96000)) == H.IN_96K_180S
     with gpu().Plan(len(x), 2, 96000, 44100, mode) as plan:
         info = plan.info
         xx = torch.empty(2 * info["xstride"], dtype=torch.float64, device="cuda")
@@ -127,35 +127,35 @@ def _c2_plan_run(mode):
 
 
 @pytest.mark.parametrize("mode", ["exact", "fast"])
-def test_c2_full_size_pins(mode):
-    say(f"c2 full size {mode}")
-    y, info = _c2_plan_run(mode)
+def test_96k_to_44k1_180s_pins(mode):
+    say(f"96k->44.1k stereo 180 s {mode}")
+    y, info = _run_96k_180s(mode)
     assert y.shape == (2, 7_938_000)
-    assert H.wav_sha_planar(y, 44100) == H.C2_PINS[f"{mode}_f64"]
+    assert H.wav_sha_planar(y, 44100) == H.PINS_96K_180S[f"{mode}_f64"]
 
 
-def test_c2_quantised_outputs_bit_exact(tmp_path):
+def test_96k_to_44k1_cli_files_bit_exact(tmp_path):
     """The benchmark workload through the streamed CLI (decode and upload behind the GPU design, output packed and
     quantised on the GPU, no dither): f64, s24, s16 and s32 files match the reference's bytes."""
-    say("c2 CLI f64/s24/s16/s32")
-    src = tmp_path / "c2.wav"
+    say("96k->44.1k stereo 180 s CLI f64/s24/s16/s32")
+    src = tmp_path / "in96k.wav"
     with open(src, "wb") as f:
-        f.write(H.wav_f64(H.c2_input(), 2, 96000))
+        f.write(H.wav_f64(H.input_96k_180s(), 2, 96000))
     for fmt in ("f64", "s24", "s16", "s32"):
-        out = tmp_path / f"c2_{fmt}.wav"
+        out = tmp_path / f"out_{fmt}.wav"
         r = subprocess.run([H.GPU_CLI, "--rate", "44100", "--format", fmt, "-v", str(src), str(out)],
                            capture_output=True, text=True, timeout=110)
         assert r.returncode == 0, r.stderr
         say("  " + fmt + ": " + r.stderr.strip().splitlines()[-1])
-        assert H.sha(out.read_bytes()) == H.C2_PINS[f"exact_{fmt}"], fmt
+        assert H.sha(out.read_bytes()) == H.PINS_96K_180S[f"exact_{fmt}"], fmt
         os.remove(out)
 
 
-def test_c3_upsampling_full_size_pin():
-    say("c3 44.1k->48k full size")
-    x = H.c3_input()
+def test_44k1_to_48k_180s_pin():
+    say("44.1k->48k stereo 180 s")
+    x = H.input_44k_180s()
     y = gpu().resample(x, 44100, 48000, "exact")
-    assert H.wav_sha_planar(y, 48000) == H.C3_EXACT
+    assert H.wav_sha_planar(y, 48000) == H.PIN_44K_TO_48K_180S
 
 
 @pytest.mark.parametrize("mode", ["exact", "fast"])
@@ -164,20 +164,20 @@ def test_long_subfft_extreme_ratio_pin(mode):
     global-memory passes; both modes give the reference's bits."""
     import hashlib
     say(f"10M->48k len 0.5 {mode}")
-    y = gpu().resample(H.c5_short_input().reshape(-1, 1), 10000000, 48000, mode, length=0.5)
+    y = gpu().resample(H.input_10m_short().reshape(-1, 1), 10000000, 48000, mode, length=0.5)
     assert y.shape == (1, 960)
-    assert hashlib.sha256(y.tobytes()).hexdigest() == H.C5_SHORT[mode]
+    assert hashlib.sha256(y.tobytes()).hexdigest() == H.PINS_10M_SHORT[mode]
 
 
 def test_extreme_ratio_full_length_pins():
     """10 MHz -> 48 kHz, 30 s mono, --len 0.5 (the 5,000,001-tap design on the GPU), both modes."""
     import hashlib
-    x = H.c5_full_input().reshape(-1, 1)
+    x = H.input_10m_30s().reshape(-1, 1)
     for mode in ("exact", "fast"):
         say(f"10M->48k 30 s len 0.5 {mode}")
         y = gpu().resample(x, 10000000, 48000, mode, length=0.5)
         assert y.shape == (1, 1440000)
-        assert hashlib.sha256(y.tobytes()).hexdigest() == H.C5_FULL[mode]
+        assert hashlib.sha256(y.tobytes()).hexdigest() == H.PINS_10M_30S[mode]
 
 
 GLOBAL_CASES = [c for c in CASES if c[3] >= 1000 and c[2] <= 2][::2]
@@ -203,8 +203,8 @@ def test_engine_variants_agree(env, mode, monkeypatch):
     for k, v in env.items():
         monkeypatch.setenv(k, v)
     say(f"variant {env} {mode}")
-    y = gpu().resample(H.c1_input(), 48000, 44100, mode)
-    assert H.wav_sha_planar(y, 44100) == H.C1_PINS[(mode, True)]
+    y = gpu().resample(H.input_48k_1s(), 48000, 44100, mode)
+    assert H.wav_sha_planar(y, 44100) == H.PINS_48K_1S[(mode, True)]
     x = np.random.default_rng(11).uniform(-0.5, 0.5, 2 * 30000)
     want = H.oracle_resample(x, 2, 96000, 44100, mode, length=0.05)
     got = gpu().resample(x.reshape(-1, 2), 96000, 44100, mode, length=0.05)

@@ -12,7 +12,7 @@ import helpers as H
 def test_inputs_match_reference_pins():
     assert H.sha(H.wav_f64(H.primary97(), 2, 48000)) == H.PRIMARY_WAV
     assert H.sha(H.wav_f64(H.subnormal97(), 2, 48000)) == H.SUBNORMAL_WAV
-    assert H.sha(H.wav_f64(H.c1_input(), 1, 48000)) == H.C1_IN
+    assert H.sha(H.wav_f64(H.input_48k_1s(), 1, 48000)) == H.IN_48K_1S
 
 
 @pytest.mark.parametrize("name,rate,mode,want", H.CLI_PINS)
@@ -31,8 +31,8 @@ def test_oracle_frame1_and_planar_pins():
 
 @pytest.mark.parametrize("mode,lpc", [("exact", True), ("fast", True), ("exact", False)])
 def test_oracle_c1_pins(mode, lpc):
-    y = H.oracle_resample(H.c1_input(), 1, 48000, 44100, mode, lpc=lpc)
-    assert H.wav_sha_planar(y, 44100) == H.C1_PINS[(mode, lpc)]
+    y = H.oracle_resample(H.input_48k_1s(), 1, 48000, 44100, mode, lpc=lpc)
+    assert H.wav_sha_planar(y, 44100) == H.PINS_48K_1S[(mode, lpc)]
     if mode == "exact" and lpc:   # first samples and peak quoted with the pins
         assert y[0, :3].tolist() == [float.fromhex("0x1.4323a1f5ecdb9p-4"), float.fromhex("0x1.50fb438506410p-2"),
                                      float.fromhex("-0x1.07f0f20da41efp-2")]
