@@ -215,6 +215,163 @@ def cpu_baseline(x, threads, workdir):
     return eng, design
 
 
+# ---- batched form: many files, channel counts and rate pairs in one batch ------------------------------------------
+BATCH_RATES = [44100, 48000, 88200, 96000, 176400, 192000]
+
+
+def batch_jobs(n=64, seed=4):
+    """64 jobs: lengths uniform in [0.5, 300] s, channels in {1, 2, 6, 8}, rate pairs from BATCH_RATES both ways
+    (never equal); even jobs uniform noise in [-0.5, 0.5], odd jobs a sum of 8 sinusoids (amplitude 0.1, random
+    frequencies below 0.45 x the lower rate, random phases), every channel its own draw."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for j in range(n):
+        secs = float(rng.uniform(0.5, 300.0))
+        ch = int(rng.choice([1, 2, 6, 8]))
+        ri, ro = (int(v) for v in rng.choice(BATCH_RATES, 2, replace=False))
+        jobs.append({"frames": int(round(secs * ri)), "channels": ch, "rate_in": ri, "rate_out": ro,
+                     "kind": "noise" if j % 2 == 0 else "sines", "seed": seed * 1000 + j})
+    return jobs
+
+
+def batch_signal(job, max_frames=None):
+    """The job's interleaved (frames, channels) float64 signal, generated on the GPU (torch)."""
+    import numpy as np
+    import torch
+    frames = job["frames"] if max_frames is None else min(job["frames"], max_frames)
+    ch = job["channels"]
+    if job["kind"] == "noise":
+        g = torch.Generator(device="cuda")
+        g.manual_seed(job["seed"])
+        return torch.rand((frames, ch), dtype=torch.float64, device="cuda", generator=g) - 0.5
+    rng = np.random.default_rng(job["seed"])
+    fmax = 0.45 * min(job["rate_in"], job["rate_out"])
+    n = torch.arange(frames, dtype=torch.float64, device="cuda")
+    x = torch.zeros((frames, ch), dtype=torch.float64, device="cuda")
+    for c in range(ch):
+        for f, ph in zip(rng.uniform(0, fmax, 8), rng.uniform(0, 2 * np.pi, 8)):
+            x[:, c] += 0.1 * torch.sin(n * (2 * np.pi * f / job["rate_in"]) + ph)
+    return x
+
+
+def bench_batch(args):
+    import numpy as np
+    import torch
+
+    import kbsrc_amd
+    jobs = batch_jobs()
+    t0 = time.perf_counter()
+    b = kbsrc_amd.Batch([(j["frames"], j["channels"], j["rate_in"], j["rate_out"]) for j in jobs], "exact")
+    t_design = time.perf_counter() - t0
+    d_in = torch.empty(b.in_doubles, dtype=torch.float64, device="cuda")
+    d_out = torch.empty(b.out_doubles, dtype=torch.float64, device="cuda")
+    t0 = time.perf_counter()
+    for k, job in enumerate(jobs):
+        x = batch_signal(job)
+        torch.cuda.synchronize()
+        b.stage_device(k, x.data_ptr(), d_in.data_ptr())
+        del x
+    torch.cuda.synchronize()
+    t_stage = time.perf_counter() - t0
+    in_samples = sum(j["frames"] * j["channels"] for j in jobs)
+    out_samples = sum(L["nout"] * j["channels"] for j, L in zip(jobs, b.layout))
+    print(f"batch: {len(jobs)} jobs, {b.groups} shapes, {in_samples / 1e9:.2f} G input samples, "
+          f"{out_samples / 1e9:.2f} G output samples; design {t_design:.2f} s, generate+stage {t_stage:.2f} s",
+          file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        b.execute(d_in.data_ptr(), d_out.data_ptr())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.execute(d_in.data_ptr(), d_out.data_ptr())
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    value = out_samples / (ms / 1e3) / 1e6
+    print(f"batched: {ms:.2f} ms per batch -> {value:.1f} output Msamples/s", file=sys.stderr, flush=True)
+
+    # the same jobs one plan at a time over the same staged input (a job's padded layout is its plan's)
+    ref_out = torch.empty_like(d_out)
+    single_ms = 0.0
+    flops = 0.0
+    for k, (job, L) in enumerate(zip(jobs, b.layout)):
+        plan = kbsrc_amd.Plan(job["frames"], job["channels"], job["rate_in"], job["rate_out"], "exact")
+        info = plan.info
+        fl = block_flops(info["Nb"], info["Mb"], True, bool(info["up"]))
+        flops += sum(fl.values()) * info["tasks"]
+        src = d_in.data_ptr() + 8 * L["in_offset"]
+        dst = ref_out.data_ptr() + 8 * L["out_offset"]
+        plan.execute(src, dst)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan.execute(src, dst)
+        torch.cuda.synchronize()
+        single_ms += (time.perf_counter() - t0) * 1e3
+        plan.close()
+    same = all(torch.equal(d_out[L["out_offset"]:L["out_offset"] + L["out_doubles"]],
+                           ref_out[L["out_offset"]:L["out_offset"] + L["out_doubles"]]) for L in b.layout)
+    print(f"one plan at a time: {single_ms:.2f} ms; outputs identical: {same}", file=sys.stderr, flush=True)
+    del ref_out
+
+    cpu = None
+    ref = os.path.join(ROOT, "oracle", "_ref", "kbsrc")
+    if not args.no_cpu and os.path.exists(ref):
+        from tests.helpers import wav_f64
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        eng, outs, names = 0.0, 0, []
+        with tempfile.TemporaryDirectory(prefix="kbsrc-bench-") as td:
+            for job in jobs[:4]:
+                x = batch_signal(job, max_frames=20 * job["rate_in"]).cpu().numpy()
+                src, dst = os.path.join(td, "in.wav"), os.path.join(td, "out.wav")
+                with open(src, "wb") as f:
+                    f.write(wav_f64(x, job["channels"], job["rate_in"]))
+                r = subprocess.run([ref, src, dst, "--rate", str(job["rate_out"]), "-t", str(threads), "-v"],
+                                   capture_output=True, text=True, timeout=300)
+                m = re.search(r"design ([0-9.]+)s\s+resample ([0-9.]+)s", r.stderr)
+                if r.returncode or not m:
+                    eng = 0.0
+                    break
+                eng += float(m.group(2))
+                nout = kbsrc_amd.output_frames(x.shape[0], job["rate_in"], job["rate_out"])
+                outs += nout * job["channels"]
+                names.append(f"{job['rate_in']}->{job['rate_out']} {job['channels']}ch {x.shape[0]} frames")
+        if eng > 0:
+            cpu = {"value": round(outs / eng / 1e6, 2), "unit": "output Msamples/s", "cores": threads,
+                   "kind": "reference",
+                   "sample": "the first 4 jobs cut to 20 s each (" + "; ".join(names) +
+                             f"): reference engine resample time {eng:.3f} s, design and file I/O excluded"}
+    line = {
+        "metric": "batched exact-mode output Msamples/s (64 mixed jobs)",
+        "value": round(value, 2),
+        "unit": "output Msamples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "64 jobs, 0.5-300 s, 1/2/6/8 channels, rate pairs among 44.1/48/88.2/96/176.4/192 kHz "
+                               "both ways, half noise half 8 sinusoids, exact mode, len 0.2, beta 40, LPC on",
+                   "jobs": len(jobs), "shapes": b.groups, "input_samples": in_samples, "output_samples": out_samples,
+                   "parallelism": "single GPU"},
+        "one_plan_at_a_time_ms": round(single_ms, 3),
+        "batch_speedup": round(single_ms / ms, 3),
+        "outputs_identical": bool(same),
+        "algorithmic_gflop_per_step": round(flops / 1e9, 2),
+        "step_tflops": round(flops / 1e9 / ms, 3),
+        "design_s": round(t_design, 3),
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["speedup_vs_cpu_engine"] = round(value / cpu["value"], 1)
+    print(json.dumps(line), flush=True)
+    b.close()
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -225,6 +382,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (calibrates HBM byte counters under rocprofv3 --pmc)")
+    ap.add_argument("--workload", default="single", choices=["single", "batch"],
+                    help="single: the stereo 96 kHz -> 44.1 kHz 180 s job (default); batch: 64 mixed jobs in one batch")
     ap.add_argument("--summarize-rocprof", metavar="DIR",
                     help="print the per-pass summary of rocprofv3 csv output under DIR/{stats,fetch,write} and exit")
     args = ap.parse_args()
@@ -234,6 +393,9 @@ def main():
         print(f"bench.py measures one GPU only (asked for {args.gpus}): the path would shard by block ranges with "
               "no exchange step, and nothing here drives a second GPU; run with --gpus 1", file=sys.stderr)
         return 2
+
+    if args.workload == "batch":
+        return bench_batch(args)
 
     import numpy as np
     import torch
