@@ -95,18 +95,22 @@ int pick_group(int policy, int n, int nsub) {
 // Test and tuning switches. Read at every execute: KBSRC_GENERIC_PASSES=1 runs every sub-FFT pass through the
 // generic kernel; KBSRC_FUSED=1 folds the split/multiply/fold kernel into the passes next to it (same bits; it saves
 // a kernel and ~1.2 GB of HBM traffic per benchmark step but measured no faster, the passes being FP64-bound).
-// Read when a plan is made: KBSRC_GLOBAL_PASSES=1 runs the passes through global memory, one launch per stage, as
-// sub-FFTs too long for an LDS tile always do.
+// Read when a plan is made: KBSRC_GLOBAL_PASSES=1 runs the passes through global scratch, as sub-FFTs too long for
+// an LDS tile always do (two LDS passes where the stage list splits, =2: one launch per stage).
 bool env_flag(const char* name) {
     const char* v = std::getenv(name);
     return v && v[0] == '1';
 }
 bool use_fixed() { return !env_flag("KBSRC_GENERIC_PASSES"); }
+bool global_forced() {   // KBSRC_GLOBAL_PASSES=1: split into two LDS passes where possible; =2: one launch per stage
+    const char* v = std::getenv("KBSRC_GLOBAL_PASSES");
+    return v && (v[0] == '1' || v[0] == '2');
+}
 
 // launch one sub-FFT pass; errors name the pass and its shape
 void run_pass(int policy, int ld, int st, PassArgs a, hipStream_t s, const char* what) {
     hipError_t e = hipSuccess;
-    const bool forced = a.gscr0 && ld != LD_FOLD && st != ST_SPLIT && env_flag("KBSRC_GLOBAL_PASSES");
+    const bool forced = a.gscr0 && ld != LD_FOLD && st != ST_SPLIT && global_forced();
     if (!forced && use_fixed() && launch_fixed(policy, ld, st, a, s, &e)) {
     } else if (forced || pass_lds_bytes(policy, a.sd.n, 1) > kLdsMax) {
         if (!a.gscr0) throw std::runtime_error(std::string(what) + ": no global scratch for a long sub-FFT");
@@ -181,7 +185,7 @@ void CFFTPlan::init(int n_, bool as_dd_, const DeviceConsts& k, hipStream_t s) {
         c.ts = ts; c.k = &k; c.dd = true; c.lane = false; c.s = s;
         c.proto.ntask = 1;
         DevBuf g0, g1;
-        if (vp.needs_global() || env_flag("KBSRC_GLOBAL_PASSES")) {
+        if (vp.needs_global() || global_forced()) {
             g0.alloc(ts * 8);
             g1.alloc(ts * 8);
             c.proto.gscr0 = g0.d();
@@ -322,7 +326,7 @@ DevBuf kernel_spectrum_gpu(const double* taps, long long Lk, int n, double scale
     RunCtx c;
     c.ts = ts; c.k = &k; c.dd = true; c.lane = false; c.s = s;
     c.proto.ntask = 1;
-    if (R.c.needs_global() || env_flag("KBSRC_GLOBAL_PASSES")) {
+    if (R.c.needs_global() || global_forced()) {
         g0.alloc(ts * 8);
         g1.alloc(ts * 8);
         c.proto.gscr0 = g0.d();
@@ -367,7 +371,7 @@ Engine::Engine(const Plan& p, const double* H, long long nH, int Jg, hipStream_t
         if (want > 0) tb_ = (int)std::min<long long>(tb_, want);
     }
     for (int i = 0; i < (blue ? 4 : 2); ++i) w_[i].alloc(per_task * tb_);
-    if (fw_.c.needs_global() || iv_.c.needs_global() || env_flag("KBSRC_GLOBAL_PASSES")) {
+    if (fw_.c.needs_global() || iv_.c.needs_global() || global_forced()) {
         g_[0].alloc(per_task * tb_);
         g_[1].alloc(per_task * tb_);
     }

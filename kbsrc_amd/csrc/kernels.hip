@@ -156,8 +156,10 @@ __global__ __launch_bounds__(256) void k_subfft(PassArgs a) {
             const int g = w % G, b = w / G;
             const int p = b / s, q = b - p * s;
             const int dbase = s * r * p + q;
-            const bool twid = p != 0;
-            const double* tw = a.tw + (long long)(st.tw + p * (r - 1)) * (A::kDD ? 4 : 2);
+            // a leading part of a longer transform (see PassArgs::pm) twiddles by the butterfly's index there
+            const long long pg = a.pm ? (long long)(j0 + g) + (long long)a.pm * p : p;
+            const bool twid = pg != 0;
+            const double* tw = a.tw + (st.tw + pg * (r - 1)) * (A::kDD ? 4 : 2);
             if (r == 4) {
                 const Cx<A> a0 = cur.get(b * Gs + g), a1 = cur.get((b + sm) * Gs + g);
                 const Cx<A> a2 = cur.get((b + 2 * sm) * Gs + g), a3 = cur.get((b + 3 * sm) * Gs + g);

@@ -77,6 +77,10 @@ struct PassArgs {
     FuseArgs f;
     double* gscr0 = nullptr;        // global scratch for sub-FFTs too long for LDS: ntask * nsub * n elements each
     double* gscr1 = nullptr;
+    // pm > 0: the pass runs the first stages of longer transforms of length N = pm * sd.n, one per (task, sub-FFT)
+    // column c = the sub-FFT index; the butterfly with index p here is butterfly c + pm * p of the long transform
+    // and takes that one's stage twiddle from sd's tables (indexed for length N).
+    int pm = 0;
 };
 
 // Spectrum x H, bin map and inverse pre-split for one batch of tasks (src/fft.hpp:361-409,411-458;

@@ -1,6 +1,10 @@
-// Sub-FFT passes whose length does not fit LDS (e.g. the 3200-point rows of a 10 MHz -> 48 kHz plan): the same
-// Stockham stages as the LDS kernels, one launch per stage, ping-ponging through two global scratch buffers.
-// Sub-FFT j of task t occupies scratch[(t * nsub + j) * n .. + n). Arithmetic identical to the other passes.
+// Sub-FFT passes whose length does not fit LDS (e.g. the 3200-point rows of a 10 MHz -> 48 kHz plan): gathered
+// into global scratch (sub-FFT j of task t at scratch[(t * nsub + j) * n .. + n)), transformed there, scattered
+// to the pass's destination. The transform runs as two LDS passes of the generic kernel when the stage list splits
+// into two parts that each fit a tile, else one launch per Stockham stage. Arithmetic identical to the other passes.
+#include <algorithm>
+#include <cstdlib>
+
 #include "bfly.hpp"
 
 #pragma clang fp contract(off)
@@ -97,7 +101,32 @@ template <class A, int ST> __global__ __launch_bounds__(256) void k_scatter(Pass
     }
 }
 
-template <class A> hipError_t run_global(int ld, int st, const PassArgs& a, hipStream_t s) {
+constexpr size_t kTile = 160 * 1024;   // LDS of one workgroup
+
+// Split point of a long sub-FFT: its first k stages, on the n / R columns c + (n / R) l of R = s_k points each, and
+// its last stages, on the R classes q + R j of n / R points each, are sub-FFTs that fit an LDS tile (the stages
+// before k never mix columns, those from k on never mix classes). k balances the two lengths; 0 if none fits.
+int split_point(int policy, const SubDesc& sd) {
+    int best = 0, best_len = 1 << 30, R = 1;
+    for (int k = 1; k < sd.nst; ++k) {
+        R *= sd.st[k - 1].r;
+        const int rest = sd.n / R, len = std::max(R, rest);
+        if (pass_lds_bytes(policy, R, 1) > kTile || pass_lds_bytes(policy, rest, 1) > kTile) continue;
+        if (len < best_len) {
+            best_len = len;
+            best = k;
+        }
+    }
+    return best;
+}
+
+void set_group(PassArgs& p, int policy) {   // sub-FFTs per workgroup within 48 KB of LDS, at most 16
+    const int G = (int)std::max<size_t>(1, (size_t)48 * 1024 / pass_lds_bytes(policy, p.sd.n, 1));
+    p.G = std::max(1, std::min({G, p.nsub, 16}));
+    p.ngrp = (p.nsub + p.G - 1) / p.G;
+}
+
+template <class A> hipError_t run_global(int policy, int ld, int st, const PassArgs& a, hipStream_t s) {
     const long long n = a.sd.n, elems = (long long)a.ntask * a.nsub * n;
     const dim3 block(256);
     auto grid = [](long long items) { return dim3((unsigned)((items + 255) / 256)); };
@@ -107,10 +136,38 @@ template <class A> hipError_t run_global(int ld, int st, const PassArgs& a, hipS
     else return hipErrorInvalidValue;
     double* in = a.gscr0;
     double* out = a.gscr1;
-    for (int si = 0; si < a.sd.nst; ++si) {
-        const long long bf = (long long)a.ntask * a.nsub * (n / a.sd.st[si].r);
-        hipLaunchKernelGGL(k_gstage<A>, grid(bf), block, 0, s, a, si, (const double*)in, out);
-        std::swap(in, out);
+    const char* env = std::getenv("KBSRC_GLOBAL_PASSES");
+    const int k = env && env[0] == '2' ? 0 : split_point(policy, a.sd);
+    if (k > 0) {   // two LDS passes through the scratch buffers
+        const int R = a.sd.st[k].s, rest = (int)n / R;
+        const long long tdoubles = n * kElt<A>;   // one long transform in the scratch buffers
+        PassArgs pa = a;
+        pa.sd.n = R;
+        pa.sd.nst = k;
+        for (int i = 0; i < k; ++i) pa.sd.st[i] = {a.sd.st[i].r, R / (a.sd.st[i].s * a.sd.st[i].r), a.sd.st[i].s,
+                                                   a.sd.st[i].tw};
+        pa.src = a.gscr0; pa.src_tstride = tdoubles; pa.dst = a.gscr1; pa.dst_tstride = tdoubles;
+        pa.ntask = (int)((long long)a.ntask * a.nsub); pa.task0 = 0; pa.tdesc = nullptr;
+        pa.nsub = rest; pa.lde = rest; pa.ldg = 1; pa.ste = 1; pa.stg = R;
+        pa.twmode = 0; pa.renorm = 0; pa.pm = rest;
+        set_group(pa, policy);
+        hipError_t e = launch_pass(policy, LD_CPLX, ST_CPLX, pa, 256, s);
+        if (e != hipSuccess) return e;
+        PassArgs pb = pa;
+        pb.sd.n = rest;
+        pb.sd.nst = a.sd.nst - k;
+        for (int i = k; i < a.sd.nst; ++i) pb.sd.st[i - k] = {a.sd.st[i].r, a.sd.st[i].m, a.sd.st[i].s / R, a.sd.st[i].tw};
+        pb.src = a.gscr1; pb.dst = a.gscr0;
+        pb.nsub = R; pb.lde = R; pb.ldg = 1; pb.ste = R; pb.stg = 1; pb.pm = 0;
+        set_group(pb, policy);
+        e = launch_pass(policy, LD_CPLX, ST_CPLX, pb, 256, s);
+        if (e != hipSuccess) return e;
+    } else {   // one launch per stage
+        for (int si = 0; si < a.sd.nst; ++si) {
+            const long long bf = (long long)a.ntask * a.nsub * (n / a.sd.st[si].r);
+            hipLaunchKernelGGL(k_gstage<A>, grid(bf), block, 0, s, a, si, (const double*)in, out);
+            std::swap(in, out);
+        }
     }
     if (st == ST_CPLX) hipLaunchKernelGGL((k_scatter<A, ST_CPLX>), grid(elems), block, 0, s, a, (const double*)in);
     else if (st == ST_OUT) hipLaunchKernelGGL((k_scatter<A, ST_OUT>), grid(elems), block, 0, s, a, (const double*)in);
@@ -122,9 +179,9 @@ template <class A> hipError_t run_global(int ld, int st, const PassArgs& a, hipS
 
 hipError_t launch_global(int policy, int ld, int st, const PassArgs& a, hipStream_t s) {
     if (!a.gscr0 || !a.gscr1) return hipErrorInvalidValue;
-    if (policy == POL_F64) return run_global<F64>(ld, st, a, s);
-    if (policy == POL_DDA) return run_global<DDA>(ld, st, a, s);
-    return run_global<DDS>(ld, st, a, s);
+    if (policy == POL_F64) return run_global<F64>(policy, ld, st, a, s);
+    if (policy == POL_DDA) return run_global<DDA>(policy, ld, st, a, s);
+    return run_global<DDS>(policy, ld, st, a, s);
 }
 
 }  // namespace kbsrc::gpu

@@ -183,10 +183,12 @@ def test_extreme_ratio_full_length_pins():
 GLOBAL_CASES = [c for c in CASES if c[3] >= 1000 and c[2] <= 2][::2]
 
 
+@pytest.mark.parametrize("level", ["1", "2"], ids=["two-lds-passes", "per-stage"])
 @pytest.mark.parametrize("case", GLOBAL_CASES, ids=lambda c: f"{c[0]}-{c[1]}-{c[2]}ch-{c[3]}-{c[4]}")
-def test_global_passes_equal_oracle(case, monkeypatch):
-    """Every sub-FFT pass forced through the global-memory form gives the oracle's bits."""
-    monkeypatch.setenv("KBSRC_GLOBAL_PASSES", "1")
+def test_global_passes_equal_oracle(case, level, monkeypatch):
+    """Every sub-FFT pass forced through the global-memory forms (split into two LDS passes, or one launch per
+    stage) gives the oracle's bits."""
+    monkeypatch.setenv("KBSRC_GLOBAL_PASSES", level)
     rin, rout, ch, frames, mode, length, block, lpc = case
     say(f"global passes {case}")
     x = np.random.default_rng(frames + rin + rout).uniform(-0.5, 0.5, frames * ch)
