@@ -17,7 +17,7 @@ struct Options {
     Precision precision = Precision::Exact;
     double length = 0.2;    // one-sided kernel length in seconds
     double beta = 40.0;     // Kaiser window beta
-    int threads = 0;        // host worker threads for the kernel design, 0 = automatic
+    int threads = 0;        // the reference's worker count (>= 0); accepted, not used by the GPU engine
     int block = 0;          // block size factor nb, 0 = automatic (nb changes the rounding of the output)
     bool lpc_edges = true;  // Burg LPC extrapolation at the signal edges
 };

@@ -26,7 +26,7 @@ typedef struct kbsrc_options {
     int precision;  /* KBSRC_EXACT (default) or KBSRC_FAST */
     double length;  /* one-sided kernel length, seconds (default 0.2) */
     double beta;    /* Kaiser beta (default 40) */
-    int threads;    /* host threads for the kernel design, 0 = automatic */
+    int threads;    /* accepted as the reference's worker count (>= 0); the GPU engine does not use it */
     int block;      /* block size factor nb, 0 = automatic */
     int lpc_edges;  /* Burg LPC edge extension, default 1 */
 } kbsrc_options;
@@ -61,7 +61,7 @@ int kbsrc_resample(const double* interleaved, long long frames, int channels, in
  * run_engine :34-88). A plan owns the kernel spectrum, twiddle tables and work buffers on the current device. */
 typedef struct kbsrc_plan kbsrc_plan;
 
-/* Plans the blocks, designs the kernel (taps on the host, spectrum on the GPU) and builds the FFT tables
+/* Plans the blocks, designs the kernel (taps and spectrum on the GPU) and builds the FFT tables
  * (src/resampler.cpp:92-167,187-206). */
 int kbsrc_plan_create(long long frames, int channels, int rate_in, int rate_out, const kbsrc_options* opt,
                       kbsrc_plan** plan);

@@ -248,7 +248,7 @@ void help() {
         "  -m, --mode M        exact (default) or fast\n"
         "  -f, --format F      f64 (default), f32, s32, s24 or s16\n"
         "      --dither        TPDF dither for integer formats\n"
-        "  -t, --threads N     host threads for the kernel design (default: automatic)\n"
+        "  -t, --threads N     accepted for compatibility (the GPU engine has no host workers)\n"
         "      --len S         one-sided kernel length in seconds (default 0.2)\n"
         "      --beta B        Kaiser beta (default 40)\n"
         "      --block N       block size factor (default: automatic)\n"
