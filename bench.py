@@ -279,6 +279,12 @@ def bench_batch(args):
     print(f"batch: {len(jobs)} jobs, {b.groups} shapes, {in_samples / 1e9:.2f} G input samples, "
           f"{out_samples / 1e9:.2f} G output samples; design {t_design:.2f} s, generate+stage {t_stage:.2f} s",
           file=sys.stderr, flush=True)
+    t_tune = 0.0
+    if args.tune:
+        t0 = time.perf_counter()
+        b.tune(d_in.data_ptr(), d_out.data_ptr())
+        t_tune = time.perf_counter() - t0
+        print(f"tuned {kbsrc_amd.tuned_shapes()} pass shapes in {t_tune:.2f} s", file=sys.stderr, flush=True)
     for _ in range(args.warmup):
         b.execute(d_in.data_ptr(), d_out.data_ptr())
     torch.cuda.synchronize()
@@ -363,6 +369,8 @@ def bench_batch(args):
         "algorithmic_gflop_per_step": round(flops / 1e9, 2),
         "step_tflops": round(flops / 1e9 / ms, 3),
         "design_s": round(t_design, 3),
+        "tuned": bool(args.tune),
+        "tune_s": round(t_tune, 3),
         "cpu_baseline": cpu,
     }
     if cpu:
@@ -382,6 +390,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--profile-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (calibrates HBM byte counters under rocprofv3 --pmc)")
+    ap.add_argument("--tune", action="store_true",
+                    help="measure kernel shapes once before timing (Plan.tune / Batch.tune; same output bits)")
     ap.add_argument("--workload", default="single", choices=["single", "batch"],
                     help="single: the stereo 96 kHz -> 44.1 kHz 180 s job (default); batch: 64 mixed jobs in one batch")
     ap.add_argument("--summarize-rocprof", metavar="DIR",
@@ -420,6 +430,11 @@ def main():
     print(f"plan: nb={info['nb']} Nb={info['Nb']} Mb={info['Mb']} tasks={info['tasks']} "
           f"design {t_design:.3f}s stage {t_stage:.3f}s", file=sys.stderr, flush=True)
 
+    if args.tune:
+        t0 = time.perf_counter()
+        plan.tune(xx.data_ptr(), out.data_ptr(), stream)
+        print(f"tuned {kbsrc_amd.tuned_shapes()} pass shapes in {time.perf_counter() - t0:.2f} s", file=sys.stderr,
+              flush=True)
     if args.profile_copy:
         a = torch.ones(1 << 25, dtype=torch.float64, device=dev)
         b = torch.empty_like(a)
@@ -520,6 +535,7 @@ def main():
         "design_s": round(t_design, 3),
         "stage_s": round(t_stage, 3),
         "e2e_engine_plus_d2h_ms": round(e2e_ms, 3),
+        "tuned": bool(args.tune),
     }
     if cpu:
         line["speedup_vs_cpu_engine"] = round(value / cpu["value"], 1)

@@ -121,6 +121,14 @@ int kbsrc_batch_execute(kbsrc_batch* batch, const double* d_in, double* d_out, v
 
 void kbsrc_batch_destroy(kbsrc_batch* batch);
 
+/* ---- Measured kernel shapes (no counterpart in the reference). Runs the plan or batch once, as its execute call
+ * does (same outputs), timing for every sub-FFT pass not measured before in this process its candidate shapes
+ * (registered specialised variants; generic group sizes and thread counts) and keeping the fastest for every later
+ * execute in the process. Any shape computes the same bits. Synchronous. */
+int kbsrc_plan_tune(kbsrc_plan* plan, const double* d_padded, double* d_out, void* stream);
+int kbsrc_batch_tune(kbsrc_batch* batch, const double* d_in, double* d_out, void* stream);
+int kbsrc_tuned_shapes(void);   /* pass shapes measured so far in this process */
+
 /* ---- The kernel design on the GPU (host buffers in and out; the plan calls do the same device to device). */
 
 /* kernel_taps (src/design.hpp:13-15, src/design.cpp:17-37): one-sided Kaiser-sinc taps h[0..L], relative cutoff

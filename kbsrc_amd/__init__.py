@@ -20,7 +20,7 @@ except ImportError:
 
 __all__ = ["resample", "output_frames", "Plan", "Stats", "version", "library_path", "FAST", "EXACT",
            "KbsrcError", "PLAN_FIELDS", "kernel_taps", "kernel_spectrum", "pack", "PACK_FORMATS", "Batch",
-           "resample_batch"]
+           "resample_batch", "tuned_shapes"]
 
 FAST, EXACT, LONG_DOUBLE = 0, 1, 2
 OK, EINVAL, ERUNTIME, ENOMEM = 0, 1, 2, 3
@@ -76,6 +76,9 @@ def _load():
         "kbsrc_batch_stage": (I, [P, I, P, I, P, P]),
         "kbsrc_batch_execute": (I, [P, P, P, P]),
         "kbsrc_batch_destroy": (None, [P]),
+        "kbsrc_plan_tune": (I, [P, P, P, P]),
+        "kbsrc_batch_tune": (I, [P, P, P, P]),
+        "kbsrc_tuned_shapes": (I, []),
         "kbsrc_kernel_taps": (I, [I, LL, LL, D, P]),
         "kbsrc_kernel_spectrum": (I, [P, I, I, D, P]),
     }
@@ -108,6 +111,11 @@ def _options(precision, length, beta, block, lpc_edges, threads=0) -> _Options:
     o.length, o.beta, o.block, o.lpc_edges, o.threads = float(length), float(beta), int(block), int(bool(lpc_edges)), \
         int(threads)
     return o
+
+
+def tuned_shapes() -> int:
+    """Sub-FFT pass shapes measured so far in this process (Plan.tune, Batch.tune)."""
+    return _lib.kbsrc_tuned_shapes()
 
 
 def version() -> str:
@@ -202,6 +210,10 @@ class Batch:
         _check(_lib.kbsrc_batch_stage(self._h, job, ctypes.c_void_p(d_interleaved), 1, ctypes.c_void_p(d_in),
                                       ctypes.c_void_p(stream)))
 
+    def tune(self, d_in: int, d_out: int, stream: int = 0):
+        """Execute once while measuring kernel shapes for the passes not measured before (same outputs)."""
+        _check(_lib.kbsrc_batch_tune(self._h, ctypes.c_void_p(d_in), ctypes.c_void_p(d_out), ctypes.c_void_p(stream)))
+
     def execute(self, d_in: int, d_out: int, stream: int = 0):
         _check(_lib.kbsrc_batch_execute(self._h, ctypes.c_void_p(d_in), ctypes.c_void_p(d_out),
                                         ctypes.c_void_p(stream)))
@@ -272,6 +284,11 @@ class Plan:
     def execute(self, d_padded: int, d_out: int, stream: int = 0):
         _check(_lib.kbsrc_plan_execute(self._h, ctypes.c_void_p(d_padded), ctypes.c_void_p(d_out),
                                        ctypes.c_void_p(stream)))
+
+    def tune(self, d_padded: int, d_out: int, stream: int = 0):
+        """Execute once while measuring kernel shapes for the passes not measured before (same outputs)."""
+        _check(_lib.kbsrc_plan_tune(self._h, ctypes.c_void_p(d_padded), ctypes.c_void_p(d_out),
+                                    ctypes.c_void_p(stream)))
 
     def execute_timed(self, d_padded: int, d_out: int, stream: int = 0):
         ms = (ctypes.c_double * 5)()

@@ -542,6 +542,31 @@ int kbsrc_batch_execute(kbsrc_batch* b, const double* d_in, double* d_out, void*
 
 void kbsrc_batch_destroy(kbsrc_batch* b) { delete b; }
 
+int kbsrc_plan_tune(kbsrc_plan* pl, const double* d_padded, double* d_out, void* stream) {
+    return guarded([&] {
+        if (!pl || !pl->eng || !d_padded || !d_out) throw std::invalid_argument("bad input buffer");
+        TuningScope tune;
+        pl->eng->execute(d_padded, d_out, static_cast<hipStream_t>(stream));
+        hip_check(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "tuning sync");
+    });
+}
+
+int kbsrc_batch_tune(kbsrc_batch* b, const double* d_in, double* d_out, void* stream) {
+    return guarded([&] {
+        if (!b || !d_in || !d_out) throw std::invalid_argument("bad input buffer");
+        hipStream_t s = static_cast<hipStream_t>(stream);
+        TuningScope tune;
+        for (kbsrc_batch::Group& G : b->groups) G.eng->execute(d_in, d_out, s);   // one after another: clean timings
+        for (const kbsrc_batch::Job& J : b->jobs)
+            if (J.p.equal && J.out_len > 0)
+                hip_check(hipMemcpyAsync(d_out + J.out_off, d_in + J.in_off, sizeof(double) * J.out_len,
+                                         hipMemcpyDeviceToDevice, s), "copy");
+        hip_check(hipStreamSynchronize(s), "tuning sync");
+    });
+}
+
+int kbsrc_tuned_shapes(void) { return tuned_shapes(); }
+
 int kbsrc_kernel_taps(int L, long long a, long long b, double beta, double* out) {
     return guarded([&] {
         if (L < 1 || a < 1 || b < 1 || !(beta > 0) || !out) throw std::invalid_argument("option out of range");

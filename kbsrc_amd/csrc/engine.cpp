@@ -4,6 +4,9 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 #pragma clang fp contract(off)
 
@@ -107,11 +110,119 @@ bool global_forced() {   // KBSRC_GLOBAL_PASSES=1: split into two LDS passes whe
     return v && (v[0] == '1' || v[0] == '2');
 }
 
+// ---- measured kernel shapes. Whatever shape runs a pass, every butterfly computes the same operations, so the
+// choice changes time, never bits. Tuning a pass times every registered specialised variant and a range of generic
+// (G, T) shapes on the pass's real arguments and keeps the fastest for (policy, length, load, store, sub-FFTs).
+struct PassKey {
+    int policy, n, ld, st, nsub;
+    bool operator<(const PassKey& o) const {
+        return std::tie(policy, n, ld, st, nsub) < std::tie(o.policy, o.n, o.ld, o.st, o.nsub);
+    }
+};
+struct PassChoice {
+    int variant;   // >= 0: specialised variant; -1: generic kernel with G, T
+    int G, T;
+};
+std::mutex g_tune_mu;
+std::map<PassKey, PassChoice> g_tuned;
+thread_local bool t_tuning = false;
+
+hipError_t launch_choice(int policy, int ld, int st, PassArgs& a, hipStream_t s, const PassChoice& c) {
+    hipError_t e = hipSuccess;
+    if (c.variant >= 0) {
+        if (!launch_fixed(policy, ld, st, a, s, &e, c.variant)) return hipErrorInvalidValue;
+        return e;
+    }
+    a.G = c.G;
+    a.ngrp = (a.nsub + c.G - 1) / c.G;
+    return launch_pass(policy, ld, st, a, c.T, s);
+}
+
+// Times each candidate shape of this pass (one warm-up launch, then the best of three), runs the fastest last so
+// that its output stands, and records it.
+hipError_t tune_pass(int policy, int ld, int st, PassArgs& a, hipStream_t s, const PassKey& key) {
+    std::vector<PassChoice> cand;
+    const int nv = use_fixed() ? fixed_variants(policy, a.sd.n, ld, st) : 0;
+    for (int v = 0; v < nv; ++v) cand.push_back({v, 0, 0});
+    const size_t one = pass_lds_bytes(policy, a.sd.n, 1);
+    const int gmax = (int)std::max<size_t>(1, std::min<size_t>({(size_t)96 * 1024 / one, (size_t)a.nsub, 16}));
+    for (int G = 1; G <= gmax; ++G)
+        for (int T = 64; T <= kThreads; T += 64)
+            if ((long long)G * a.sd.n >= T) cand.push_back({-1, G, T});
+    if (cand.empty()) cand.push_back({-1, 1, kThreads});
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hip_check(hipEventCreate(&e0), "tune event");
+    hip_check(hipEventCreate(&e1), "tune event");
+    float best_ms = 1e30f;
+    PassChoice best = cand[0];
+    hipError_t err = hipSuccess;
+    for (const PassChoice& c : cand) {
+        PassArgs t = a;
+        if ((err = launch_choice(policy, ld, st, t, s, c)) != hipSuccess) break;
+        float ms = 1e30f;
+        for (int rep = 0; rep < 3 && err == hipSuccess; ++rep) {
+            hip_check(hipEventRecord(e0, s), "tune event");
+            err = launch_choice(policy, ld, st, t, s, c);
+            hip_check(hipEventRecord(e1, s), "tune event");
+            hip_check(hipEventSynchronize(e1), "tune event");
+            float m = 0;
+            hip_check(hipEventElapsedTime(&m, e0, e1), "tune event");
+            ms = std::min(ms, m);
+        }
+        if (err != hipSuccess) break;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = c;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (err != hipSuccess) return err;
+    {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        g_tuned[key] = best;
+    }
+    return launch_choice(policy, ld, st, a, s, best);
+}
+
+}  // namespace
+
+TuningScope::TuningScope() { t_tuning = true; }
+TuningScope::~TuningScope() { t_tuning = false; }
+int tuned_shapes() {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    return (int)g_tuned.size();
+}
+
+namespace {
+
 // launch one sub-FFT pass; errors name the pass and its shape
 void run_pass(int policy, int ld, int st, PassArgs a, hipStream_t s, const char* what) {
     hipError_t e = hipSuccess;
     const bool forced = a.gscr0 && ld != LD_FOLD && st != ST_SPLIT && global_forced();
-    if (!forced && use_fixed() && launch_fixed(policy, ld, st, a, s, &e)) {
+    const bool tunable = !forced && ld != LD_FOLD && st != ST_SPLIT && pass_lds_bytes(policy, a.sd.n, 1) <= kLdsMax;
+    const PassKey key{policy, a.sd.n, ld, st, a.nsub};
+    bool known = false;
+    PassChoice choice{};
+    if (tunable && !t_tuning) {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        const auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) {
+            known = true;
+            choice = it->second;
+        }
+    }
+    if (tunable && t_tuning) {
+        std::unique_lock<std::mutex> lock(g_tune_mu);
+        const auto it = g_tuned.find(key);
+        known = it != g_tuned.end();
+        if (known) choice = it->second;
+    }
+    if (tunable && t_tuning && !known) {
+        e = tune_pass(policy, ld, st, a, s, key);
+    } else if (known) {
+        e = launch_choice(policy, ld, st, a, s, choice);
+    } else if (!forced && use_fixed() && launch_fixed(policy, ld, st, a, s, &e)) {
     } else if (forced || pass_lds_bytes(policy, a.sd.n, 1) > kLdsMax) {
         if (!a.gscr0) throw std::runtime_error(std::string(what) + ": no global scratch for a long sub-FFT");
         e = launch_global(policy, ld, st, a, s);

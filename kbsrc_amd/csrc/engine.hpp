@@ -105,6 +105,14 @@ struct RunCtx {
 void run_cfft(const CFFTPlan& f, const RunCtx& c, int ld, const double* src, int st, double* dst,
               double* const scr[3], double* t_col, double* t_row);
 
+// While a TuningScope lives on this thread, every sub-FFT pass whose shape (policy, length, load, store, sub-FFTs)
+// has not been measured in this process times its candidate kernel shapes and keeps the fastest for all later runs.
+struct TuningScope {
+    TuningScope();
+    ~TuningScope();
+};
+int tuned_shapes();   // pass shapes measured so far in this process
+
 // The Kaiser-sinc taps h[0..L] (src/design.cpp:17-37) on the GPU, as (hi, lo) pairs in device memory.
 DevBuf kernel_taps_gpu(long long L, long long a, long long b, double beta, hipStream_t s);
 

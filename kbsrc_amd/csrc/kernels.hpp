@@ -173,7 +173,9 @@ size_t pass_lds_bytes(int policy, int n, int G);
 hipError_t launch_pass(int policy, int ld, int st, const PassArgs& a, int threads, hipStream_t s);
 // Compile-time specialised pass for the benchmark transform lengths; false when (policy, n, ld, st) has none.
 // Sets a.G and a.ngrp to the specialisation's group size.
-bool launch_fixed(int policy, int ld, int st, PassArgs& a, hipStream_t s, hipError_t* err);
+// variant: index among the registered shapes of (policy, n, ld, st); -1: the default (or KBSRC_TUNE's)
+bool launch_fixed(int policy, int ld, int st, PassArgs& a, hipStream_t s, hipError_t* err, int variant = -1);
+int fixed_variants(int policy, int n, int ld, int st);
 bool has_fixed(int policy, int n, int ld, int st);
 // Sub-FFT pass through global memory, one launch per stage (lengths whose tile does not fit LDS).
 hipError_t launch_global(int policy, int ld, int st, const PassArgs& a, hipStream_t s);

@@ -444,8 +444,14 @@ int tune_index(int n) {
     return 0;
 }
 
-bool launch_fixed(int policy, int ld, int st, PassArgs& a, hipStream_t s, hipError_t* err) {
-    int skip = tune_index(a.sd.n);
+int fixed_variants(int policy, int n, int ld, int st) {
+    int c = 0;
+    for (const Entry& e : kEntries) c += e.policy == policy && e.n == n && e.ld == ld && e.st == st;
+    return c;
+}
+
+bool launch_fixed(int policy, int ld, int st, PassArgs& a, hipStream_t s, hipError_t* err, int variant) {
+    int skip = variant >= 0 ? variant : tune_index(a.sd.n);
     for (const Entry& e : kEntries) {
         if (e.policy == policy && e.n == a.sd.n && e.ld == ld && e.st == st && skip-- <= 0) {
             a.G = e.G;
