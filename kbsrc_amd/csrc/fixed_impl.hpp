@@ -1,0 +1,426 @@
+// Sub-FFT passes specialised at compile time for given transform lengths (included by the kernel tables).
+//
+// Same arithmetic as the generic pass in kernels.hip (and so as the reference: src/dsp/fft.hpp:86-151), but the
+// radix sequence, the number of sub-FFTs per workgroup G and the thread count T are template constants. That
+// turns every index division into shifts or multiplies, unrolls the butterflies, and lets the stages run in a
+// single LDS buffer: each thread computes all of its butterflies of a stage into registers, the workgroup
+// synchronises, and the results are written back in place. Half the LDS of the ping-pong form, so twice the
+// workgroups per CU.
+#pragma once
+#include <algorithm>
+#include <cstdlib>
+#include <type_traits>
+#include <utility>
+
+#include "bfly.hpp"
+
+#pragma clang fp contract(off)
+
+namespace kbsrc::gpu {
+
+namespace {
+
+template <class A, int CAP> struct Lds {
+    double* b;
+    __device__ __forceinline__ Cx<A> get(int i) const {
+        if constexpr (A::kDD) return {dd{b[i], b[CAP + i]}, dd{b[2 * CAP + i], b[3 * CAP + i]}};
+        else return {b[i], b[CAP + i]};
+    }
+    __device__ __forceinline__ void put(int i, Cx<A> v) const {
+        if constexpr (A::kDD) {
+            b[i] = v.re.hi; b[CAP + i] = v.re.lo; b[2 * CAP + i] = v.im.hi; b[3 * CAP + i] = v.im.lo;
+        } else {
+            b[i] = v.re; b[CAP + i] = v.im;
+        }
+    }
+};
+
+// Compile-time stage list in the reference's radix order.
+template <int... R> struct Radices {
+    static constexpr int nst = sizeof...(R);
+    static constexpr int r[nst] = {R...};
+    static constexpr int N = (R * ...);
+    static constexpr int stride(int i) {   // product of the earlier radices
+        int p = 1;
+        for (int k = 0; k < i; ++k) p *= r[k];
+        return p;
+    }
+    static constexpr int twoff(int i) {    // offset of stage i's twiddles: sum of m_k (r_k - 1) before it
+        int o = 0;
+        for (int k = 0; k < i; ++k) o += (N / (stride(k) * r[k])) * (r[k] - 1);
+        return o;
+    }
+    // complex values one thread holds in the widest stage, for G sub-FFTs on T threads
+    static constexpr int hold(int G, int T) {
+        int h = 0;
+        for (int k = 0; k < nst; ++k) {
+            const int W = G * (N / r[k]), it = (W + T - 1) / T;
+            h = it * r[k] > h ? it * r[k] : h;
+        }
+        return h;
+    }
+};
+
+// One Stockham stage over G sub-FFTs in place: compute all of this thread's butterflies, sync, write back.
+template <class A, class RL, int G, int T, int I>
+__device__ __forceinline__ void stage(const Lds<A, RL::N * G>& L, int tid, const double* tw_all, const double* oddc) {
+    constexpr int N = RL::N, r = RL::r[I], s = RL::stride(I), nbf = N / r, W = G * nbf, IT = (W + T - 1) / T;
+    const double* tws = tw_all + (long long)RL::twoff(I) * kElt<A>;
+    Cx<A> held[IT][r];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int w = tid + it * T;
+        if (W % T == 0 || w < W) {
+            const int g = w % G, b = w / G, p = b / s;
+            Cx<A> x[r];
+#pragma unroll
+            for (int t = 0; t < r; ++t) x[t] = L.get((b + t * nbf) * G + g);
+            butterfly<A, r>(x, held[it], p != 0, tws + (long long)p * (r - 1) * kElt<A>, oddc);
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int w = tid + it * T;
+        if (W % T == 0 || w < W) {
+            const int g = w % G, b = w / G, p = b / s, q = b - p * s;
+            const int dbase = s * r * p + q;
+#pragma unroll
+            for (int u = 0; u < r; ++u) L.put((dbase + u * s) * G + g, held[it][u]);
+        }
+    }
+    __syncthreads();
+}
+
+template <class A, class RL, int G, int T, int... I>
+__device__ __forceinline__ void all_stages(const Lds<A, RL::N * G>& L, int tid, const double* tw, const double* oddc,
+                                           std::integer_sequence<int, I...>) {
+    (stage<A, RL, G, T, I>(L, tid, tw, oddc), ...);
+}
+
+// Workgroup index remapped so that each of the 8 XCDs (which take workgroups round-robin) works on a contiguous
+// range of the grid: neighbouring sub-FFT groups share cache lines in that XCD's L2.
+__device__ __forceinline__ int xcd_wg() {
+    const int n = gridDim.x, i = blockIdx.x, full = n & ~7;
+    if (i >= full) return i;
+    return (i & 7) * (full >> 3) + (i >> 3);
+}
+
+// Arithmetic of the splits and the bin map: the reference's scalar dd (accurate add) or FP64.
+template <class A> using SplitPol = std::conditional_t<A::kDD, DDA, F64>;
+
+template <class A> __device__ __forceinline__ Val<A> ldH(const double* H, long long i) {
+    if constexpr (A::kDD) {
+        const double2 v = reinterpret_cast<const double2*>(H)[i];
+        return dd{v.x, v.y};
+    } else {
+        return H[i];
+    }
+}
+
+// Slot pairing of fused passes: local sub-FFT g is member (g & 1) of slot grp*G/2 + g/2; member 0 is the slot's
+// own index s, member 1 its mirror M - s. Returns the sub-FFT index, or -1 for an empty place.
+template <int G> __device__ __forceinline__ int slot_index(int g, int grp, int M) {
+    const int slot = grp * (G / 2) + g / 2, mem = g & 1;
+    if (2 * slot > M) return -1;
+    if (mem && (slot == 0 || 2 * slot == M)) return -1;
+    return mem ? M - slot : slot;
+}
+template <int G> __device__ __forceinline__ int slot_local(int j, int grp, int M) {
+    const int mem = (j != 0 && 2 * j > M) ? 1 : 0;
+    const int slot = mem ? M - j : j;
+    return 2 * (slot - grp * (G / 2)) + mem;
+}
+
+// Y[idx] = X[idx] H[idx] with the bin map (src/resampler.cpp:64-78): down folds bins above the output Nyquist onto
+// Mb - j, up extends X periodically.
+template <class S> __device__ __forceinline__ Cx<S> ybin(const FuseArgs& f, const double* X, int idx) {
+    if (!f.up) {
+        const Cx<S> x = gld<S>(X, idx);
+        const Val<S> hk = ldH<S>(f.H, idx);
+        if (idx == f.hinv) return {S::mul(S::mul(S::lift(2.0), x.re), hk), S::lift(0.0)};
+        Cx<S> y = {S::mul(x.re, hk), S::mul(x.im, hk)};
+        const int j = f.Mb - idx;
+        if (j >= f.K && j <= f.jmax) {
+            const Cx<S> xj = gld<S>(X, j);
+            const Val<S> hj = ldH<S>(f.H, j);
+            y = c_add<S>(y, c_conj<S>(Cx<S>{S::mul(xj.re, hj), S::mul(xj.im, hj)}));
+        }
+        return y;
+    }
+    const int kk = idx % f.Nb;
+    const Cx<S> v = kk <= f.Nb / 2 ? gld<S>(X, kk) : c_conj<S>(gld<S>(X, f.Nb - kk));
+    const Val<S> hk = ldH<S>(f.H, idx);
+    Cx<S> y = {S::mul(v.re, hk), S::mul(v.im, hk)};
+    if (idx == f.hinv) y.im = S::lift(0.0);
+    return y;
+}
+
+// inverse pre-split of bin k from Y[k] and its partner Y[hinv - k] (src/dsp/fft.hpp:412-458)
+template <class S> __device__ __forceinline__ Cx<S> presplit(const FuseArgs& f, int k, Cx<S> Y, Cx<S> Yp) {
+    if (!S::kDD || 2 * k <= f.hinv) {   // v-form: A = Y[k], B = conj(Y[hinv-k])
+        const Cx<S> B = c_conj<S>(Yp);
+        const Cx<S> E = c_add<S>(Y, B), D = c_mul<S>(c_sub<S>(Y, B), c_conj<S>(gld<S>(f.wi, k)));
+        return c_conj<S>(Cx<S>{S::sub(E.re, D.im), S::add(E.im, D.re)});
+    }
+    // dd, upper half: the partner's E and split (A = Y[hinv-k], B = conj(Y[k])) with this bin's twiddle
+    const Cx<S> B = c_conj<S>(Y);
+    const Cx<S> E = c_add<S>(Yp, B), sp = c_sub<S>(Yp, B);
+    const Cx<S> Ej = c_conj<S>(E);
+    const Cx<S> Dj = c_mul<S>(Cx<S>{S::neg(sp.re), sp.im}, c_conj<S>(gld<S>(f.wi, k)));
+    return c_conj<S>(Cx<S>{S::sub(Ej.re, Dj.im), S::add(Ej.im, Dj.re)});
+}
+
+// forward split X[i] from z[i] and z[h - i] (src/dsp/fft.hpp:365-407), 0 < i < h
+template <class S> __device__ __forceinline__ Cx<S> fsplit(const FuseArgs& f, int i, Cx<S> zi, Cx<S> zp) {
+    const Val<S> half = S::lift(0.5);
+    Cx<S> v;
+    if (!S::kDD || 2 * i <= f.h) {   // v-form: A = z[i], B = conj(z[h-i])
+        const Cx<S> B = c_conj<S>(zp);
+        const Cx<S> E = c_add<S>(zi, B), O = c_mul_mi<S>(c_sub<S>(zi, B));
+        v = c_add<S>(E, c_mul<S>(O, gld<S>(f.wf, i)));
+    } else {                          // dd, upper half: from the partner's E and O (A = z[h-i], B = conj(z[i]))
+        const Cx<S> B = c_conj<S>(zi);
+        const Cx<S> E = c_add<S>(zp, B), O = c_mul_mi<S>(c_sub<S>(zp, B));
+        v = c_add<S>(c_conj<S>(E), c_mul<S>(c_conj<S>(O), gld<S>(f.wf, i)));
+    }
+    return {S::mul(v.re, half), S::mul(v.im, half)};
+}
+
+template <class A> __device__ __forceinline__ Cx<A> renorm_dd(Cx<A> v) {
+    if constexpr (A::kDD) return {quick2(v.re.hi, v.re.lo), quick2(v.im.hi, v.im.lo)};
+    else return v;
+}
+
+// __launch_bounds__'s second value is the minimum number of waves per SIMD the register allocation must allow:
+// 4 caps a thread at 128 VGPRs, so that LDS (not registers) limits occupancy.
+template <class A, class RL, int G, int T, int LD, int ST>
+__global__ __launch_bounds__(T, 4) void k_fixed(PassArgs a) {
+    constexpr int N = RL::N, CAP = N * G;
+    constexpr bool kPaired = LD == LD_FOLD || ST == ST_SPLIT;
+    using S = SplitPol<A>;
+    static_assert(std::is_same_v<Val<S>, Val<A>>);
+    extern __shared__ double lds[];   // kPlanes * CAP doubles
+    const Lds<A, CAP> L{lds};
+    const int tid = threadIdx.x;
+    const int wg = xcd_wg();
+    const int task = wg / a.ngrp, grp = wg - task * a.ngrp;
+    const int j0 = grp * G;
+    const TaskIO io = task_io(a.tdesc, a.task0 + task, a.nblk, a.in_cstride, a.in_bstride, a.nout);
+    // sub-FFT index of local g, or -1
+    auto jof = [&](int g) -> int {
+        if constexpr (kPaired) return slot_index<G>(g, grp, a.f.M);
+        else return j0 + g < a.nsub ? j0 + g : -1;
+    };
+
+    if constexpr (LD == LD_FOLD) {
+        // Y = X H with the bin map for every element, then the pre-split of each with its mirror bin
+        const double* X = a.src + task * a.src_tstride;
+        constexpr int IT = (G * N + T - 1) / T;
+        Cx<A> yh = {A::lift(0.0), A::lift(0.0)};
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int w = tid + it * T;
+            if (w < G * N) {
+                const int g = w % G, e = w / G, j = jof(g);
+                if (j >= 0) {
+                    const int k = e * a.f.M + j;
+                    L.put(e * G + g, ybin<S>(a.f, X, k));
+                    if (k == 0) yh = ybin<S>(a.f, X, a.f.hinv);
+                }
+            }
+        }
+        __syncthreads();
+        Cx<A> zs[IT];
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int w = tid + it * T;
+            if (w < G * N) {
+                const int g = w % G, e = w / G, j = jof(g);
+                if (j >= 0) {
+                    const int k = e * a.f.M + j;
+                    Cx<A> yp;
+                    if (k == 0) {
+                        yp = yh;
+                    } else {
+                        const int kp = a.f.hinv - k, ep = kp / a.f.M, jp = kp - ep * a.f.M;
+                        yp = L.get(ep * G + slot_local<G>(jp, grp, a.f.M));
+                    }
+                    zs[it] = presplit<S>(a.f, k, L.get(e * G + g), yp);
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int it = 0; it < IT; ++it) {
+            const int w = tid + it * T;
+            if (w < G * N) {
+                const int g = w % G, e = w / G;
+                if (jof(g) >= 0) L.put(e * G + g, zs[it]);
+            }
+        }
+    } else {
+        // load, lanes sub-FFT-fastest: a wave writes one contiguous LDS range (no bank conflicts) and reads 64/G
+        // consecutive elements of each of its G sub-FFTs (or G neighbouring columns of 64/G rows)
+        for (int w = tid; w < G * N; w += T) {
+            const int g = w % G, e = w / G;
+            const int j = jof(g);
+            if (j < 0) continue;
+            const long long i = e * a.lde + (long long)j * a.ldg;
+            Cx<A> v;
+            if constexpr (LD == LD_REAL_F64) {
+                const double2 pr = reinterpret_cast<const double2*>(a.src + io.in_off)[i];
+                v = {A::lift(pr.x), A::lift(pr.y)};
+            } else {
+                v = gld<A>(a.src + task * a.src_tstride, i);
+            }
+            L.put(e * G + g, v);
+        }
+    }
+    __syncthreads();
+    all_stages<A, RL, G, T>(L, tid, a.tw, a.oddc, std::make_integer_sequence<int, RL::nst>{});
+
+    if constexpr (ST == ST_SPLIT) {
+        // the sub-FFT outputs are z[M*e + j] (natural order); X[i] needs z[i] and z[h - i]
+        double* X = a.dst + task * a.dst_tstride;
+        for (int w = tid; w < G * N; w += T) {
+            const int g = w % G, e = w / G, j = jof(g);
+            if (j < 0) continue;
+            const int i = e * a.f.M + j;
+            const Cx<S> zi = renorm_dd<A>(L.get(e * G + g));
+            if (i == 0) {
+                gst<S>(X, 0, Cx<S>{S::add(zi.re, zi.im), S::lift(0.0)});
+                if (a.f.h <= a.f.xmax) gst<S>(X, a.f.h, Cx<S>{S::sub(zi.re, zi.im), S::lift(0.0)});
+                continue;
+            }
+            if (i > a.f.xmax) continue;
+            const int ip = a.f.h - i, ep = ip / a.f.M, jp = ip - ep * a.f.M;
+            const Cx<S> zp = renorm_dd<A>(L.get(ep * G + slot_local<G>(jp, grp, a.f.M)));
+            gst<S>(X, i, fsplit<S>(a.f, i, zi, zp));
+        }
+    } else {
+        for (int w = tid; w < G * N; w += T) {   // lanes sub-FFT-fastest, as the load
+            const int g = w % G, e = w / G;
+            const int j = jof(g);
+            if (j < 0) continue;
+            const long long jg = j;
+            const long long i = e * a.ste + jg * a.stg;
+            Cx<A> v = L.get(e * G + g);
+            if (a.twmode && e != 0 && (a.twmode == 1 || jg != 0)) v = c_mul<A>(v, gld<A>(a.tw4, i));
+            if constexpr (A::kDD) {
+                if (a.renorm) v = renorm_dd<A>(v);
+            }
+            if constexpr (ST == ST_CPLX) {
+                gst<A>(a.dst + task * a.dst_tstride, i, v);
+            } else {
+                double* o = a.out + io.out_off;
+                const long long x0 = 2 * i - a.margP;
+                if (x0 >= 0 && x0 < a.hopP) {
+                    const long long d = io.blk * a.hopP + x0 - a.o;
+                    if (d >= 0 && d < io.nout) {
+                        if constexpr (A::kDD) o[d] = v.re.hi + v.re.lo;
+                        else o[d] = v.re;
+                    }
+                }
+                const long long x1 = x0 + 1;
+                if (x1 >= 0 && x1 < a.hopP) {
+                    const long long d = io.blk * a.hopP + x1 - a.o;
+                    if (d >= 0 && d < io.nout) {
+                        const Val<A> ni = A::neg(v.im);
+                        if constexpr (A::kDD) o[d] = ni.hi + ni.lo;
+                        else o[d] = ni;
+                    }
+                }
+            }
+        }
+    }
+}
+
+
+template <class A, class RL, int G, int T, int LD, int ST> hipError_t launch_fixed_one(const PassArgs& a, hipStream_t s) {
+    static_assert(RL::hold(G, T) <= (A::kDD ? 8 : 16), "too many values held per thread");
+    static_assert(LD != LD_FOLD || (G * RL::N + T - 1) / T <= (A::kDD ? 8 : 16), "too many values held per thread");
+    static_assert((LD != LD_FOLD && ST != ST_SPLIT) || G % 2 == 0, "paired passes need an even group");
+    constexpr size_t bytes = (size_t)kPlanes<A> * RL::N * G * sizeof(double);
+    static_assert(bytes <= 160 * 1024, "LDS tile too large");
+    static const hipError_t attr = bytes > 64 * 1024
+        ? hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fixed<A, RL, G, T, LD, ST>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes)
+        : hipSuccess;
+    if (attr != hipSuccess) return attr;
+    hipLaunchKernelGGL((k_fixed<A, RL, G, T, LD, ST>), dim3(a.ngrp * a.ntask), dim3(T), bytes, s, a);
+    return hipGetLastError();
+}
+
+template <int P> struct PolicyOf;
+template <> struct PolicyOf<POL_F64> { using type = F64; };
+template <> struct PolicyOf<POL_DDA> { using type = DDA; };
+template <> struct PolicyOf<POL_DDS> { using type = DDS; };
+
+#define KB_FIXED(POL, G, T, LD, ST, ...)                                                                            \
+    FixedEntry{POL, Radices<__VA_ARGS__>::N, LD, ST, G, T,                                                              \
+          &launch_fixed_one<PolicyOf<POL>::type, Radices<__VA_ARGS__>, G, T, LD, ST>}
+
+// ---- shapes chosen at compile time for lengths without a measured table entry: the radix list in the reference's
+// order, and the (G, T) with the best product of lane use over the stages and waves per CU, within 80 KB of LDS and
+// the register bound on held values (measured tuning may replace the choice at run time).
+constexpr int radix_count(int n) {
+    int c = 0;
+    while (n % 4 == 0) { n /= 4; ++c; }
+    while (n % 2 == 0) { n /= 2; ++c; }
+    for (int f : {3, 5, 7})
+        while (n % f == 0) { n /= f; ++c; }
+    return c;
+}
+constexpr int radix_at(int n, int i) {
+    int c = 0;
+    while (n % 4 == 0) { n /= 4; if (c++ == i) return 4; }
+    while (n % 2 == 0) { n /= 2; if (c++ == i) return 2; }
+    for (int f : {3, 5, 7})
+        while (n % f == 0) { n /= f; if (c++ == i) return f; }
+    return 0;
+}
+template <int N, class I> struct RadOf;
+template <int N, int... I> struct RadOf<N, std::integer_sequence<int, I...>> {
+    using type = Radices<radix_at(N, I)...>;
+};
+template <int N> using RadicesOf = typename RadOf<N, std::make_integer_sequence<int, radix_count(N)>>::type;
+
+struct Shape {
+    int G, T;
+};
+constexpr Shape auto_shape(int n, bool dd) {
+    const int planes = dd ? 4 : 2, hmax = dd ? 8 : 16, nst = radix_count(n);
+    Shape best{1, 128};
+    long long best_score = -1, best_elems = -1;
+    for (int T = 128; T <= 512; T += 64) {
+        for (int G = 1; G <= 16; ++G) {
+            const long long lds = (long long)planes * 8 * n * G;
+            if (lds > 80 * 1024) break;
+            int h = 0;
+            long long eff = 0;   // lane use in 1e-6 units, summed over stages
+            for (int i = 0; i < nst; ++i) {
+                const int r = radix_at(n, i), W = G * (n / r), it = (W + T - 1) / T;
+                h = it * r > h ? it * r : h;
+                eff += 1000000LL * W / ((long long)it * T);
+            }
+            if (h > hmax) continue;
+            const long long wgs = 160 * 1024 / lds, waves = wgs * T / 64 < 16 ? wgs * T / 64 : 16;
+            const long long score = eff / nst * waves / 16 / 100;   // coarse, so near-equal shapes tie
+            if (score > best_score || (score == best_score && (long long)G * n > best_elems)) {
+                best_score = score;
+                best_elems = (long long)G * n;
+                best = {G, T};
+            }
+        }
+    }
+    return best;
+}
+
+#define KB_AUTO(POL, N, LD, ST)                                                                                   \
+    FixedEntry{POL, N, LD, ST, auto_shape(N, POL != POL_F64).G, auto_shape(N, POL != POL_F64).T,                   \
+               &launch_fixed_one<PolicyOf<POL>::type, RadicesOf<N>, auto_shape(N, POL != POL_F64).G,              \
+                                 auto_shape(N, POL != POL_F64).T, LD, ST>}
+
+}  // namespace
+}  // namespace kbsrc::gpu

@@ -173,6 +173,17 @@ size_t pass_lds_bytes(int policy, int n, int G);
 hipError_t launch_pass(int policy, int ld, int st, const PassArgs& a, int threads, hipStream_t s);
 // Compile-time specialised pass for the benchmark transform lengths; false when (policy, n, ld, st) has none.
 // Sets a.G and a.ngrp to the specialisation's group size.
+// A specialised pass: policy, length, load and store kinds, sub-FFTs per workgroup G, threads T, and its launcher.
+struct FixedEntry {
+    int policy, n, ld, st, G, T;
+    hipError_t (*launch)(const PassArgs&, hipStream_t);
+};
+// generated tables of compile-time shapes (kernels_fixed_auto*.hip)
+const FixedEntry* fixed_table_auto0(int* count);
+const FixedEntry* fixed_table_auto1(int* count);
+const FixedEntry* fixed_table_auto2(int* count);
+const FixedEntry* fixed_table_auto3(int* count);
+
 // variant: index among the registered shapes of (policy, n, ld, st); -1: the default (or KBSRC_TUNE's)
 bool launch_fixed(int policy, int ld, int st, PassArgs& a, hipStream_t s, hipError_t* err, int variant = -1);
 int fixed_variants(int policy, int n, int ld, int st);
