@@ -407,7 +407,6 @@ int kbsrc_batch_create(int njobs, const kbsrc_job* jobs, const kbsrc_options* op
         b->opt = o;
         b->jobs.resize(njobs);
         auto align = [](long long v) { return (v + 31) / 32 * 32; };
-        std::vector<long long> keys;   // (rate_in, rate_out, nb) of each group
         for (int j = 0; j < njobs; ++j) {
             const kbsrc_job& J = jobs[j];
             if (J.channels < 1 || J.frames < 0) throw std::invalid_argument("bad input buffer");
@@ -422,7 +421,7 @@ int kbsrc_batch_create(int njobs, const kbsrc_job* jobs, const kbsrc_options* op
             job.out_len = p.ch * p.nout;
             b->out_total += align(job.out_len);
             if (p.equal) continue;
-            int g = 0;
+            int g = 0;   // jobs with the same rates and block size share a group
             for (; g < (int)b->groups.size(); ++g) {
                 const Plan& q = b->groups[g].p;
                 if (q.fi == p.fi && q.fo == p.fo && q.nb == p.nb) break;
