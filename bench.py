@@ -380,6 +380,64 @@ def bench_batch(args):
     return 0
 
 
+# ---- split-precision form: radix-16 DFT stages, int8 MFMA slices against the plain dd pass ---------------------
+def bench_split(args):
+    import numpy as np
+    import torch
+
+    import kbsrc_amd
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from test_gpu_split import dd_vectors, errors
+    nvec = 1 << 22   # 4,194,304 radix-16 DFTs = 67,108,864 points
+    x = torch.from_numpy(dd_vectors(nvec, 5).reshape(-1)).cuda()
+    y = torch.empty_like(x)
+    ref = torch.empty_like(x)
+    sample = 16
+
+    def timed(fn, *a):
+        for _ in range(max(1, args.warmup)):
+            fn(x.data_ptr(), nvec, *a, y.data_ptr())
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn(x.data_ptr(), nvec, *a, y.data_ptr())
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / args.steps
+
+    xs = x[: sample * 64].cpu().numpy().reshape(sample, 16, 4)
+    plain_ms = timed(kbsrc_amd.plain_dft16)
+    ref.copy_(y)
+    e_plain = max(errors(xs, ref[: sample * 64].cpu().numpy().reshape(sample, 16, 4), sample))
+    rows = {}
+    for s in (6, 7, 8, 12, 15, 16):
+        ms = timed(kbsrc_amd.split_dft16, s)
+        e = max(errors(xs, y[: sample * 64].cpu().numpy().reshape(sample, 16, 4), sample))
+        d = float(((y - ref).abs().max() / ref.abs().max()).item())
+        rows[s] = {"ms": round(ms, 4), "ps_per_point": round(ms * 1e9 / (16 * nvec), 2),
+                   "err_vs_256bit_log2": round(float(np.log2(e)), 1), "diff_vs_plain_dd": d,
+                   "speed_vs_plain": round(plain_ms / ms, 3)}
+        print(f"slices {s}: {ms:.3f} ms, error 2^{np.log2(e):.1f}, x{plain_ms / ms:.3f} vs plain", file=sys.stderr,
+              flush=True)
+    ok = [s for s in rows if rows[s]["err_vs_256bit_log2"] <= -100]
+    s100 = min(ok) if ok else None
+    line = {
+        "metric": "split-MFMA vs plain dd radix-16 DFT speed ratio at <= 2^-100 error",
+        "value": rows[s100]["speed_vs_plain"] if s100 else None,
+        "unit": "x (plain dd time / split time)",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": rows[s100]["ms"] if s100 else None,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "int8 MFMA + f64",
+        "data": "synthetic",
+        "config": {"workload": f"{nvec} forward DFTs of 16 complex dd values (67.1 M points), magnitudes 2^-20..2^20",
+                   "slices_at_2^-100": s100, "parallelism": "single GPU"},
+        "plain_dd": {"ms": round(plain_ms, 4), "ps_per_point": round(plain_ms * 1e9 / (16 * nvec), 2),
+                     "err_vs_256bit_log2": round(float(np.log2(e_plain)), 1)},
+        "split": rows,
+    }
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -392,8 +450,9 @@ def main():
                     help="also run a plain 256 MiB device copy (calibrates HBM byte counters under rocprofv3 --pmc)")
     ap.add_argument("--tune", action="store_true",
                     help="measure kernel shapes once before timing (Plan.tune / Batch.tune; same output bits)")
-    ap.add_argument("--workload", default="single", choices=["single", "batch"],
-                    help="single: the stereo 96 kHz -> 44.1 kHz 180 s job (default); batch: 64 mixed jobs in one batch")
+    ap.add_argument("--workload", default="single", choices=["single", "batch", "split"],
+                    help="single: the stereo 96 kHz -> 44.1 kHz 180 s job (default); batch: 64 mixed jobs in one "
+                         "batch; split: int8-MFMA split-precision radix-16 DFTs against the plain dd pass")
     ap.add_argument("--summarize-rocprof", metavar="DIR",
                     help="print the per-pass summary of rocprofv3 csv output under DIR/{stats,fetch,write} and exit")
     args = ap.parse_args()
@@ -406,6 +465,8 @@ def main():
 
     if args.workload == "batch":
         return bench_batch(args)
+    if args.workload == "split":
+        return bench_split(args)
 
     import numpy as np
     import torch

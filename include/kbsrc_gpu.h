@@ -129,6 +129,13 @@ int kbsrc_plan_tune(kbsrc_plan* plan, const double* d_padded, double* d_out, voi
 int kbsrc_batch_tune(kbsrc_batch* batch, const double* d_in, double* d_out, void* stream);
 int kbsrc_tuned_shapes(void);   /* pass shapes measured so far in this process */
 
+/* ---- Split-precision (Ozaki) radix-16 DFT on the int8 matrix cores, beside the plain dd pass (no counterpart in the
+ * reference; not used by the conversion, which is bit-exact dd). nvec vectors of 16 complex dd values in device
+ * memory, element format {re.hi, im.hi, re.lo, im.lo}; y = forward DFT of each. slices: 4-8, 10, 12, 14, 15, 16 base-128
+ * digits per operand. Synchronous. */
+int kbsrc_split_dft16(const double* d_x, long long nvec, int slices, double* d_y, void* stream);
+int kbsrc_plain_dft16(const double* d_x, long long nvec, double* d_y, void* stream);
+
 /* ---- The kernel design on the GPU (host buffers in and out; the plan calls do the same device to device). */
 
 /* kernel_taps (src/design.hpp:13-15, src/design.cpp:17-37): one-sided Kaiser-sinc taps h[0..L], relative cutoff

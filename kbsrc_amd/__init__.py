@@ -20,7 +20,8 @@ except ImportError:
 
 __all__ = ["resample", "output_frames", "Plan", "Stats", "version", "library_path", "FAST", "EXACT",
            "KbsrcError", "PLAN_FIELDS", "kernel_taps", "kernel_spectrum", "pack", "PACK_FORMATS", "Batch",
-           "resample_batch", "tuned_shapes"]
+           "resample_batch", "tuned_shapes",
+           "split_dft16", "plain_dft16"]
 
 FAST, EXACT, LONG_DOUBLE = 0, 1, 2
 OK, EINVAL, ERUNTIME, ENOMEM = 0, 1, 2, 3
@@ -79,6 +80,8 @@ def _load():
         "kbsrc_plan_tune": (I, [P, P, P, P]),
         "kbsrc_batch_tune": (I, [P, P, P, P]),
         "kbsrc_tuned_shapes": (I, []),
+        "kbsrc_split_dft16": (I, [P, LL, I, P, P]),
+        "kbsrc_plain_dft16": (I, [P, LL, P, P]),
         "kbsrc_kernel_taps": (I, [I, LL, LL, D, P]),
         "kbsrc_kernel_spectrum": (I, [P, I, I, D, P]),
     }
@@ -111,6 +114,16 @@ def _options(precision, length, beta, block, lpc_edges, threads=0) -> _Options:
     o.length, o.beta, o.block, o.lpc_edges, o.threads = float(length), float(beta), int(block), int(bool(lpc_edges)), \
         int(threads)
     return o
+
+
+def split_dft16(d_x: int, nvec: int, slices: int, d_y: int, stream: int = 0):
+    """Split-precision (int8 MFMA) forward DFTs of nvec 16-point complex dd vectors (device pointers)."""
+    _check(_lib.kbsrc_split_dft16(ctypes.c_void_p(d_x), nvec, slices, ctypes.c_void_p(d_y), ctypes.c_void_p(stream)))
+
+
+def plain_dft16(d_x: int, nvec: int, d_y: int, stream: int = 0):
+    """The same DFTs by the engine's plain dd radix-4 x 4 pass."""
+    _check(_lib.kbsrc_plain_dft16(ctypes.c_void_p(d_x), nvec, ctypes.c_void_p(d_y), ctypes.c_void_p(stream)))
 
 
 def tuned_shapes() -> int:

@@ -416,6 +416,23 @@ void run_cfft(const CFFTPlan& f, const RunCtx& c, int ld, const double* src, int
     }
 }
 
+void plain_dft16(const double* x, long long nvec, double* y, hipStream_t s) {
+    if (nvec < 1) return;
+    if (nvec > 0x7fffffffLL / 16) throw std::invalid_argument("bad input buffer");
+    DeviceConsts k;
+    k.init(s);
+    SubFFT sf;
+    sf.init(16, true, s);
+    PassArgs a;
+    fill_sub(a, sf, k, true, POL_DDS, (int)nvec);
+    a.ntask = 1;
+    a.src = x; a.src_tstride = 0; a.lde = 1; a.ldg = 16;
+    a.dst = y; a.dst_tstride = 0; a.ste = 1; a.stg = 16;
+    a.renorm = 1;
+    run_pass(POL_DDS, LD_CPLX, ST_CPLX, a, s, "radix-16 pass");
+    hip_check(hipStreamSynchronize(s), "radix-16 sync");
+}
+
 DevBuf kernel_taps_gpu(long long L, long long a, long long b, double beta, hipStream_t s) {
     TapArgs t;
     t.L = L; t.a = a; t.b = b; t.beta = beta;

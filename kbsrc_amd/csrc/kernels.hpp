@@ -201,6 +201,10 @@ hipError_t launch_circ(const double* h, long long Lk, long long n, double* r, hi
 // hi parts of n (hi, lo) pairs
 hipError_t launch_hi(const double* pairs, long long n, double* out, hipStream_t s);
 hipError_t launch_pack(PackArgs a, hipStream_t s);
+// Split-precision radix-16 DFT (int8 MFMA slices; not on the conversion path): nvec vectors of 16 complex dd (plan
+// element format) x -> y; adig: the S slices of M / 2 in the MFMA A layout, (slice, M-tile, lane) -> 8 bytes.
+hipError_t launch_split_dft16(const double* x, long long nvec, int slices, const long* adig, double* y,
+                              hipStream_t s);
 hipError_t launch_cis_table(const TableArgs& a, hipStream_t s);
 // Bluestein chirp b[t] = cis(t^2 mod 2n, 2n), t < n, into bc (plan format) and the chirp sequence v (dd format,
 // m elements, zeroed by the caller): v[t] = v[m - t] = b[t] (src/dsp/fft.hpp:231-236)

@@ -49,6 +49,9 @@ const FixedEntry kEntries[] = {
     KB_FIXED(POL_DDS, 8, 448, LD_FOLD, ST_CPLX, 4, 4, 2, 7),        // 224: 44.1k->48k
     KB_FIXED(POL_DDS, 8, 320, LD_CPLX, ST_SPLIT, 4, 4, 2, 5),       // 160: 48k->44.1k
     KB_FIXED(POL_DDS, 4, 128, LD_FOLD, ST_CPLX, 5, 5, 5),           // 125: 48k->44.1k
+    // 16 (radix 4 x 4): the plain dd baseline of the split-precision radix-16 comparison
+    KB_FIXED(POL_DDS, 64, 256, LD_CPLX, ST_CPLX, 4, 4),
+    KB_FIXED(POL_DDS, 32, 128, LD_CPLX, ST_CPLX, 4, 4),
     // fast (FP64)
     KB_FIXED(POL_F64, 8, 256, LD_REAL_F64, ST_CPLX, 4, 2, 5, 5),    // 200: 96k->44.1k fast forward columns
     KB_FIXED(POL_F64, 6, 128, LD_REAL_F64, ST_CPLX, 4, 2, 5, 5),

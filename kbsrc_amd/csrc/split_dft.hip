@@ -1,0 +1,143 @@
+// Split-precision (Ozaki) form of a radix-16 DFT stage on the int8 matrix cores, measured against the plain
+// double-double stage. Not on the conversion path: the engine computes every butterfly in dd, bit for bit as the
+// reference does; this form exists to measure what int8 MFMA slices buy at double-double accuracy.
+//
+// y = F x for vectors x of 16 complex dd values, F[k][n] = e^{-2 pi i k n / 16}, as one real 32 x 32 product
+// [Re y; Im y] = M [Re x; Im x]. Each vector is scaled by a power of two to |z| < 1/2 and cut into S base-128
+// digits, z = sum_j b_j 2^{-7 j}: the S - 1 low ones balanced in [-64, 63], the leading one in [-65, 65]; M / 2
+// likewise into a_i. The products a_i b_j with i + j <= S + 1 run on
+// v_mfma_i32_16x16x32_i8 (exact int32 sums of 32 terms), summed per weight 2^{-7 (i + j)} in int32 (exact: at most
+// S products of |a|, |b| <= 65 over 32 terms), and the weighted sums are recombined exactly in 128-bit integers
+// and rounded once to dd. Error: the digits kept (about 2^{-7 S} of the vector's largest value) and the products
+// dropped.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace kbsrc::gpu {
+
+namespace {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// 128-bit integer nearest to (hi + lo) * 2^k (k such that the result fits)
+__device__ __forceinline__ __int128 to_fixed(double hi, double lo, int k) {
+    const double hs = __builtin_ldexp(hi, k), ls = __builtin_ldexp(lo, k);
+    const double r1 = __builtin_rint(hs);
+    const double rem = (hs - r1) + ls;
+    return (__int128)r1 + (__int128)__builtin_rint(rem);
+}
+
+// Wave layout of v_mfma_i32_16x16x32_i8 (checked on the device against an integer product): lane l holds A row
+// l % 16, k = 8 (l / 16) .. + 7; B column l % 16, same k; D column l % 16, rows 4 (l / 16) .. + 3.
+template <int S>
+__global__ __launch_bounds__(256) void k_split_dft16(const double* __restrict__ x, long long nvec,
+                                                     const long* __restrict__ adig, double* __restrict__ y) {
+    const int lane = threadIdx.x & 63, q = lane >> 4, col = lane & 15;
+    const long long v = ((long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16 + col;
+    const bool live = v < nvec;
+    // this lane's 8 reals of vector v: k = 8q .. 8q+7, k < 16: Re x[k], else Im x[k - 16]
+    double hi[8], lo[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int k = 8 * q + i, n = k & 15, part = k >> 4;
+        if (live) {
+            const double* e = x + (v * 16 + n) * 4;
+            hi[i] = e[part];
+            lo[i] = e[2 + part];
+        } else {
+            hi[i] = 0.0;
+            lo[i] = 0.0;
+        }
+    }
+    // the vector's scale: largest |hi| over its 32 reals (4 lanes share a column)
+    double m = 0.0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) m = __builtin_fmax(m, __builtin_fabs(hi[i]));
+    m = __builtin_fmax(m, __shfl_xor(m, 16));
+    m = __builtin_fmax(m, __shfl_xor(m, 32));
+    int ex = 0;
+    if (m > 0) (void)__builtin_frexp(m, &ex);
+    const int e = m > 0 ? ex + 1 : 0;   // |x| 2^{-e} < 1/2
+    // S base-128 digits of each real, most significant first, packed per slice: byte i of b[j] is digit j of
+    // real 8q + i
+    long b[S];
+#pragma unroll
+    for (int j = 0; j < S; ++j) b[j] = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        __int128 Z = to_fixed(hi[i], lo[i], 7 * S - e);
+#pragma unroll
+        for (int j = S - 1; j >= 1; --j) {
+            const int d = (int)((Z + 64) & 127) - 64;
+            Z = (Z - d) >> 7;
+            b[j] |= (long)(uint8_t)(int8_t)d << (8 * i);
+        }
+        b[0] |= (long)(uint8_t)(int8_t)(int)Z << (8 * i);
+    }
+    // products a_i b_j, i + j <= S + 1 (1-based), summed per weight t = i + j
+    v4i acc[S][2];
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+        acc[t][0] = v4i{0, 0, 0, 0};
+        acc[t][1] = v4i{0, 0, 0, 0};
+    }
+#pragma unroll
+    for (int i = 0; i < S; ++i) {
+        const long a0 = adig[(i * 2 + 0) * 64 + lane], a1 = adig[(i * 2 + 1) * 64 + lane];
+#pragma unroll
+        for (int j = 0; i + j + 2 <= S + 1; ++j) {
+            acc[i + j][0] = __builtin_amdgcn_mfma_i32_16x16x32_i8(a0, b[j], acc[i + j][0], 0, 0, 0);
+            acc[i + j][1] = __builtin_amdgcn_mfma_i32_16x16x32_i8(a1, b[j], acc[i + j][1], 0, 0, 0);
+        }
+    }
+    if (!live) return;
+    // rows 16 mt + 4 q + r: exact sum over weights in 128 bits, one rounding to dd, scale back
+    const int shift = 1 + e - 7 * (S + 1);
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            __int128 V = 0;
+#pragma unroll
+            for (int t = 0; t < S; ++t) V = V * 128 + acc[t][mt][r];
+            const double h = (double)V;
+            const double l = (double)(V - (__int128)h);
+            const int row = 16 * mt + 4 * q + r, k = row & 15, part = row >> 4;
+            double* o = y + (v * 16 + k) * 4;
+            o[part] = __builtin_ldexp(h, shift);
+            o[2 + part] = __builtin_ldexp(l, shift);
+        }
+    }
+}
+
+template <int S> hipError_t launch_s(const double* x, long long nvec, const long* adig, double* y, hipStream_t s) {
+    const long long waves = (nvec + 15) / 16, blocks = (waves + 3) / 4;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_split_dft16<S>), dim3((unsigned)blocks), dim3(256), 0, s, x, nvec, adig, y);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_split_dft16(const double* x, long long nvec, int slices, const long* adig, double* y,
+                              hipStream_t s) {
+    if (nvec < 1) return hipSuccess;
+    switch (slices) {
+        case 4: return launch_s<4>(x, nvec, adig, y, s);
+        case 5: return launch_s<5>(x, nvec, adig, y, s);
+        case 6: return launch_s<6>(x, nvec, adig, y, s);
+        case 7: return launch_s<7>(x, nvec, adig, y, s);
+        case 8: return launch_s<8>(x, nvec, adig, y, s);
+        case 10: return launch_s<10>(x, nvec, adig, y, s);
+        case 12: return launch_s<12>(x, nvec, adig, y, s);
+        case 14: return launch_s<14>(x, nvec, adig, y, s);
+        case 15: return launch_s<15>(x, nvec, adig, y, s);
+        case 16: return launch_s<16>(x, nvec, adig, y, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace kbsrc::gpu
