@@ -10,6 +10,7 @@ reference's sources) on the same workload on this host, and reports the roofline
 """
 import argparse
 import json
+import math
 import os
 import re
 import subprocess
@@ -420,6 +421,9 @@ def bench_split(args):
               flush=True)
     ok = [s for s in rows if rows[s]["err_vs_256bit_log2"] <= -100]
     s100 = min(ok) if ok else None
+    del x, y, ref
+    torch.cuda.empty_cache()
+    conv = bench_split_conversion(args, (6, 7, 8, s100 or 16))
     line = {
         "metric": "split-MFMA vs plain dd radix-16 DFT speed ratio at <= 2^-100 error",
         "value": rows[s100]["speed_vs_plain"] if s100 else None,
@@ -433,9 +437,78 @@ def bench_split(args):
         "plain_dd": {"ms": round(plain_ms, 4), "ps_per_point": round(plain_ms * 1e9 / (16 * nvec), 2),
                      "err_vs_256bit_log2": round(float(np.log2(e_plain)), 1)},
         "split": rows,
+        "conversion": conv,
     }
     print(json.dumps(line), flush=True)
     return 0
+
+
+def bench_split_conversion(args, slices):
+    """The stereo 96 kHz -> 44.1 kHz 180 s exact conversion, plain and in the split form (its 320-point forward
+    row sub-FFTs run their first two radix-4 stages as the int8-MFMA radix-16 stage): time per conversion, the
+    largest difference from the plain output over every sample, and one middle block's error against the 2^-256
+    block restatement (tests/hp_block.py)."""
+    import numpy as np
+    import torch
+
+    import kbsrc_amd
+    import hp_block as hb
+    from tests.helpers import input_96k_180s
+
+    frames = RATE_IN * SECONDS
+    x = input_96k_180s(frames)
+    plan = kbsrc_amd.Plan(frames, CH, RATE_IN, RATE_OUT, "exact")
+    info = plan.info
+    dev = torch.device("cuda:0")
+    xx = torch.empty(CH * info["xstride"], dtype=torch.float64, device=dev)
+    out = torch.empty(CH * info["nout"], dtype=torch.float64, device=dev)
+    plain = torch.empty_like(out)
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    plan.stage(x, xx.data_ptr(), stream)
+    del x
+
+    def timed(run):
+        for _ in range(max(1, args.warmup)):
+            run()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            run()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / args.steps
+
+    plain_ms = timed(lambda: plan.execute(xx.data_ptr(), plain.data_ptr(), stream))
+    peak = float(plain.abs().max().item())
+    b = info["nblk"] // 2
+    P, Q, hop = info["P"], info["Q"], info["hop"]
+    blk = xx[b * hop * Q: b * hop * Q + info["Nb"]].cpu().numpy()
+    gc = math.gcd(info["c2"], RATE_IN)
+    taps = kbsrc_amd.kernel_taps(info["Lk"], info["c2"] // gc, RATE_IN // gc, 40.0)
+    spec = kbsrc_amd.kernel_spectrum(taps, info["Nb"], info["Nb"])
+    t0 = time.perf_counter()
+    ref = hb.block_output(blk, spec, P, Q, info["nb"], info["marg"], False)
+    d0 = b * hop * P - info["o"]
+
+    def block_err(y):
+        return float(hb.errors(y[d0:d0 + len(ref)].cpu().numpy(), ref).max() / peak)
+
+    e_plain = block_err(plain)
+    print(f"conversion plain: {plain_ms:.3f} ms; block {b} error 2^{math.log2(e_plain + 1e-300):.1f} of peak "
+          f"(2^-256 block in {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    rows = {}
+    for sl in slices:
+        ms = timed(lambda: plan.execute_split(xx.data_ptr(), out.data_ptr(), sl, stream))
+        d = float((out - plain).abs().max().item()) / peak
+        e = block_err(out)
+        rows[sl] = {"ms": round(ms, 3), "speed_vs_plain": round(plain_ms / ms, 3),
+                    "max_diff_vs_plain_over_peak_log2": round(math.log2(d), 1) if d > 0 else None,
+                    "block_err_vs_2^-256_over_peak_log2": round(math.log2(e), 1) if e > 0 else None}
+        print(f"conversion split S={sl}: {ms:.3f} ms (x{plain_ms / ms:.3f}), vs plain 2^{math.log2(d + 1e-300):.1f}, "
+              f"block error 2^{math.log2(e + 1e-300):.1f} of peak", file=sys.stderr, flush=True)
+    plan.close()
+    return {"workload": "stereo 96000 -> 44100 Hz, 180 s, exact, block 630 (f64 output)",
+            "plain": {"ms": round(plain_ms, 3), "block_err_vs_2^-256_over_peak_log2": round(math.log2(e_plain), 1)},
+            "split": rows}
 
 
 def main():

@@ -91,6 +91,12 @@ int kbsrc_plan_execute(kbsrc_plan* plan, const double* d_padded, double* d_out, 
  * ms[0] forward column, ms[1] forward row, ms[2] split/multiply/fold, ms[3] inverse column, ms[4] inverse row. */
 int kbsrc_plan_execute_timed(kbsrc_plan* plan, const double* d_padded, double* d_out, void* stream, double* ms);
 
+/* The split-precision form of the same conversion (no counterpart in the reference; not bit-exact): exact-mode plans
+ * only; every sub-FFT whose first two Stockham stages are radix 4 runs them as one radix-16 stage on the int8
+ * matrix cores with `slices` base-128 slices per operand (4-8, 10, 12, 14, 15, 16), the rest in dd. Asynchronous on
+ * stream; the first call per slice count makes the stage tables (synchronous). */
+int kbsrc_plan_execute_split(kbsrc_plan* plan, const double* d_padded, double* d_out, int slices, void* stream);
+
 void kbsrc_plan_destroy(kbsrc_plan* plan);
 
 /* ---- Batches: many jobs (files, channel counts, rate pairs, lengths) in one device input buffer and one device

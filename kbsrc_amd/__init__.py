@@ -67,6 +67,7 @@ def _load():
         "kbsrc_plan_query": (I, [P, ctypes.POINTER(LL)]),
         "kbsrc_plan_stage": (I, [P, P, P, P]),
         "kbsrc_plan_execute": (I, [P, P, P, P]),
+        "kbsrc_plan_execute_split": (I, [P, P, P, I, P]),
         "kbsrc_plan_execute_timed": (I, [P, P, P, P, ctypes.POINTER(D)]),
         "kbsrc_plan_destroy": (None, [P]),
         "kbsrc_plan_stage_device": (I, [P, P, P, P]),
@@ -297,6 +298,12 @@ class Plan:
     def execute(self, d_padded: int, d_out: int, stream: int = 0):
         _check(_lib.kbsrc_plan_execute(self._h, ctypes.c_void_p(d_padded), ctypes.c_void_p(d_out),
                                        ctypes.c_void_p(stream)))
+
+    def execute_split(self, d_padded: int, d_out: int, slices: int, stream: int = 0):
+        """The split-precision form (exact-mode plans): sub-FFTs whose first two stages are radix 4 run them as one
+        int8-MFMA radix-16 stage with ``slices`` base-128 slices per operand. Not bit-exact; asynchronous."""
+        _check(_lib.kbsrc_plan_execute_split(self._h, ctypes.c_void_p(d_padded), ctypes.c_void_p(d_out), int(slices),
+                                             ctypes.c_void_p(stream)))
 
     def tune(self, d_padded: int, d_out: int, stream: int = 0):
         """Execute once while measuring kernel shapes for the passes not measured before (same outputs)."""

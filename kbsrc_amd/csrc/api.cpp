@@ -385,6 +385,14 @@ int kbsrc_plan_execute(kbsrc_plan* pl, const double* d_padded, double* d_out, vo
     });
 }
 
+int kbsrc_plan_execute_split(kbsrc_plan* pl, const double* d_padded, double* d_out, int slices, void* stream) {
+    return guarded([&] {
+        if (!pl || !pl->eng || !d_padded || !d_out) throw std::invalid_argument("bad input buffer");
+        NearestRounding rn(true);   // the split tables are made in host dd arithmetic on first use
+        pl->eng->execute(d_padded, d_out, static_cast<hipStream_t>(stream), nullptr, slices);
+    });
+}
+
 int kbsrc_plan_execute_timed(kbsrc_plan* pl, const double* d_padded, double* d_out, void* stream, double* ms) {
     return guarded([&] {
         if (!pl || !pl->eng || !d_padded || !d_out || !ms) throw std::invalid_argument("bad input buffer");
@@ -577,40 +585,9 @@ int kbsrc_split_dft16(const double* d_x, long long nvec, int slices, double* d_y
     return guarded([&] {
         if (nvec < 0 || (nvec > 0 && (!d_x || !d_y))) throw std::invalid_argument("bad input buffer");
         const int S = slices;
-        if (!(S >= 4 && S <= 8) && S != 10 && S != 12 && S != 14 && S != 15 && S != 16)
-            throw std::invalid_argument("option out of range");
+        if (!split_slices_ok(S)) throw std::invalid_argument("option out of range");
         NearestRounding rn(true);
-        // digits of M / 2, M = [[Fr, -Fi], [Fi, Fr]], F = e^{-2 pi i k n / 16}: S - 1 balanced low base-128 digits
-        // and the leading remainder, most significant first
-        std::vector<int8_t> dig((size_t)S * 32 * 32);
-        for (int r = 0; r < 32; ++r)
-            for (int cc = 0; cc < 32; ++cc) {
-                const int kk = r & 15, nn = cc & 15;
-                dd c, sn;
-                cis((long long)kk * nn % 16, 16, c, sn);   // e^{+i theta}: Fr = c, Fi = -sn
-                dd v = (r < 16) == (cc < 16) ? c : (r < 16 ? sn : ddneg(sn));
-                v = dd{v.hi / 2, v.lo / 2};
-                const double hs = std::ldexp(v.hi, 7 * S), ls = std::ldexp(v.lo, 7 * S);
-                const double r1 = std::nearbyint(hs);
-                __int128 Z = (__int128)r1 + (__int128)std::nearbyint((hs - r1) + ls);
-                for (int j = S - 1; j >= 1; --j) {
-                    const int d = (int)((Z + 64) & 127) - 64;
-                    Z = (Z - d) >> 7;
-                    dig[((size_t)j * 32 + r) * 32 + cc] = (int8_t)d;
-                }
-                dig[(size_t)r * 32 + cc] = (int8_t)(int)Z;
-            }
-        // MFMA A layout: (slice, M-tile, lane) -> bytes b = A[16 mt + lane % 16][8 (lane / 16) + b]
-        std::vector<uint64_t> ad((size_t)S * 2 * 64);
-        for (int j = 0; j < S; ++j)
-            for (int mt = 0; mt < 2; ++mt)
-                for (int l = 0; l < 64; ++l) {
-                    uint64_t w = 0;
-                    for (int bb = 0; bb < 8; ++bb)
-                        w |= (uint64_t)(uint8_t)dig[((size_t)j * 32 + 16 * mt + l % 16) * 32 + 8 * (l / 16) + bb]
-                             << (8 * bb);
-                    ad[((size_t)j * 2 + mt) * 64 + l] = w;
-                }
+        const std::vector<uint64_t> ad = split_matrix_digits(S);
         hipStream_t s = static_cast<hipStream_t>(stream);
         DevBuf dad;
         dad.upload(ad.data(), ad.size() * sizeof(uint64_t), s);

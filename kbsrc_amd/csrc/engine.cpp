@@ -82,6 +82,66 @@ void SubFFT::init(int n, bool as_dd, hipStream_t s) {
     tw.upload(v.data(), v.size() * sizeof(double), s);
 }
 
+void SubFFT::init_split(hipStream_t s) {
+    if (tw16.d() || desc.nst < 2 || desc.st[0].r != 4 || desc.st[1].r != 4) return;
+    const int n = desc.n;
+    CisTable ct(n);
+    std::vector<dd> re, im;
+    for (int p = 0; p < n / 16; ++p)
+        for (int k = 1; k < 16; ++k) {
+            dd c, sn;
+            ct.get(-(long long)p * k, c, sn);
+            re.push_back(c);
+            im.push_back(sn);
+        }
+    const std::vector<double> v = pack_cx(re, im, true);
+    tw16.upload(v.data(), v.size() * sizeof(double), s);
+    hip_check(hipStreamSynchronize(s), "split twiddles");   // the host table is freed on return
+}
+
+void CFFTPlan::init_split(hipStream_t s) {
+    if (blue) {
+        inner->init_split(s);
+    } else if (four) {
+        s1.init_split(s);
+        s2.init_split(s);
+    } else {
+        sd.init_split(s);
+    }
+}
+
+std::vector<uint64_t> split_matrix_digits(int S) {
+    std::vector<int8_t> dig((size_t)S * 32 * 32);
+    for (int r = 0; r < 32; ++r)
+        for (int cc = 0; cc < 32; ++cc) {
+            const int kk = r & 15, nn = cc & 15;
+            dd c, sn;
+            cis((long long)kk * nn % 16, 16, c, sn);   // e^{+i theta}: Fr = c, Fi = -sn
+            dd v = (r < 16) == (cc < 16) ? c : (r < 16 ? sn : ddneg(sn));
+            v = dd{v.hi / 2, v.lo / 2};
+            const double hs = std::ldexp(v.hi, 7 * S), ls = std::ldexp(v.lo, 7 * S);
+            const double r1 = std::nearbyint(hs);
+            __int128 Z = (__int128)r1 + (__int128)std::nearbyint((hs - r1) + ls);
+            for (int j = S - 1; j >= 1; --j) {
+                const int d = (int)((Z + 64) & 127) - 64;
+                Z = (Z - d) >> 7;
+                dig[((size_t)j * 32 + r) * 32 + cc] = (int8_t)d;
+            }
+            dig[(size_t)r * 32 + cc] = (int8_t)(int)Z;
+        }
+    std::vector<uint64_t> ad((size_t)S * 2 * 64);
+    for (int j = 0; j < S; ++j)
+        for (int mt = 0; mt < 2; ++mt)
+            for (int l = 0; l < 64; ++l) {
+                uint64_t w = 0;
+                for (int bb = 0; bb < 8; ++bb)
+                    w |= (uint64_t)(uint8_t)dig[((size_t)j * 32 + 16 * mt + l % 16) * 32 + 8 * (l / 16) + bb]
+                         << (8 * bb);
+                ad[((size_t)j * 2 + mt) * 64 + l] = w;
+            }
+    return ad;
+}
+
 namespace {
 
 constexpr int kThreads = 256;
@@ -204,6 +264,13 @@ void run_pass(int policy, int ld, int st, PassArgs a, hipStream_t s, const char*
     const PassKey key{policy, a.sd.n, ld, st, a.nsub};
     bool known = false;
     PassChoice choice{};
+    if (a.split > 0 && a.tw16 && policy != POL_F64 && ld != LD_FOLD && st != ST_SPLIT) {   // the split form
+        e = launch_split_pass(policy, ld, st, a, s);
+        if (e != hipSuccess)
+            throw HipError(std::string(what) + " (split form, n=" + std::to_string(a.sd.n) + "): " +
+                           hipGetErrorString(e));
+        return;
+    }
     if (tunable && !t_tuning) {
         std::lock_guard<std::mutex> lock(g_tune_mu);
         const auto it = g_tuned.find(key);
@@ -237,6 +304,7 @@ void run_pass(int policy, int ld, int st, PassArgs a, hipStream_t s, const char*
 void fill_sub(PassArgs& a, const SubFFT& sf, const DeviceConsts& k, bool as_dd, int policy, int nsub) {
     a.sd = sf.desc;
     a.tw = sf.tw.d();
+    a.tw16 = sf.tw16.d();
     a.oddc = as_dd ? k.odd_dd.d() : k.odd_f64.d();
     a.nsub = nsub;
     a.G = pick_group(policy, sf.desc.n, nsub);
@@ -506,10 +574,24 @@ Engine::Engine(const Plan& p, const double* H, long long nH, int Jg, hipStream_t
     hip_check(hipStreamSynchronize(s), "engine set-up");   // H and the tables are in place; the caller may free H
 }
 
-void Engine::execute(const double* xx, double* out, hipStream_t s, PassTimes* times) {
+void Engine::execute(const double* xx, double* out, hipStream_t s, PassTimes* times, int split) {
     PassTimes local;
     const int pol_four = dd_ ? POL_DDS : POL_F64;
-    const bool fused = use_fixed() && env_flag("KBSRC_FUSED") && !Jg_ && fw_.c.four && iv_.c.four &&
+    if (split) {   // tables of the split form, made on first use
+        if (!dd_ || !split_slices_ok(split)) throw std::invalid_argument("option out of range");
+        fw_.c.init_split(s);
+        iv_.c.init_split(s);
+        if (!adig_.count(split)) {
+            const std::vector<uint64_t> ad = split_matrix_digits(split);
+            adig_[split].upload(ad.data(), ad.size() * sizeof(uint64_t), s);
+            hip_check(hipStreamSynchronize(s), "split matrix");
+        }
+        if (!g_[0].d()) {
+            g_[0].alloc((size_t)ts_ * sizeof(double) * tb_);
+            g_[1].alloc((size_t)ts_ * sizeof(double) * tb_);
+        }
+    }
+    const bool fused = !split && use_fixed() && env_flag("KBSRC_FUSED") && !Jg_ && fw_.c.four && iv_.c.four &&
                        has_fixed(pol_four, fw_.c.N2, LD_CPLX, ST_SPLIT) && has_fixed(pol_four, iv_.c.N1, LD_FOLD, ST_CPLX);
     RunCtx c;
     c.ts = ts_; c.k = &k_; c.dd = dd_; c.lane = true; c.s = s; c.times = times ? &local : nullptr;
@@ -517,6 +599,8 @@ void Engine::execute(const double* xx, double* out, hipStream_t s, PassTimes* ti
     a.gscr0 = g_[0].d();
     a.gscr1 = g_[1].d();
     a.tdesc = tdesc_;
+    a.split = split;
+    a.adig = split ? reinterpret_cast<const long*>(adig_[split].d()) : nullptr;
     a.nblk = p_.nblk;
     a.in_cstride = p_.xstride;
     a.in_bstride = (long long)p_.hop * p_.Q;

@@ -3,6 +3,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
+#include <map>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -55,7 +57,11 @@ struct DeviceConsts {
 struct SubFFT {
     SubDesc desc;
     DevBuf tw;   // stage twiddles in the plan's element format
+    DevBuf tw16;   // split form: the radix-16 stage's twiddles (dd elements), built by init_split
     void init(int n, bool as_dd, hipStream_t s);
+    // the split form's radix-16 stage twiddles w_n^{p k} at p * 15 + k - 1 (p < n / 16, k = 1..15), for lengths
+    // whose first two stages are radix 4; host dd arithmetic, round to nearest
+    void init_split(hipStream_t s);
 };
 
 // Complex FFT of length n as the reference computes it (src/dsp/fft.hpp:226-265): four-step N1 x N2 above
@@ -71,6 +77,7 @@ struct CFFTPlan {
     std::unique_ptr<CFFTPlan> inner;
     DevBuf bc, vf;   // chirp (n) and the inner FFT of the chirp sequence divided by m (m)
     void init(int n, bool as_dd, const DeviceConsts& k, hipStream_t s);
+    void init_split(hipStream_t s);   // SubFFT::init_split on every sub-FFT length of the plan
     long long work_elems() const { return blue ? m : n; }   // elements per task a run needs in each buffer
     bool needs_global() const;   // some sub-FFT too long for an LDS tile: passes go through global scratch
 };
@@ -136,7 +143,9 @@ public:
     Engine(const Plan& p, const double* H, long long nH, int Jg, hipStream_t s, long long ntask = 0,
            const long long* tdesc = nullptr, size_t buffer_budget = 0);
     // xx: device, ch x plan.xstride padded planar input. out: device, ch x plan.nout. times: optional per-pass timing.
-    void execute(const double* xx, double* out, hipStream_t s, PassTimes* times = nullptr);
+    // split > 0: the split-precision form (exact-mode plans only; not bit-exact): sub-FFTs whose first two stages
+    // are radix 4 run them as one int8-MFMA radix-16 stage with `split` base-128 slices per operand.
+    void execute(const double* xx, double* out, hipStream_t s, PassTimes* times = nullptr, int split = 0);
     long long tasks() const { return ntask_; }
     int batch() const { return tb_; }
     const Plan& plan() const { return p_; }
@@ -151,6 +160,12 @@ private:
     DeviceConsts k_;
     RFFTPlan fw_, iv_;
     DevBuf H_, w_[4], g_[2];
+    std::map<int, DevBuf> adig_;   // split form: the DFT matrix's slices per slice count
 };
+
+// The split form's 16-point DFT matrix as S base-128 slices in the MFMA A layout: M / 2, M = [[Fr, -Fi], [Fi, Fr]],
+// F = e^{-2 pi i k n / 16}, S - 1 balanced low digits and the leading remainder, most significant first; 64-bit
+// word (slice j, M-tile mt, lane l) holds bytes b = slice j of M[16 mt + l % 16][8 (l / 16) + b]. Round to nearest.
+std::vector<uint64_t> split_matrix_digits(int S);
 
 }  // namespace kbsrc::gpu

@@ -81,6 +81,12 @@ struct PassArgs {
     // column c = the sub-FFT index; the butterfly with index p here is butterfly c + pm * p of the long transform
     // and takes that one's stage twiddle from sd's tables (indexed for length N).
     int pm = 0;
+    // split > 0 (exact-mode plans only): the split-precision form. A sub-FFT whose first two stages are radix 4
+    // runs them as one radix-16 int8-MFMA stage with `split` base-128 slices per operand (adig: the DFT matrix's
+    // slices in the MFMA A layout; tw16: w_n^{p k} at p * 15 + k - 1, p < n / 16, k = 1..15), through gscr0/1.
+    int split = 0;
+    const double* tw16 = nullptr;
+    const long* adig = nullptr;
 };
 
 // Spectrum x H, bin map and inverse pre-split for one batch of tasks (src/fft.hpp:361-409,411-458;
@@ -201,10 +207,18 @@ hipError_t launch_circ(const double* h, long long Lk, long long n, double* r, hi
 // hi parts of n (hi, lo) pairs
 hipError_t launch_hi(const double* pairs, long long n, double* out, hipStream_t s);
 hipError_t launch_pack(PackArgs a, hipStream_t s);
-// Split-precision radix-16 DFT (int8 MFMA slices; not on the conversion path): nvec vectors of 16 complex dd (plan
+// Split-precision radix-16 DFT (int8 MFMA slices; not on the bit-exact path): nvec vectors of 16 complex dd (plan
 // element format) x -> y; adig: the S slices of M / 2 in the MFMA A layout, (slice, M-tile, lane) -> 8 bytes.
+bool split_slices_ok(int slices);   // the slice counts built: 4-8, 10, 12, 14, 15, 16
 hipError_t launch_split_dft16(const double* x, long long nvec, int slices, const long* adig, double* y,
                               hipStream_t s);
+// The same as the first stage of nsub sub-FFTs of length n (16 | n) stored one after another in x: y[t n + 16 p + k]
+// = w_n^{p k} DFT16(x[t n + p + (n / 16) m])[k] (tw16 as in PassArgs), dd elements.
+hipError_t launch_split16_stage(const double* x, double* y, int n, long long nsub, int slices, const double* tw16,
+                                const long* adig, hipStream_t s);
+// A sub-FFT pass in the split form (PassArgs::split): gather to gscr0, the radix-16 split stage, the remaining
+// stages as one LDS pass per class q + 16 j, scatter (four-step twiddle, renormalisation, output as the pass does).
+hipError_t launch_split_pass(int policy, int ld, int st, const PassArgs& a, hipStream_t s);
 hipError_t launch_cis_table(const TableArgs& a, hipStream_t s);
 // Bluestein chirp b[t] = cis(t^2 mod 2n, 2n), t < n, into bc (plan format) and the chirp sequence v (dd format,
 // m elements, zeroed by the caller): v[t] = v[m - t] = b[t] (src/dsp/fft.hpp:231-236)

@@ -126,14 +126,46 @@ void set_group(PassArgs& p, int policy) {   // sub-FFTs per workgroup within 48 
     p.ngrp = (p.nsub + p.G - 1) / p.G;
 }
 
-template <class A> hipError_t run_global(int policy, int ld, int st, const PassArgs& a, hipStream_t s) {
-    const long long n = a.sd.n, elems = (long long)a.ntask * a.nsub * n;
-    const dim3 block(256);
-    auto grid = [](long long items) { return dim3((unsigned)((items + 255) / 256)); };
-    if (ld == LD_REAL_F64) hipLaunchKernelGGL((k_gather<A, LD_REAL_F64>), grid(elems), block, 0, s, a);
-    else if (ld == LD_REAL_DD) hipLaunchKernelGGL((k_gather<A, LD_REAL_DD>), grid(elems), block, 0, s, a);
-    else if (ld == LD_CPLX) hipLaunchKernelGGL((k_gather<A, LD_CPLX>), grid(elems), block, 0, s, a);
+dim3 grid_of(long long items) { return dim3((unsigned)((items + 255) / 256)); }
+
+template <class A> hipError_t gather(int ld, const PassArgs& a, hipStream_t s) {
+    const long long elems = (long long)a.ntask * a.nsub * a.sd.n;
+    if (ld == LD_REAL_F64) hipLaunchKernelGGL((k_gather<A, LD_REAL_F64>), grid_of(elems), dim3(256), 0, s, a);
+    else if (ld == LD_REAL_DD) hipLaunchKernelGGL((k_gather<A, LD_REAL_DD>), grid_of(elems), dim3(256), 0, s, a);
+    else if (ld == LD_CPLX) hipLaunchKernelGGL((k_gather<A, LD_CPLX>), grid_of(elems), dim3(256), 0, s, a);
     else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+template <class A> hipError_t scatter(int st, const PassArgs& a, const double* in, hipStream_t s) {
+    const long long elems = (long long)a.ntask * a.nsub * a.sd.n;
+    if (st == ST_CPLX) hipLaunchKernelGGL((k_scatter<A, ST_CPLX>), grid_of(elems), dim3(256), 0, s, a, in);
+    else if (st == ST_OUT) hipLaunchKernelGGL((k_scatter<A, ST_OUT>), grid_of(elems), dim3(256), 0, s, a, in);
+    else return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+// the stages k.. of a's sub-FFT (R = the stride of stage k) as one LDS pass over its R classes q + R j of n / R
+// points, every (task, sub-FFT) transform stored contiguously in src and dst
+PassArgs tail_pass(const PassArgs& a, int k, int elt, const double* src, double* dst) {
+    const int R = a.sd.st[k].s, rest = a.sd.n / R;
+    const long long tdoubles = (long long)a.sd.n * elt;
+    PassArgs pb = a;
+    pb.sd.n = rest;
+    pb.sd.nst = a.sd.nst - k;
+    for (int i = k; i < a.sd.nst; ++i) pb.sd.st[i - k] = {a.sd.st[i].r, a.sd.st[i].m, a.sd.st[i].s / R, a.sd.st[i].tw};
+    pb.src = src; pb.src_tstride = tdoubles; pb.dst = dst; pb.dst_tstride = tdoubles;
+    pb.ntask = (int)((long long)a.ntask * a.nsub); pb.task0 = 0; pb.tdesc = nullptr;
+    pb.nsub = R; pb.lde = R; pb.ldg = 1; pb.ste = R; pb.stg = 1;
+    pb.twmode = 0; pb.renorm = 0; pb.pm = 0; pb.split = 0;
+    return pb;
+}
+
+template <class A> hipError_t run_global(int policy, int ld, int st, const PassArgs& a, hipStream_t s) {
+    const long long n = a.sd.n;
+    const dim3 block(256);
+    auto grid = grid_of;
+    if (hipError_t e = gather<A>(ld, a, s); e != hipSuccess) return e;
     double* in = a.gscr0;
     double* out = a.gscr1;
     const char* env = std::getenv("KBSRC_GLOBAL_PASSES");
@@ -153,12 +185,7 @@ template <class A> hipError_t run_global(int policy, int ld, int st, const PassA
         set_group(pa, policy);
         hipError_t e = launch_pass(policy, LD_CPLX, ST_CPLX, pa, 256, s);
         if (e != hipSuccess) return e;
-        PassArgs pb = pa;
-        pb.sd.n = rest;
-        pb.sd.nst = a.sd.nst - k;
-        for (int i = k; i < a.sd.nst; ++i) pb.sd.st[i - k] = {a.sd.st[i].r, a.sd.st[i].m, a.sd.st[i].s / R, a.sd.st[i].tw};
-        pb.src = a.gscr1; pb.dst = a.gscr0;
-        pb.nsub = R; pb.lde = R; pb.ldg = 1; pb.ste = R; pb.stg = 1; pb.pm = 0;
+        PassArgs pb = tail_pass(a, k, kElt<A>, a.gscr1, a.gscr0);
         set_group(pb, policy);
         e = launch_pass(policy, LD_CPLX, ST_CPLX, pb, 256, s);
         if (e != hipSuccess) return e;
@@ -169,13 +196,31 @@ template <class A> hipError_t run_global(int policy, int ld, int st, const PassA
             std::swap(in, out);
         }
     }
-    if (st == ST_CPLX) hipLaunchKernelGGL((k_scatter<A, ST_CPLX>), grid(elems), block, 0, s, a, (const double*)in);
-    else if (st == ST_OUT) hipLaunchKernelGGL((k_scatter<A, ST_OUT>), grid(elems), block, 0, s, a, (const double*)in);
-    else return hipErrorInvalidValue;
-    return hipGetLastError();
+    return scatter<A>(st, a, in, s);
 }
 
 }  // namespace
+
+hipError_t launch_split_pass(int policy, int ld, int st, const PassArgs& a, hipStream_t s) {
+    if (policy == POL_F64 || !a.gscr0 || !a.gscr1 || !a.tw16 || !a.adig || a.sd.nst < 2 || a.sd.st[0].r != 4 ||
+        a.sd.st[1].r != 4)
+        return hipErrorInvalidValue;
+    const bool dds = policy == POL_DDS;
+    hipError_t e = dds ? gather<DDS>(ld, a, s) : gather<DDA>(ld, a, s);
+    if (e != hipSuccess) return e;
+    const long long nsub = (long long)a.ntask * a.nsub;
+    e = launch_split16_stage(a.gscr0, a.gscr1, a.sd.n, nsub, a.split, a.tw16, a.adig, s);
+    if (e != hipSuccess) return e;
+    const double* res = a.gscr1;
+    if (a.sd.nst > 2) {
+        PassArgs pb = tail_pass(a, 2, 4, a.gscr1, a.gscr0);
+        set_group(pb, policy);
+        e = launch_pass(policy, LD_CPLX, ST_CPLX, pb, 256, s);
+        if (e != hipSuccess) return e;
+        res = a.gscr0;
+    }
+    return dds ? scatter<DDS>(st, a, res, s) : scatter<DDA>(st, a, res, s);
+}
 
 hipError_t launch_global(int policy, int ld, int st, const PassArgs& a, hipStream_t s) {
     if (!a.gscr0 || !a.gscr1) return hipErrorInvalidValue;

@@ -1,6 +1,7 @@
-// Split-precision (Ozaki) form of a radix-16 DFT stage on the int8 matrix cores, measured against the plain
-// double-double stage. Not on the conversion path: the engine computes every butterfly in dd, bit for bit as the
-// reference does; this form exists to measure what int8 MFMA slices buy at double-double accuracy.
+// Split-precision (Ozaki) form of a radix-16 DFT stage on the int8 matrix cores. Not on the bit-exact path: the
+// plain engine computes every butterfly in dd, bit for bit as the reference does; the split form of a conversion
+// runs the first two radix-4 stages of its sub-FFTs (those whose length 16 divides) as this stage instead, and a
+// standalone radix-16 DFT measures the stage against the plain dd one.
 //
 // y = F x for vectors x of 16 complex dd values, F[k][n] = e^{-2 pi i k n / 16}, as one real 32 x 32 product
 // [Re y; Im y] = M [Re x; Im x]. Each vector is scaled by a power of two to |z| < 1/2 and cut into S base-128
@@ -9,12 +10,14 @@
 // v_mfma_i32_16x16x32_i8 (exact int32 sums of 32 terms), summed per weight 2^{-7 (i + j)} in int32 (exact: at most
 // S products of |a|, |b| <= 65 over 32 terms), and the weighted sums are recombined exactly in 128-bit integers
 // and rounded once to dd. Error: the digits kept (about 2^{-7 S} of the vector's largest value) and the products
-// dropped.
+// dropped. FP64 dd keeps the stage's twiddle multiply.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
 
-#include "kernels.hpp"
+#include "bfly.hpp"
+
+#pragma clang fp contract(off)
 
 namespace kbsrc::gpu {
 
@@ -32,26 +35,13 @@ __device__ __forceinline__ __int128 to_fixed(double hi, double lo, int k) {
 
 // Wave layout of v_mfma_i32_16x16x32_i8 (checked on the device against an integer product): lane l holds A row
 // l % 16, k = 8 (l / 16) .. + 7; B column l % 16, same k; D column l % 16, rows 4 (l / 16) .. + 3.
+//
+// One wave, 16 vectors (column col = lane % 16 of the product). On entry lane l holds the 8 reals k = 8 q .. 8 q + 7
+// (q = l / 16) of its column's vector, k < 16: Re x[k], else Im x[k - 16], as (hi, lo). On return it holds
+// y[4 q + r] = (re[r], im[r]) for r = 0..3, as normalised dd. Every lane of the wave must call it.
 template <int S>
-__global__ __launch_bounds__(256) void k_split_dft16(const double* __restrict__ x, long long nvec,
-                                                     const long* __restrict__ adig, double* __restrict__ y) {
-    const int lane = threadIdx.x & 63, q = lane >> 4, col = lane & 15;
-    const long long v = ((long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16 + col;
-    const bool live = v < nvec;
-    // this lane's 8 reals of vector v: k = 8q .. 8q+7, k < 16: Re x[k], else Im x[k - 16]
-    double hi[8], lo[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int k = 8 * q + i, n = k & 15, part = k >> 4;
-        if (live) {
-            const double* e = x + (v * 16 + n) * 4;
-            hi[i] = e[part];
-            lo[i] = e[2 + part];
-        } else {
-            hi[i] = 0.0;
-            lo[i] = 0.0;
-        }
-    }
+__device__ __forceinline__ void split_core(const double (&hi)[8], const double (&lo)[8], const long* __restrict__ adig,
+                                           int lane, dd (&re)[4], dd (&im)[4]) {
     // the vector's scale: largest |hi| over its 32 reals (4 lanes share a column)
     double m = 0.0;
 #pragma unroll
@@ -93,7 +83,6 @@ __global__ __launch_bounds__(256) void k_split_dft16(const double* __restrict__ 
             acc[i + j][1] = __builtin_amdgcn_mfma_i32_16x16x32_i8(a1, b[j], acc[i + j][1], 0, 0, 0);
         }
     }
-    if (!live) return;
     // rows 16 mt + 4 q + r: exact sum over weights in 128 bits, one rounding to dd, scale back
     const int shift = 1 + e - 7 * (S + 1);
 #pragma unroll
@@ -105,11 +94,76 @@ __global__ __launch_bounds__(256) void k_split_dft16(const double* __restrict__ 
             for (int t = 0; t < S; ++t) V = V * 128 + acc[t][mt][r];
             const double h = (double)V;
             const double l = (double)(V - (__int128)h);
-            const int row = 16 * mt + 4 * q + r, k = row & 15, part = row >> 4;
-            double* o = y + (v * 16 + k) * 4;
-            o[part] = __builtin_ldexp(h, shift);
-            o[2 + part] = __builtin_ldexp(l, shift);
+            const dd v{__builtin_ldexp(h, shift), __builtin_ldexp(l, shift)};
+            if (mt == 0) re[r] = v;
+            else im[r] = v;
         }
+    }
+}
+
+// the 8 reals of lane l's share of vector x (16 dd elements {re.hi, im.hi, re.lo, im.lo}, element n at x + n * step
+// elements); zeros when the lane's column has no vector
+__device__ __forceinline__ void load_reals(const double* x, long long step, bool live, int q, double (&hi)[8],
+                                           double (&lo)[8]) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const int k = 8 * q + i, n = k & 15, part = k >> 4;
+        if (live) {
+            const double* e = x + n * step * 4;
+            hi[i] = e[part];
+            lo[i] = e[2 + part];
+        } else {
+            hi[i] = 0.0;
+            lo[i] = 0.0;
+        }
+    }
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_split_dft16(const double* __restrict__ x, long long nvec,
+                                                     const long* __restrict__ adig, double* __restrict__ y) {
+    const int lane = threadIdx.x & 63, q = lane >> 4, col = lane & 15;
+    const long long v = ((long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16 + col;
+    const bool live = v < nvec;
+    double hi[8], lo[8];
+    load_reals(x + (live ? v * 64 : 0), 1, live, q, hi, lo);
+    dd re[4], im[4];
+    split_core<S>(hi, lo, adig, lane, re, im);
+    if (!live) return;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        double* o = y + (v * 16 + 4 * q + r) * 4;
+        o[0] = re[r].hi;
+        o[1] = im[r].hi;
+        o[2] = re[r].lo;
+        o[3] = im[r].lo;
+    }
+}
+
+// The first two radix-4 Stockham stages of a sub-FFT of length n (16 | n) as one radix-16 stage: for each sub-FFT
+// t (elements t n .. t n + n - 1) and column p < n / 16, y[t n + 16 p + k] = w_n^{p k} DFT16(x[t n + p + (n / 16) m])[k],
+// the twiddle w_n^{p k} = tw16[p * 15 + k - 1] applied in dd (accurate add) for p k != 0.
+template <int S>
+__global__ __launch_bounds__(256) void k_split16_stage(const double* __restrict__ x, double* __restrict__ y, int n,
+                                                       long long nvec, const double* __restrict__ tw16,
+                                                       const long* __restrict__ adig) {
+    const int lane = threadIdx.x & 63, q = lane >> 4, col = lane & 15;
+    const long long v = ((long long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * 16 + col;
+    const bool live = v < nvec;
+    const int nv = n >> 4;
+    const long long t = live ? v / nv : 0;
+    const int p = live ? (int)(v - t * nv) : 0;
+    double hi[8], lo[8];
+    load_reals(x + (t * n + p) * 4, nv, live, q, hi, lo);
+    dd re[4], im[4];
+    split_core<S>(hi, lo, adig, lane, re, im);
+    if (!live) return;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int k = 4 * q + r;
+        cx<dd> z{re[r], im[r]};
+        if (p != 0 && k != 0) z = c_mul<DDA>(z, gld<DDA>(tw16, (long long)p * 15 + k - 1));
+        gst<DDA>(y, t * n + 16 * p + k, z);
     }
 }
 
@@ -120,7 +174,21 @@ template <int S> hipError_t launch_s(const double* x, long long nvec, const long
     return hipGetLastError();
 }
 
+template <int S>
+hipError_t launch_st(const double* x, double* y, int n, long long nsub, const double* tw16, const long* adig,
+                     hipStream_t s) {
+    const long long nvec = nsub * (n / 16), waves = (nvec + 15) / 16, blocks = (waves + 3) / 4;
+    if (blocks > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_split16_stage<S>), dim3((unsigned)blocks), dim3(256), 0, s, x, y, n, nvec, tw16, adig);
+    return hipGetLastError();
+}
+
 }  // namespace
+
+bool split_slices_ok(int slices) {
+    return (slices >= 4 && slices <= 8) || slices == 10 || slices == 12 || slices == 14 || slices == 15 ||
+           slices == 16;
+}
 
 hipError_t launch_split_dft16(const double* x, long long nvec, int slices, const long* adig, double* y,
                               hipStream_t s) {
@@ -136,6 +204,24 @@ hipError_t launch_split_dft16(const double* x, long long nvec, int slices, const
         case 14: return launch_s<14>(x, nvec, adig, y, s);
         case 15: return launch_s<15>(x, nvec, adig, y, s);
         case 16: return launch_s<16>(x, nvec, adig, y, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_split16_stage(const double* x, double* y, int n, long long nsub, int slices, const double* tw16,
+                                const long* adig, hipStream_t s) {
+    if (n < 16 || n % 16 || nsub < 1) return hipErrorInvalidValue;
+    switch (slices) {
+        case 4: return launch_st<4>(x, y, n, nsub, tw16, adig, s);
+        case 5: return launch_st<5>(x, y, n, nsub, tw16, adig, s);
+        case 6: return launch_st<6>(x, y, n, nsub, tw16, adig, s);
+        case 7: return launch_st<7>(x, y, n, nsub, tw16, adig, s);
+        case 8: return launch_st<8>(x, y, n, nsub, tw16, adig, s);
+        case 10: return launch_st<10>(x, y, n, nsub, tw16, adig, s);
+        case 12: return launch_st<12>(x, y, n, nsub, tw16, adig, s);
+        case 14: return launch_st<14>(x, y, n, nsub, tw16, adig, s);
+        case 15: return launch_st<15>(x, y, n, nsub, tw16, adig, s);
+        case 16: return launch_st<16>(x, y, n, nsub, tw16, adig, s);
         default: return hipErrorInvalidValue;
     }
 }
