@@ -5,12 +5,11 @@
 //
 // y = F x for vectors x of 16 complex dd values, F[k][n] = e^{-2 pi i k n / 16}, as one real 32 x 32 product
 // [Re y; Im y] = M [Re x; Im x]. Each vector is scaled by a power of two to |z| < 1/2 and cut into S base-128
-// digits, z = sum_j b_j 2^{-7 j}: the S - 1 low ones balanced in [-64, 63], the leading one in [-65, 65]; M / 2
-// likewise into a_i. The products a_i b_j with i + j <= S + 1 run on
-// v_mfma_i32_16x16x32_i8 (exact int32 sums of 32 terms), summed per weight 2^{-7 (i + j)} in int32 (exact: at most
-// S products of |a|, |b| <= 65 over 32 terms), and the weighted sums are recombined exactly in 128-bit integers
-// and rounded once to dd. Error: the digits kept (about 2^{-7 S} of the vector's largest value) and the products
-// dropped. FP64 dd keeps the stage's twiddle multiply.
+// digits in [-64, 64], z = sum_j b_j 2^{-7 (j + 1)}; M / 2 likewise into a_i (on the host). The products a_i b_j
+// with i + j <= S + 1 run on v_mfma_i32_16x16x32_i8 (exact int32 sums of 32 terms), summed per weight
+// 2^{-7 (i + j)} in int32 (exact: at most S products of |a|, |b| <= 65 over 32 terms), and the weighted sums are
+// recombined in FP64 (exact groups, one TwoSum chain) into dd. Error: the digits kept (about 2^{-7 S} of the
+// vector's largest value) and the products dropped. FP64 dd keeps the stage's twiddle multiply.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -24,14 +23,6 @@ namespace kbsrc::gpu {
 namespace {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
-
-// 128-bit integer nearest to (hi + lo) * 2^k (k such that the result fits)
-__device__ __forceinline__ __int128 to_fixed(double hi, double lo, int k) {
-    const double hs = __builtin_ldexp(hi, k), ls = __builtin_ldexp(lo, k);
-    const double r1 = __builtin_rint(hs);
-    const double rem = (hs - r1) + ls;
-    return (__int128)r1 + (__int128)__builtin_rint(rem);
-}
 
 // Wave layout of v_mfma_i32_16x16x32_i8 (checked on the device against an integer product): lane l holds A row
 // l % 16, k = 8 (l / 16) .. + 7; B column l % 16, same k; D column l % 16, rows 4 (l / 16) .. + 3.
@@ -51,50 +42,65 @@ __device__ __forceinline__ void split_core(const double (&hi)[8], const double (
     int ex = 0;
     if (m > 0) (void)__builtin_frexp(m, &ex);
     const int e = m > 0 ? ex + 1 : 0;   // |x| 2^{-e} < 1/2
-    // S base-128 digits of each real, most significant first, packed per slice: byte i of b[j] is digit j of
-    // real 8q + i
+    // S base-128 digits of each real v = (hi + lo) 2^{-e}, most significant first, by nearest-integer peeling:
+    // t = 128 h, d = rint(t), h = t - d, all exact, so v = sum_j d_j 128^{-(j+1)} + remainder and every |d_j| <= 64.
+    // The low part joins the remainder by an exact TwoSum every 6 digits (before its scaled size reaches 2^-12);
+    // the last digit is rounded, so the remainder dropped is at most (1/2 + 2^-19) 128^{-S}. Byte i of b[j] is
+    // digit j of real 8q + i.
     long b[S];
 #pragma unroll
     for (int j = 0; j < S; ++j) b[j] = 0;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        __int128 Z = to_fixed(hi[i], lo[i], 7 * S - e);
+        double h = __builtin_ldexp(hi[i], -e), l = __builtin_ldexp(lo[i], -e);
 #pragma unroll
-        for (int j = S - 1; j >= 1; --j) {
-            const int d = (int)((Z + 64) & 127) - 64;
-            Z = (Z - d) >> 7;
-            b[j] |= (long)(uint8_t)(int8_t)d << (8 * i);
-        }
-        b[0] |= (long)(uint8_t)(int8_t)(int)Z << (8 * i);
-    }
-    // products a_i b_j, i + j <= S + 1 (1-based), summed per weight t = i + j
-    v4i acc[S][2];
-#pragma unroll
-    for (int t = 0; t < S; ++t) {
-        acc[t][0] = v4i{0, 0, 0, 0};
-        acc[t][1] = v4i{0, 0, 0, 0};
-    }
-#pragma unroll
-    for (int i = 0; i < S; ++i) {
-        const long a0 = adig[(i * 2 + 0) * 64 + lane], a1 = adig[(i * 2 + 1) * 64 + lane];
-#pragma unroll
-        for (int j = 0; i + j + 2 <= S + 1; ++j) {
-            acc[i + j][0] = __builtin_amdgcn_mfma_i32_16x16x32_i8(a0, b[j], acc[i + j][0], 0, 0, 0);
-            acc[i + j][1] = __builtin_amdgcn_mfma_i32_16x16x32_i8(a1, b[j], acc[i + j][1], 0, 0, 0);
+        for (int j = 0; j < S; ++j) {
+            if (j > 0 && j % 6 == 0) {
+                const dd z = knuth2(h, __builtin_ldexp(l, 42));
+                h = z.hi;
+                l = z.lo;
+            }
+            const double t = h * 128.0;
+            const double d = __builtin_rint(t);
+            h = t - d;
+            b[j] |= (long)(uint8_t)(int8_t)(int)d << (8 * i);
         }
     }
-    // rows 16 mt + 4 q + r: exact sum over weights in 128 bits, one rounding to dd, scale back
-    const int shift = 1 + e - 7 * (S + 1);
-#pragma unroll
+    // products a_i b_j, i + j <= S + 1 (1-based), summed per weight t = i + j, one M-tile (real, then imaginary
+    // rows) at a time; acc[t] weighs 2^{1 + e - 7 (t + 2)} (M / 2 and v carry 2^{-1} and 2^{-e})
+#pragma nounroll
     for (int mt = 0; mt < 2; ++mt) {
+        v4i acc[S];
+#pragma unroll
+        for (int t = 0; t < S; ++t) acc[t] = v4i{0, 0, 0, 0};
+#pragma unroll
+        for (int i = 0; i < S; ++i) {
+            const long a = adig[(i * 2 + mt) * 64 + lane];
+#pragma unroll
+            for (int j = 0; i + j + 2 <= S + 1; ++j)
+                acc[i + j] = __builtin_amdgcn_mfma_i32_16x16x32_i8(a, b[j], acc[i + j], 0, 0, 0);
+        }
+        // exact groups of 4 weights (|G| < 2^43, every Horner step exact in FP64), then an exact TwoSum chain with
+        // the error terms summed: the dd result is within ~2^-105 of the exact sum
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            __int128 V = 0;
+            double s = 0.0, err = 0.0;
 #pragma unroll
-            for (int t = 0; t < S; ++t) V = V * 128 + acc[t][mt][r];
-            const double h = (double)V;
-            const double l = (double)(V - (__int128)h);
-            const dd v{__builtin_ldexp(h, shift), __builtin_ldexp(l, shift)};
+            for (int t0 = 0; t0 < S; t0 += 4) {
+                const int tn = t0 + 4 < S ? t0 + 4 : S;
+                double G = (double)acc[t0][r];
+#pragma unroll
+                for (int t = t0 + 1; t < tn; ++t) G = __builtin_fma(G, 128.0, (double)acc[t][r]);
+                const double g = __builtin_ldexp(G, 1 + e - 7 * (tn + 1));
+                if (t0 == 0) {
+                    s = g;
+                } else {
+                    const dd z = knuth2(s, g);
+                    s = z.hi;
+                    err += z.lo;
+                }
+            }
+            const dd v = quick2(s, err);
             if (mt == 0) re[r] = v;
             else im[r] = v;
         }
