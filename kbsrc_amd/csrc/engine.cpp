@@ -114,10 +114,12 @@ std::vector<uint64_t> split_matrix_digits(int S) {
     std::vector<int8_t> dig((size_t)S * 32 * 32);
     for (int r = 0; r < 32; ++r)
         for (int cc = 0; cc < 32; ++cc) {
-            const int kk = r & 15, nn = cc & 15;
+            // row r: Re y[r] (r < 16) or Im y[r - 16]; column cc: Re x[cc / 2] (cc even) or Im x[cc / 2]
+            const int kk = r & 15, nn = cc >> 1;
+            const bool row_re = r < 16, col_re = (cc & 1) == 0;
             dd c, sn;
             cis((long long)kk * nn % 16, 16, c, sn);   // e^{+i theta}: Fr = c, Fi = -sn
-            dd v = (r < 16) == (cc < 16) ? c : (r < 16 ? sn : ddneg(sn));
+            dd v = row_re == col_re ? c : (row_re ? sn : ddneg(sn));
             v = dd{v.hi / 2, v.lo / 2};
             const double hs = std::ldexp(v.hi, 7 * S), ls = std::ldexp(v.lo, 7 * S);
             const double r1 = std::nearbyint(hs);

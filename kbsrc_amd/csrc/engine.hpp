@@ -163,9 +163,10 @@ private:
     std::map<int, DevBuf> adig_;   // split form: the DFT matrix's slices per slice count
 };
 
-// The split form's 16-point DFT matrix as S base-128 slices in the MFMA A layout: M / 2, M = [[Fr, -Fi], [Fi, Fr]],
-// F = e^{-2 pi i k n / 16}, S - 1 balanced low digits and the leading remainder, most significant first; 64-bit
-// word (slice j, M-tile mt, lane l) holds bytes b = slice j of M[16 mt + l % 16][8 (l / 16) + b]. Round to nearest.
+// The split form's 16-point DFT matrix as S base-128 slices in the MFMA A layout: M / 2, M the real form of
+// F = e^{-2 pi i k n / 16} with rows (Re y[0..15], Im y[0..15]) and columns (Re x[0], Im x[0], Re x[1], ...),
+// S - 1 balanced low digits and the leading remainder, most significant first; 64-bit word (slice j, M-tile mt,
+// lane l) holds bytes b = slice j of M[16 mt + l % 16][8 (l / 16) + b]. Round to nearest.
 std::vector<uint64_t> split_matrix_digits(int S);
 
 }  // namespace kbsrc::gpu

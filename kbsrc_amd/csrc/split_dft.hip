@@ -4,7 +4,7 @@
 // standalone radix-16 DFT measures the stage against the plain dd one.
 //
 // y = F x for vectors x of 16 complex dd values, F[k][n] = e^{-2 pi i k n / 16}, as one real 32 x 32 product
-// [Re y; Im y] = M [Re x; Im x]. Each vector is scaled by a power of two to |z| < 1/2 and cut into S base-128
+// [Re y; Im y] = M x' with x' = (Re x[0], Im x[0], Re x[1], ...). Each vector is scaled by a power of two to |z| < 1/2 and cut into S base-128
 // digits in [-64, 64], z = sum_j b_j 2^{-7 (j + 1)}; M / 2 likewise into a_i (on the host). The products a_i b_j
 // with i + j <= S + 1 run on v_mfma_i32_16x16x32_i8 (exact int32 sums of 32 terms), summed per weight
 // 2^{-7 (i + j)} in int32 (exact: at most S products of |a|, |b| <= 65 over 32 terms), and the weighted sums are
@@ -28,7 +28,7 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 // l % 16, k = 8 (l / 16) .. + 7; B column l % 16, same k; D column l % 16, rows 4 (l / 16) .. + 3.
 //
 // One wave, 16 vectors (column col = lane % 16 of the product). On entry lane l holds the 8 reals k = 8 q .. 8 q + 7
-// (q = l / 16) of its column's vector, k < 16: Re x[k], else Im x[k - 16], as (hi, lo). On return it holds
+// (q = l / 16) of its column's vector, real k = Re x[k / 2] for even k, Im x[k / 2] for odd k, as (hi, lo). On return it holds
 // y[4 q + r] = (re[r], im[r]) for r = 0..3, as normalised dd. Every lane of the wave must call it.
 template <int S>
 __device__ __forceinline__ void split_core(const double (&hi)[8], const double (&lo)[8], const long* __restrict__ adig,
@@ -108,19 +108,21 @@ __device__ __forceinline__ void split_core(const double (&hi)[8], const double (
 }
 
 // the 8 reals of lane l's share of vector x (16 dd elements {re.hi, im.hi, re.lo, im.lo}, element n at x + n * step
-// elements); zeros when the lane's column has no vector
+// elements): elements 4 q .. 4 q + 3, re and im, k = 2 (n - 4 q) + part, two 16-byte loads each; zeros when the
+// lane's column has no vector
 __device__ __forceinline__ void load_reals(const double* x, long long step, bool live, int q, double (&hi)[8],
                                            double (&lo)[8]) {
 #pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        const int k = 8 * q + i, n = k & 15, part = k >> 4;
+    for (int i = 0; i < 4; ++i) {
         if (live) {
-            const double* e = x + n * step * 4;
-            hi[i] = e[part];
-            lo[i] = e[2 + part];
+            const double2* e = reinterpret_cast<const double2*>(x + (4 * q + i) * step * 4);
+            const double2 h = e[0], l = e[1];
+            hi[2 * i] = h.x;
+            hi[2 * i + 1] = h.y;
+            lo[2 * i] = l.x;
+            lo[2 * i + 1] = l.y;
         } else {
-            hi[i] = 0.0;
-            lo[i] = 0.0;
+            hi[2 * i] = hi[2 * i + 1] = lo[2 * i] = lo[2 * i + 1] = 0.0;
         }
     }
 }
@@ -137,13 +139,7 @@ __global__ __launch_bounds__(256) void k_split_dft16(const double* __restrict__ 
     split_core<S>(hi, lo, adig, lane, re, im);
     if (!live) return;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        double* o = y + (v * 16 + 4 * q + r) * 4;
-        o[0] = re[r].hi;
-        o[1] = im[r].hi;
-        o[2] = re[r].lo;
-        o[3] = im[r].lo;
-    }
+    for (int r = 0; r < 4; ++r) gst<DDA>(y, v * 16 + 4 * q + r, cx<dd>{re[r], im[r]});
 }
 
 // The first two radix-4 Stockham stages of a sub-FFT of length n (16 | n) as one radix-16 stage: for each sub-FFT
