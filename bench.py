@@ -256,6 +256,26 @@ def batch_signal(job, max_frames=None):
     return x
 
 
+def graph_timing(args, owner, d_src, d_out):
+    """Time replays of the execute captured as a HIP graph (Plan.capture / Batch.capture) into a second output
+    buffer, and check that they give d_out's bits (d_out holds an eager execute's output)."""
+    import torch
+    alt = torch.empty_like(d_out)
+    g = owner.capture(d_src.data_ptr(), alt.data_ptr())
+    for _ in range(max(1, args.warmup)):
+        g.launch()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        g.launch()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    same = bool(torch.equal(alt, d_out))
+    g.close()
+    del alt
+    return ms, same
+
+
 def bench_batch(args):
     import numpy as np
     import torch
@@ -296,6 +316,9 @@ def bench_batch(args):
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
     value = out_samples / (ms / 1e3) / 1e6
     print(f"batched: {ms:.2f} ms per batch -> {value:.1f} output Msamples/s", file=sys.stderr, flush=True)
+    graph_ms, graph_same = graph_timing(args, b, d_in, d_out)
+    print(f"batched, captured graph replay: {graph_ms:.2f} ms per batch; same output: {graph_same}", file=sys.stderr,
+          flush=True)
 
     # the same jobs one plan at a time over the same staged input (a job's padded layout is its plan's)
     ref_out = torch.empty_like(d_out)
@@ -364,6 +387,8 @@ def bench_batch(args):
                                "both ways, half noise half 8 sinusoids, exact mode, len 0.2, beta 40, LPC on",
                    "jobs": len(jobs), "shapes": b.groups, "input_samples": in_samples, "output_samples": out_samples,
                    "parallelism": "single GPU"},
+        "graph_ms_per_step": round(graph_ms, 3),
+        "graph_outputs_identical": graph_same,
         "one_plan_at_a_time_ms": round(single_ms, 3),
         "batch_speedup": round(single_ms / ms, 3),
         "outputs_identical": bool(same),
@@ -630,6 +655,8 @@ def main():
     plan.execute(xx.data_ptr(), out.data_ptr(), stream)
     host = out.cpu()
     e2e_ms = (time.perf_counter() - t0) * 1e3
+    graph_ms, graph_same = graph_timing(args, plan, xx, out)
+    print(f"captured graph replay: {graph_ms:.3f} ms/step; same output: {graph_same}", file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu and args.mode == "exact":
@@ -669,6 +696,8 @@ def main():
         "design_s": round(t_design, 3),
         "stage_s": round(t_stage, 3),
         "e2e_engine_plus_d2h_ms": round(e2e_ms, 3),
+        "graph_ms_per_step": round(graph_ms, 4),
+        "graph_outputs_identical": graph_same,
         "tuned": bool(args.tune),
     }
     if cpu:

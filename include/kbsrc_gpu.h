@@ -127,6 +127,17 @@ int kbsrc_batch_execute(kbsrc_batch* batch, const double* d_in, double* d_out, v
 
 void kbsrc_batch_destroy(kbsrc_batch* batch);
 
+/* ---- Captured executes (no counterpart in the reference: its engine is host code). One plan or batch execute on
+ * fixed device buffers, captured once into an instantiated HIP graph and replayed with one launch: for serving
+ * loops that convert into the same buffers again and again. The graph uses the plan's (batch's) work buffers, so
+ * it must not run concurrently with another execute or replay of the same plan, and the plan must outlive it.
+ * Replays give the bits of kbsrc_plan_execute / kbsrc_batch_execute. */
+typedef struct kbsrc_graph kbsrc_graph;
+int kbsrc_plan_capture(kbsrc_plan* plan, const double* d_padded, double* d_out, kbsrc_graph** graph);
+int kbsrc_batch_capture(kbsrc_batch* batch, const double* d_in, double* d_out, kbsrc_graph** graph);
+int kbsrc_graph_launch(kbsrc_graph* graph, void* stream);   /* asynchronous on stream */
+void kbsrc_graph_destroy(kbsrc_graph* graph);
+
 /* ---- Measured kernel shapes (no counterpart in the reference). Runs the plan or batch once, as its execute call
  * does (same outputs), timing for every sub-FFT pass not measured before in this process its candidate shapes
  * (registered specialised variants; generic group sizes and thread counts) and keeping the fastest for every later

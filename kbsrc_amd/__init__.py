@@ -21,7 +21,7 @@ except ImportError:
 __all__ = ["resample", "output_frames", "Plan", "Stats", "version", "library_path", "FAST", "EXACT",
            "KbsrcError", "PLAN_FIELDS", "kernel_taps", "kernel_spectrum", "pack", "PACK_FORMATS", "Batch",
            "resample_batch", "tuned_shapes",
-           "split_dft16", "plain_dft16"]
+           "split_dft16", "plain_dft16", "Graph"]
 
 FAST, EXACT, LONG_DOUBLE = 0, 1, 2
 OK, EINVAL, ERUNTIME, ENOMEM = 0, 1, 2, 3
@@ -80,6 +80,10 @@ def _load():
         "kbsrc_batch_destroy": (None, [P]),
         "kbsrc_plan_tune": (I, [P, P, P, P]),
         "kbsrc_batch_tune": (I, [P, P, P, P]),
+        "kbsrc_plan_capture": (I, [P, P, P, ctypes.POINTER(P)]),
+        "kbsrc_batch_capture": (I, [P, P, P, ctypes.POINTER(P)]),
+        "kbsrc_graph_launch": (I, [P, P]),
+        "kbsrc_graph_destroy": (None, [P]),
         "kbsrc_tuned_shapes": (I, []),
         "kbsrc_split_dft16": (I, [P, LL, I, P, P]),
         "kbsrc_plain_dft16": (I, [P, LL, P, P]),
@@ -195,6 +199,29 @@ class _Job(ctypes.Structure):
                 ("rate_out", ctypes.c_int)]
 
 
+class Graph:
+    """One plan or batch execute on fixed device buffers, captured as a HIP graph (kbsrc_*_capture); ``launch``
+    replays it with one call. Keep the plan or batch alive while the graph exists."""
+
+    def __init__(self, handle, owner):
+        self._h = handle
+        self._owner = owner
+
+    def launch(self, stream: int = 0):
+        _check(_lib.kbsrc_graph_launch(self._h, ctypes.c_void_p(stream)))
+
+    def close(self):
+        if self._h:
+            _lib.kbsrc_graph_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Batch:
     """Many jobs in one pair of device buffers (kbsrc_batch_*): jobs = [(frames, channels, rate_in, rate_out)],
     one set of options. ``layout[j]`` = dict(in_offset, in_doubles, out_offset, out_doubles, nout, group) in
@@ -231,6 +258,12 @@ class Batch:
     def execute(self, d_in: int, d_out: int, stream: int = 0):
         _check(_lib.kbsrc_batch_execute(self._h, ctypes.c_void_p(d_in), ctypes.c_void_p(d_out),
                                         ctypes.c_void_p(stream)))
+
+    def capture(self, d_in: int, d_out: int) -> Graph:
+        """The batch execute on these buffers as a replayable HIP graph."""
+        g = ctypes.c_void_p()
+        _check(_lib.kbsrc_batch_capture(self._h, ctypes.c_void_p(d_in), ctypes.c_void_p(d_out), ctypes.byref(g)))
+        return Graph(g, self)
 
     def close(self):
         if self._h:
@@ -298,6 +331,12 @@ class Plan:
     def execute(self, d_padded: int, d_out: int, stream: int = 0):
         _check(_lib.kbsrc_plan_execute(self._h, ctypes.c_void_p(d_padded), ctypes.c_void_p(d_out),
                                        ctypes.c_void_p(stream)))
+
+    def capture(self, d_padded: int, d_out: int) -> Graph:
+        """The execute on these buffers as a replayable HIP graph."""
+        g = ctypes.c_void_p()
+        _check(_lib.kbsrc_plan_capture(self._h, ctypes.c_void_p(d_padded), ctypes.c_void_p(d_out), ctypes.byref(g)))
+        return Graph(g, self)
 
     def execute_split(self, d_padded: int, d_out: int, slices: int, stream: int = 0):
         """The split-precision form (exact-mode plans): sub-FFTs whose first two stages are radix 4 run them as one

@@ -7,6 +7,7 @@
 #include <numeric>
 #include <cstdint>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <new>
 #include <stdexcept>
@@ -83,6 +84,19 @@ template <class F> int guarded(F&& f) {
 }
 
 }  // namespace
+
+// One captured execute (a plan's or a batch's) as an instantiated HIP graph.
+struct kbsrc_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    kbsrc_graph() = default;
+    kbsrc_graph(const kbsrc_graph&) = delete;
+    kbsrc_graph& operator=(const kbsrc_graph&) = delete;
+    ~kbsrc_graph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
 
 struct kbsrc_plan {
     Plan p;
@@ -527,25 +541,80 @@ int kbsrc_batch_stage(kbsrc_batch* b, int job, const double* interleaved, int on
     });
 }
 
+namespace {
+
+// every group's engine on the batch's streams, forked from and joined back into s
+void batch_run(kbsrc_batch* b, const double* d_in, double* d_out, hipStream_t s) {
+    const int ns = (int)b->streams.size();
+    hip_check(hipEventRecord(b->events[ns], s), "batch fork");
+    for (int i = 0; i < ns; ++i) hip_check(hipStreamWaitEvent(b->streams[i], b->events[ns], 0), "batch fork");
+    for (size_t g = 0; g < b->groups.size(); ++g)   // groups round-robin over the batch's streams
+        b->groups[g].eng->execute(d_in, d_out, b->streams[g % ns]);
+    for (const kbsrc_batch::Job& J : b->jobs)
+        if (J.p.equal && J.out_len > 0)
+            hip_check(hipMemcpyAsync(d_out + J.out_off, d_in + J.in_off, sizeof(double) * J.out_len,
+                                     hipMemcpyDeviceToDevice, b->streams[0]), "copy");
+    for (int i = 0; i < ns; ++i) {
+        hip_check(hipEventRecord(b->events[i], b->streams[i]), "batch join");
+        hip_check(hipStreamWaitEvent(s, b->events[i], 0), "batch join");
+    }
+}
+
+// the work `run` enqueues on a private stream, captured into an instantiated HIP graph
+kbsrc_graph* capture(const std::function<void(hipStream_t)>& run) {
+    hipStream_t cs = nullptr;
+    hip_check(hipStreamCreateWithFlags(&cs, hipStreamNonBlocking), "capture stream");
+    auto g = std::make_unique<kbsrc_graph>();
+    hipError_t e = hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        try {
+            run(cs);
+        } catch (...) {
+            hipGraph_t partial = nullptr;
+            (void)hipStreamEndCapture(cs, &partial);
+            if (partial) (void)hipGraphDestroy(partial);
+            (void)hipStreamDestroy(cs);
+            throw;
+        }
+        e = hipStreamEndCapture(cs, &g->graph);
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    (void)hipStreamDestroy(cs);
+    hip_check(e, "graph capture");
+    return g.release();
+}
+
+}  // namespace
+
 int kbsrc_batch_execute(kbsrc_batch* b, const double* d_in, double* d_out, void* stream) {
     return guarded([&] {
         if (!b || !d_in || !d_out) throw std::invalid_argument("bad input buffer");
-        hipStream_t s = static_cast<hipStream_t>(stream);
-        const int ns = (int)b->streams.size();
-        hip_check(hipEventRecord(b->events[ns], s), "batch fork");
-        for (int i = 0; i < ns; ++i) hip_check(hipStreamWaitEvent(b->streams[i], b->events[ns], 0), "batch fork");
-        for (size_t g = 0; g < b->groups.size(); ++g)   // groups round-robin over the batch's streams
-            b->groups[g].eng->execute(d_in, d_out, b->streams[g % ns]);
-        for (const kbsrc_batch::Job& J : b->jobs)
-            if (J.p.equal && J.out_len > 0)
-                hip_check(hipMemcpyAsync(d_out + J.out_off, d_in + J.in_off, sizeof(double) * J.out_len,
-                                         hipMemcpyDeviceToDevice, b->streams[0]), "copy");
-        for (int i = 0; i < ns; ++i) {
-            hip_check(hipEventRecord(b->events[i], b->streams[i]), "batch join");
-            hip_check(hipStreamWaitEvent(s, b->events[i], 0), "batch join");
-        }
+        batch_run(b, d_in, d_out, static_cast<hipStream_t>(stream));
     });
 }
+
+int kbsrc_plan_capture(kbsrc_plan* pl, const double* d_padded, double* d_out, kbsrc_graph** graph) {
+    return guarded([&] {
+        if (!pl || !pl->eng || !d_padded || !d_out || !graph) throw std::invalid_argument("bad input buffer");
+        *graph = capture([&](hipStream_t cs) { pl->eng->execute(d_padded, d_out, cs); });
+    });
+}
+
+int kbsrc_batch_capture(kbsrc_batch* b, const double* d_in, double* d_out, kbsrc_graph** graph) {
+    return guarded([&] {
+        if (!b || !d_in || !d_out || !graph) throw std::invalid_argument("bad input buffer");
+        *graph = capture([&](hipStream_t cs) { batch_run(b, d_in, d_out, cs); });
+    });
+}
+
+int kbsrc_graph_launch(kbsrc_graph* g, void* stream) {
+    return guarded([&] {
+        if (!g || !g->exec) throw std::invalid_argument("bad input buffer");
+        hip_check(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)), "graph launch");
+    });
+}
+
+void kbsrc_graph_destroy(kbsrc_graph* g) { delete g; }
 
 void kbsrc_batch_destroy(kbsrc_batch* b) { delete b; }
 
