@@ -88,3 +88,18 @@ def test_split_form_errors_per_block(fi, fo):
     assert worst[8] < 2.0 ** -44
     assert worst[15] <= 2.0 ** -52 and diff[15] <= 2.0 ** -52
     assert diff[6] > 2.0 ** -60   # the split form really ran: 6 slices cannot match dd
+
+
+def test_split_form_rejections():
+    """The split form needs an exact-mode plan and a built slice count; a rejected call names the reason."""
+    import torch
+    k = gpu()
+    x = np.random.default_rng(3).uniform(-0.5, 0.5, 48000)
+    for mode, slices in (("fast", 8), ("exact", 3), ("exact", 9), ("exact", 17), ("exact", -1)):
+        plan = k.Plan(len(x), 1, 48000, 44100, mode)
+        d_pad = torch.zeros(plan.info["xstride"], dtype=torch.float64, device="cuda")
+        d_out = torch.empty(plan.info["nout"], dtype=torch.float64, device="cuda")
+        plan.stage(x, d_pad.data_ptr())
+        with pytest.raises(ValueError, match="option out of range"):
+            plan.execute_split(d_pad.data_ptr(), d_out.data_ptr(), slices)
+        plan.close()
