@@ -147,9 +147,10 @@ int kbsrc_batch_tune(kbsrc_batch* batch, const double* d_in, double* d_out, void
 int kbsrc_tuned_shapes(void);   /* pass shapes measured so far in this process */
 
 /* ---- Split-precision (Ozaki) radix-16 DFT on the int8 matrix cores, beside the plain dd pass (no counterpart in the
- * reference; not used by the conversion, which is bit-exact dd). nvec vectors of 16 complex dd values in device
- * memory, element format {re.hi, im.hi, re.lo, im.lo}; y = forward DFT of each. slices: 4-8, 10, 12, 14, 15, 16 base-128
- * digits per operand. Synchronous. */
+ * reference; the bit-exact conversion computes in dd; kbsrc_plan_execute_split uses this stage). nvec vectors of 16
+ * complex dd values in device memory, element format {re.hi, im.hi, re.lo, im.lo}; y = forward DFT of each.
+ * slices: 4-8, 10, 12, 14, 15, 16 base-128 digits per operand. Synchronous; the tables are made on the first call
+ * per device (and slice count) and kept for the process. */
 int kbsrc_split_dft16(const double* d_x, long long nvec, int slices, double* d_y, void* stream);
 int kbsrc_plain_dft16(const double* d_x, long long nvec, double* d_y, void* stream);
 

@@ -646,6 +646,7 @@ int kbsrc_tuned_shapes(void) { return tuned_shapes(); }
 int kbsrc_plain_dft16(const double* d_x, long long nvec, double* d_y, void* stream) {
     return guarded([&] {
         if (nvec < 0 || (nvec > 0 && (!d_x || !d_y))) throw std::invalid_argument("bad input buffer");
+        NearestRounding rn(true);   // the stage tables are made in host dd arithmetic on first use
         plain_dft16(d_x, nvec, d_y, static_cast<hipStream_t>(stream));
     });
 }
@@ -656,11 +657,8 @@ int kbsrc_split_dft16(const double* d_x, long long nvec, int slices, double* d_y
         const int S = slices;
         if (!split_slices_ok(S)) throw std::invalid_argument("option out of range");
         NearestRounding rn(true);
-        const std::vector<uint64_t> ad = split_matrix_digits(S);
         hipStream_t s = static_cast<hipStream_t>(stream);
-        DevBuf dad;
-        dad.upload(ad.data(), ad.size() * sizeof(uint64_t), s);
-        hip_check(launch_split_dft16(d_x, nvec, S, reinterpret_cast<const long*>(dad.d()), d_y, s), "split radix-16");
+        hip_check(launch_split_dft16(d_x, nvec, S, split_dft16_digits(S, s), d_y, s), "split radix-16");
         hip_check(hipStreamSynchronize(s), "split radix-16 sync");
     });
 }

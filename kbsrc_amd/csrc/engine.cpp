@@ -486,13 +486,51 @@ void run_cfft(const CFFTPlan& f, const RunCtx& c, int ld, const double* src, int
     }
 }
 
+namespace {
+
+// tables made once per device and kept for the life of the process (never freed: no teardown order to get wrong)
+struct Dft16Tables {
+    DeviceConsts k;
+    SubFFT sf;
+    std::map<int, DevBuf> adig;   // split matrix slices per slice count
+};
+Dft16Tables& dft16_tables(hipStream_t s) {
+    static std::mutex mu;
+    static auto* per_device = new std::map<int, Dft16Tables*>();
+    int dev = 0;
+    hip_check(hipGetDevice(&dev), "device");
+    std::lock_guard<std::mutex> lock(mu);
+    Dft16Tables*& t = (*per_device)[dev];
+    if (!t) {
+        auto fresh = std::make_unique<Dft16Tables>();
+        fresh->k.init(s);
+        fresh->sf.init(16, true, s);
+        t = fresh.release();
+    }
+    return *t;
+}
+
+}  // namespace
+
+const long* split_dft16_digits(int S, hipStream_t s) {
+    Dft16Tables& t = dft16_tables(s);
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lock(mu);
+    DevBuf& b = t.adig[S];
+    if (!b.d()) {
+        const std::vector<uint64_t> ad = split_matrix_digits(S);
+        b.upload(ad.data(), ad.size() * sizeof(uint64_t), s);
+        hip_check(hipStreamSynchronize(s), "split matrix");
+    }
+    return reinterpret_cast<const long*>(b.d());
+}
+
 void plain_dft16(const double* x, long long nvec, double* y, hipStream_t s) {
     if (nvec < 1) return;
     if (nvec > 0x7fffffffLL / 16) throw std::invalid_argument("bad input buffer");
-    DeviceConsts k;
-    k.init(s);
-    SubFFT sf;
-    sf.init(16, true, s);
+    const Dft16Tables& t = dft16_tables(s);
+    const DeviceConsts& k = t.k;
+    const SubFFT& sf = t.sf;
     PassArgs a;
     fill_sub(a, sf, k, true, POL_DDS, (int)nvec);
     a.ntask = 1;

@@ -123,6 +123,8 @@ int tuned_shapes();   // pass shapes measured so far in this process
 // nvec 16-point complex DFTs of dd vectors (plan element format, device) by the engine's dd Stockham pass: the
 // plain baseline of the split-precision form. Synchronous.
 void plain_dft16(const double* x, long long nvec, double* y, hipStream_t s);
+// the split matrix slices (split_matrix_digits) on the current device, made on first use and kept for the process
+const long* split_dft16_digits(int S, hipStream_t s);
 
 // The Kaiser-sinc taps h[0..L] (src/design.cpp:17-37) on the GPU, as (hi, lo) pairs in device memory.
 DevBuf kernel_taps_gpu(long long L, long long a, long long b, double beta, hipStream_t s);
