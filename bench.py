@@ -406,6 +406,102 @@ def bench_batch(args):
     return 0
 
 
+# ---- extreme-ratio plans: 10 MHz -> 48 kHz mono 30 s and 8-channel 384 kHz -> 44.1 kHz 600 s, --len 0.5 -------
+def bench_extreme(args):
+    import numpy as np
+    import torch
+
+    import kbsrc_amd
+    from tests import helpers as H
+
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev).cuda_stream
+    cases = []
+
+    def one(name, x, rate_in, rate_out, split_slices=()):
+        frames, ch = x.shape
+        t0 = time.perf_counter()
+        plan = kbsrc_amd.Plan(frames, ch, rate_in, rate_out, "exact", length=0.5)
+        t_design = time.perf_counter() - t0
+        info = plan.info
+        xx = torch.empty(ch * info["xstride"], dtype=torch.float64, device=dev)
+        out = torch.empty(ch * info["nout"], dtype=torch.float64, device=dev)
+        plan.stage(x, xx.data_ptr(), stream)
+        torch.cuda.synchronize()
+
+        def timed(run):
+            for _ in range(max(1, args.warmup)):
+                run()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                run()
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t0) * 1e3 / args.steps
+
+        ms = timed(lambda: plan.execute(xx.data_ptr(), out.data_ptr(), stream))
+        kt = plan.execute_timed(xx.data_ptr(), out.data_ptr(), stream)
+        nout_total = ch * info["nout"]
+        fl = block_flops(info["Nb"], info["Mb"], True, bool(info["up"]))
+        gflop = sum(fl.values()) * info["tasks"] / 1e9
+        row = {"workload": name, "ms": round(ms, 3), "output_msamples_s": round(nout_total / (ms / 1e3) / 1e6, 1),
+               "tflops": round(gflop / ms, 2), "nb": info["nb"], "Nb": info["Nb"], "Mb": info["Mb"],
+               "tasks": info["tasks"], "design_s": round(t_design, 3),
+               "pass_ms": {k: round(v, 3) for k, v in kt.items()}}
+        print(f"{name}: {ms:.3f} ms -> {row['output_msamples_s']} output Msamples/s, {row['tflops']} TFLOP/s "
+              f"(design {t_design:.2f} s)", file=sys.stderr, flush=True)
+        if split_slices:
+            plain = out.clone()
+            peak = float(plain.abs().max().item())
+            row["split"] = {}
+            for sl in split_slices:
+                sms = timed(lambda: plan.execute_split(xx.data_ptr(), out.data_ptr(), sl, stream))
+                d = float((out - plain).abs().max().item()) / peak
+                row["split"][sl] = {"ms": round(sms, 3), "speed_vs_plain": round(ms / sms, 3),
+                                    "max_diff_vs_plain_over_peak": d}
+                print(f"  split S={sl}: {sms:.3f} ms (x{ms / sms:.3f}), max diff vs plain {d:.2e} of peak",
+                      file=sys.stderr, flush=True)
+            del plain
+        plan.close()
+        del xx, out
+        torch.cuda.empty_cache()
+        cases.append(row)
+
+    one("mono 10000000 -> 48000 Hz, 30 s, exact, --len 0.5", H.input_10m_30s().reshape(-1, 1), 10000000, 48000)
+    one("8 ch 384000 -> 44100 Hz, 600 s, exact, --len 0.5", H.input_384k(H.FRAMES_384K_FULL), 384000, 44100,
+        split_slices=(8, 15))
+    cpu = None
+    ref = os.path.join(ROOT, "oracle", "_ref", "kbsrc")
+    if not args.no_cpu and os.path.exists(ref):   # bounded sample: 2 s of the 384 kHz input at the same block
+        from tests.golden.make_extreme_pins import write_wav
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        with tempfile.TemporaryDirectory(prefix="kbsrc-bench-") as td:
+            src, dst = os.path.join(td, "in.wav"), os.path.join(td, "out.wav")
+            write_wav(src, H.FRAMES_384K_SHORT, 8, 384000, H.input_384k_chunks(H.FRAMES_384K_SHORT))
+            r = subprocess.run([ref, src, dst, "--rate", "44100", "--len", "0.5", "--block", "1536", "-t",
+                                str(threads), "-v"], capture_output=True, text=True, timeout=300)
+            m = re.search(r"design ([0-9.]+)s\s+resample ([0-9.]+)s", r.stderr)
+            if not r.returncode and m:
+                eng = float(m.group(2))
+                cpu = {"value": round(8 * 88200 / eng / 1e6, 2), "unit": "output Msamples/s", "cores": threads,
+                       "kind": "reference",
+                       "sample": f"8 ch x 2 s of the 384 kHz input at block 1536 (the full plan's transform "
+                                 f"shapes): reference engine resample {eng:.3f} s, design and file I/O excluded"}
+    big = cases[1]
+    line = {
+        "metric": "exact-mode output Msamples/s, extreme-ratio plans (8 ch 384k->44.1k 600 s headline)",
+        "value": big["output_msamples_s"], "unit": "output Msamples/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": big["ms"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": big["workload"], "parallelism": "single GPU"},
+        "cases": cases, "cpu_baseline": cpu,
+    }
+    if cpu:
+        line["speedup_vs_cpu_engine"] = round(big["output_msamples_s"] / cpu["value"], 1)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
 # ---- split-precision form: radix-16 DFT stages, int8 MFMA slices against the plain dd pass ---------------------
 def bench_split(args):
     import numpy as np
@@ -548,9 +644,10 @@ def main():
                     help="also run a plain 256 MiB device copy (calibrates HBM byte counters under rocprofv3 --pmc)")
     ap.add_argument("--tune", action="store_true",
                     help="measure kernel shapes once before timing (Plan.tune / Batch.tune; same output bits)")
-    ap.add_argument("--workload", default="single", choices=["single", "batch", "split"],
+    ap.add_argument("--workload", default="single", choices=["single", "batch", "split", "extreme"],
                     help="single: the stereo 96 kHz -> 44.1 kHz 180 s job (default); batch: 64 mixed jobs in one "
-                         "batch; split: int8-MFMA split-precision radix-16 DFTs against the plain dd pass")
+                         "batch; split: int8-MFMA split-precision radix-16 DFTs against the plain dd pass; extreme: "
+                         "10 MHz -> 48 kHz 30 s and 8-channel 384 kHz -> 44.1 kHz 600 s at --len 0.5")
     ap.add_argument("--summarize-rocprof", metavar="DIR",
                     help="print the per-pass summary of rocprofv3 csv output under DIR/{stats,fetch,write} and exit")
     args = ap.parse_args()
@@ -565,6 +662,8 @@ def main():
         return bench_batch(args)
     if args.workload == "split":
         return bench_split(args)
+    if args.workload == "extreme":
+        return bench_extreme(args)
 
     import numpy as np
     import torch

@@ -189,6 +189,7 @@ const FixedEntry* fixed_table_auto0(int* count);
 const FixedEntry* fixed_table_auto1(int* count);
 const FixedEntry* fixed_table_auto2(int* count);
 const FixedEntry* fixed_table_auto3(int* count);
+const FixedEntry* fixed_table_extreme(int* count);
 
 // variant: index among the registered shapes of (policy, n, ld, st); -1: the default (or KBSRC_TUNE's)
 bool launch_fixed(int policy, int ld, int st, PassArgs& a, hipStream_t s, hipError_t* err, int variant = -1);

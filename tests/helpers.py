@@ -54,6 +54,34 @@ PINS_10M_30S = {"exact": "41ba6929f40b098befd88dc7036a0b42fd9b04912f91ff13cb043e
                 "fast": "f673ce48cae1a103858663581d33bbbce6ce4e566fda554955d46769b77c45cf"}
 
 
+# 8-channel 384 kHz -> 44.1 kHz, --len 0.5 (ratio 147/1280): channel c is default_rng(20 + c).uniform(-0.5, 0.5)
+FRAMES_384K_SHORT = 768_000        # 2 s, run at --block 1536 (the full-length plan's block: same transform shapes)
+FRAMES_384K_FULL = 230_400_000     # 600 s
+
+
+# sha256 of the planar float64 output, measured on the compiled reference (tests/golden/make_extreme_pins.py)
+PINS_384K_SHORT = {"exact": "d0865900cdf580cbde077c69e9e0f9517ff551f62d01583309bc9830a3cdc9fa",
+                   "fast": "c8e53844cbbe5ec98211be5bc68f9e126f4c53fea0b5836999338793cd5fedb9"}
+PIN_384K_600S = "a5456c800f46db850a20cadab8344ae5e4d8e0f8686cf38e15b23c0c5afe55b6"   # exact, auto block (1536)
+
+
+def input_384k_chunks(frames, chunk=1 << 22):
+    """(n, 8) interleaved blocks of the 384 kHz input, in order (each channel's stream drawn chunk by chunk)."""
+    rngs = [np.random.default_rng(20 + c) for c in range(8)]
+    for i in range(0, frames, chunk):
+        n = min(chunk, frames - i)
+        yield np.stack([r.uniform(-0.5, 0.5, n) for r in rngs], 1)
+
+
+def input_384k(frames):
+    x = np.empty((frames, 8))
+    i = 0
+    for blk in input_384k_chunks(frames):
+        x[i:i + len(blk)] = blk
+        i += len(blk)
+    return x
+
+
 def input_10m_30s():
     return np.random.default_rng(5).uniform(-0.5, 0.5, 300_000_000)
 

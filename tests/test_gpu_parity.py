@@ -160,8 +160,8 @@ def test_44k1_to_48k_180s_pin():
 
 @pytest.mark.parametrize("mode", ["exact", "fast"])
 def test_long_subfft_extreme_ratio_pin(mode):
-    """10 MHz -> 48 kHz at --len 0.5: the exact engine's 3200-point rows exceed an LDS tile and run as
-    global-memory passes; both modes give the reference's bits."""
+    """10 MHz -> 48 kHz at --len 0.5: the exact engine's 3150 x 3200 four-step (100 KB of LDS per sub-FFT, one
+    per workgroup in the specialised passes); both modes give the reference's bits."""
     import hashlib
     say(f"10M->48k len 0.5 {mode}")
     y = gpu().resample(H.input_10m_short().reshape(-1, 1), 10000000, 48000, mode, length=0.5)
@@ -178,6 +178,29 @@ def test_extreme_ratio_full_length_pins():
         y = gpu().resample(x, 10000000, 48000, mode, length=0.5)
         assert y.shape == (1, 1440000)
         assert hashlib.sha256(y.tobytes()).hexdigest() == H.PINS_10M_30S[mode]
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+def test_384k_8ch_pin(mode):
+    """8 channels 384 kHz -> 44.1 kHz at --len 0.5 and the full-length plan's block (1536: a 1,966,080-point
+    forward transform, 960 x 1024 four-step, inverse 336 x 336), on 2 s: the reference's bits."""
+    import hashlib
+    say(f"384k->44.1k 8 ch len 0.5 block 1536 {mode}")
+    y = gpu().resample(H.input_384k(H.FRAMES_384K_SHORT), 384000, 44100, mode, length=0.5, block=1536)
+    assert y.shape == (8, 88200)
+    assert hashlib.sha256(y.tobytes()).hexdigest() == H.PINS_384K_SHORT[mode]
+
+
+def test_384k_8ch_full_length_pin():
+    """8 channels 384 kHz -> 44.1 kHz, 600 s (230,400,000 frames per channel, 14.7 GB of input), exact, --len 0.5:
+    the reference's bits over all 1176 blocks."""
+    import hashlib
+    say("384k->44.1k 8 ch 600 s len 0.5 exact")
+    x = H.input_384k(H.FRAMES_384K_FULL)
+    y = gpu().resample(x, 384000, 44100, "exact", length=0.5)
+    del x
+    assert y.shape == (8, 26_460_000)
+    assert hashlib.sha256(y.tobytes()).hexdigest() == H.PIN_384K_600S
 
 
 GLOBAL_CASES = [c for c in CASES if c[3] >= 1000 and c[2] <= 2][::2]
