@@ -2,6 +2,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -161,7 +162,8 @@ int pick_group(int policy, int n, int nsub) {
 // generic kernel; KBSRC_FUSED=1 folds the split/multiply/fold kernel into the passes next to it (same bits; it saves
 // a kernel and ~1.2 GB of HBM traffic per benchmark step but measured no faster, the passes being FP64-bound).
 // Read when a plan is made: KBSRC_GLOBAL_PASSES=1 runs the passes through global scratch, as sub-FFTs too long for
-// an LDS tile always do (two LDS passes where the stage list splits, =2: one launch per stage).
+// an LDS tile always do (two LDS passes where the stage list splits, =2: one launch per stage). Read at every split
+// execute: KBSRC_SPLIT_GLOBAL=1 runs every split-form pass through the global-memory route (same bits).
 bool env_flag(const char* name) {
     const char* v = std::getenv(name);
     return v && v[0] == '1';
@@ -267,7 +269,26 @@ void run_pass(int policy, int ld, int st, PassArgs a, hipStream_t s, const char*
     bool known = false;
     PassChoice choice{};
     if (a.split > 0 && a.tw16 && policy != POL_F64 && ld != LD_FOLD && st != ST_SPLIT) {   // the split form
-        e = launch_split_pass(policy, ld, st, a, s);
+        // inside one LDS pass where the sub-FFT fits (KBSRC_SPLIT_GLOBAL=1: always the global-memory route)
+        if (split_lds_ok(policy, ld, st, a) && !env_flag("KBSRC_SPLIT_GLOBAL")) {
+            // up to 80 KB of LDS per workgroup: the split kernel's registers allow two workgroups per CU, and more
+            // vectors per workgroup keep its four waves busy in the matrix stage (measured on 320-point rows:
+            // G = 4 beat G = 2 by ~10% at 15 slices)
+            a.G = (int)std::max<size_t>(1, std::min<size_t>({(size_t)a.nsub, 16, 80 * 1024 / pass_lds_bytes(policy, a.sd.n, 1)}));
+            int threads = kThreads;
+            if (const char* v = std::getenv("KBSRC_SPLIT_SHAPE")) {   // tuning knob: "G,T"
+                int g = 0, t = 0;
+                if (std::sscanf(v, "%d,%d", &g, &t) == 2 && g >= 1 && t >= 64 && t <= 1024 && t % 64 == 0 &&
+                    pass_lds_bytes(policy, a.sd.n, g) <= kLdsMax) {
+                    a.G = std::min(g, a.nsub);
+                    threads = t;
+                }
+            }
+            a.ngrp = (a.nsub + a.G - 1) / a.G;
+            e = launch_split_lds(policy, ld, st, a, threads, s);
+        } else {
+            e = launch_split_pass(policy, ld, st, a, s);
+        }
         if (e != hipSuccess)
             throw HipError(std::string(what) + " (split form, n=" + std::to_string(a.sd.n) + "): " +
                            hipGetErrorString(e));

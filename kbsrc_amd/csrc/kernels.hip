@@ -9,6 +9,7 @@
 //     a 16-byte "lo" complex, two dwordx4 accesses per element; FP64 elements are the 16-byte hi half alone;
 //   - every (channel, block) task of a batch is one launch dimension, so a launch has thousands of workgroups.
 #include "kernels.hpp"
+#include "split_core.hpp"
 
 #pragma clang fp contract(off)
 
@@ -114,7 +115,10 @@ __device__ __forceinline__ void bfly_odd(const Planes<A>& cur, const Planes<A>& 
     }
 }
 
-template <class A, int LD, int ST>
+// SPL > 0 (split form, dd policies, PassArgs::pm == 0): the first two radix-4 stages run as one int8-MFMA
+// radix-16 stage with SPL slices (split_core.hpp); y[16 p + k] = w_n^{p k} DFT16(x[p + (n / 16) m])[k], the
+// twiddle in dd with the accurate add, as the global-memory route's stage computes it.
+template <class A, int LD, int ST, int SPL = 0>
 __global__ __launch_bounds__(256) void k_subfft(PassArgs a) {
     extern __shared__ double lds[];
     const int tid = threadIdx.x, nthr = blockDim.x;
@@ -147,8 +151,48 @@ __global__ __launch_bounds__(256) void k_subfft(PassArgs a) {
     }
     __syncthreads();
 
+    int si0 = 0;
+    if constexpr (SPL > 0) {
+        const int nv = n >> 4, NV = G * nv;
+        const int lane = tid & 63, q = lane >> 4, col = lane & 15;
+        for (int base = (tid >> 6) * 16; base < NV; base += (nthr >> 6) * 16) {   // uniform over the wave
+            const int v = base + col;
+            const bool live = v < NV;
+            const int g = live ? v % G : 0, p = live ? v / G : 0;
+            double hi[8], lo[8];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                if (live) {
+                    const Cx<A> c = cur.get((p + nv * (4 * q + i)) * Gs + g);
+                    hi[2 * i] = c.re.hi;
+                    hi[2 * i + 1] = c.im.hi;
+                    lo[2 * i] = c.re.lo;
+                    lo[2 * i + 1] = c.im.lo;
+                } else {
+                    hi[2 * i] = hi[2 * i + 1] = lo[2 * i] = lo[2 * i + 1] = 0.0;
+                }
+            }
+            dd re[4], im[4];
+            split::split_core<SPL>(hi, lo, a.adig, lane, re, im);
+            if (live) {
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int k = 4 * q + r;
+                    cx<dd> z{re[r], im[r]};
+                    if (p != 0 && k != 0) z = c_mul<DDA>(z, gload<DDA>(a.tw16, (long long)p * 15 + k - 1));
+                    nxt.put((16 * p + k) * Gs + g, z);
+                }
+            }
+        }
+        __syncthreads();
+        const Planes<A> t = cur;
+        cur = nxt;
+        nxt = t;
+        si0 = 2;
+    }
+
     // ---- Stockham stages, ping-pong between the two LDS halves (src/dsp/fft.hpp:115-150)
-    for (int si = 0; si < a.sd.nst; ++si) {
+    for (int si = si0; si < a.sd.nst; ++si) {
         const StageDesc st = a.sd.st[si];
         const int r = st.r, s = st.s, sm = n / r;
         const int nbf = sm;   // butterflies per sub-FFT
@@ -423,7 +467,46 @@ template <class A, int LD, int ST> hipError_t launch_one(const PassArgs& a, int 
     return hipGetLastError();
 }
 
+template <int LD, int ST, int SPL> hipError_t launch_one_split(const PassArgs& a, int threads, hipStream_t s) {
+    const size_t lds = (size_t)2 * kPlanes<DDS> * a.sd.n * a.G * sizeof(double);
+    static bool attr_set = false;
+    if (!attr_set) {
+        hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_subfft<DDS, LD, ST, SPL>),
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        if (e != hipSuccess) return e;
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((k_subfft<DDS, LD, ST, SPL>), dim3(a.ngrp * a.ntask), dim3(threads), lds, s, a);
+    return hipGetLastError();
+}
+
+template <int SPL> hipError_t launch_split_slices(int ld, int st, const PassArgs& a, int threads, hipStream_t s) {
+    if (ld == LD_REAL_F64 && st == ST_CPLX) return launch_one_split<LD_REAL_F64, ST_CPLX, SPL>(a, threads, s);
+    if (ld == LD_CPLX && st == ST_CPLX) return launch_one_split<LD_CPLX, ST_CPLX, SPL>(a, threads, s);
+    if (ld == LD_CPLX && st == ST_OUT) return launch_one_split<LD_CPLX, ST_OUT, SPL>(a, threads, s);
+    return hipErrorInvalidValue;
+}
+
 }  // namespace
+
+bool split_lds_ok(int policy, int ld, int st, const PassArgs& a) {
+    const bool slices = a.split == 6 || a.split == 7 || a.split == 8 || a.split == 15;
+    const bool io = (ld == LD_REAL_F64 && st == ST_CPLX) || (ld == LD_CPLX && (st == ST_CPLX || st == ST_OUT));
+    return policy == POL_DDS && slices && io && a.pm == 0 && a.tw16 && a.adig && a.sd.nst >= 2 &&
+           a.sd.st[0].r == 4 && a.sd.st[1].r == 4 && pass_lds_bytes(policy, a.sd.n, 1) <= 160 * 1024;
+}
+
+hipError_t launch_split_lds(int policy, int ld, int st, const PassArgs& a, int threads, hipStream_t s) {
+    if (!split_lds_ok(policy, ld, st, a) || threads % 64 || pass_lds_bytes(policy, a.sd.n, a.G) > 160 * 1024)
+        return hipErrorInvalidValue;
+    switch (a.split) {
+        case 6: return launch_split_slices<6>(ld, st, a, threads, s);
+        case 7: return launch_split_slices<7>(ld, st, a, threads, s);
+        case 8: return launch_split_slices<8>(ld, st, a, threads, s);
+        case 15: return launch_split_slices<15>(ld, st, a, threads, s);
+        default: return hipErrorInvalidValue;
+    }
+}
 
 size_t pass_lds_bytes(int policy, int n, int G) {
     return (size_t)2 * (policy == POL_F64 ? 2 : 4) * n * G * sizeof(double);

@@ -220,6 +220,11 @@ hipError_t launch_split16_stage(const double* x, double* y, int n, long long nsu
 // A sub-FFT pass in the split form (PassArgs::split): gather to gscr0, the radix-16 split stage, the remaining
 // stages as one LDS pass per class q + 16 j, scatter (four-step twiddle, renormalisation, output as the pass does).
 hipError_t launch_split_pass(int policy, int ld, int st, const PassArgs& a, hipStream_t s);
+// The same inside one LDS pass of the generic kernel (G sub-FFTs per workgroup as a's G, ngrp): exact-mode
+// four-step sub-FFTs (POL_DDS) that fit a tile, slice counts 6, 7, 8 and 15; split_lds_ok says whether a pass
+// qualifies. Same bits as the global-memory route.
+bool split_lds_ok(int policy, int ld, int st, const PassArgs& a);
+hipError_t launch_split_lds(int policy, int ld, int st, const PassArgs& a, int threads, hipStream_t s);
 hipError_t launch_cis_table(const TableArgs& a, hipStream_t s);
 // Bluestein chirp b[t] = cis(t^2 mod 2n, 2n), t < n, into bc (plan format) and the chirp sequence v (dd format,
 // m elements, zeroed by the caller): v[t] = v[m - t] = b[t] (src/dsp/fft.hpp:231-236)

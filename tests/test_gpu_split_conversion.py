@@ -103,3 +103,27 @@ def test_split_form_rejections():
         with pytest.raises(ValueError, match="option out of range"):
             plan.execute_split(d_pad.data_ptr(), d_out.data_ptr(), slices)
         plan.close()
+
+
+@pytest.mark.parametrize("fi,fo,block", [(96000, 44100, 630), (44100, 48000, 630), (384000, 44100, 1536)])
+def test_split_routes_give_the_same_bits(fi, fo, block, monkeypatch):
+    """The split form inside one LDS pass and through the global-memory route (gather, stage, tail pass, scatter)
+    compute the same operations: bit-identical outputs, for every pass that has a radix-16 stage."""
+    import torch
+    k = gpu()
+    x = np.random.default_rng(fi + fo).uniform(-0.5, 0.5, (2 * fi, 2))
+    plan = k.Plan(x.shape[0], 2, fi, fo, "exact", length=0.5 if fi == 384000 else 0.2, block=block)
+    d_pad = torch.zeros(2 * plan.info["xstride"], dtype=torch.float64, device="cuda")
+    d_out = torch.empty(2 * plan.info["nout"], dtype=torch.float64, device="cuda")
+    plan.stage(x, d_pad.data_ptr())
+    for s in (6, 15):
+        say(f"{fi} -> {fo}: split S={s}, LDS and global routes")
+        monkeypatch.delenv("KBSRC_SPLIT_GLOBAL", raising=False)
+        plan.execute_split(d_pad.data_ptr(), d_out.data_ptr(), s)
+        torch.cuda.synchronize()
+        lds = d_out.cpu().numpy().copy()
+        monkeypatch.setenv("KBSRC_SPLIT_GLOBAL", "1")
+        plan.execute_split(d_pad.data_ptr(), d_out.data_ptr(), s)
+        torch.cuda.synchronize()
+        assert np.array_equal(d_out.cpu().numpy(), lds), s
+    plan.close()
