@@ -269,8 +269,17 @@ void run_pass(int policy, int ld, int st, PassArgs a, hipStream_t s, const char*
     bool known = false;
     PassChoice choice{};
     if (a.split > 0 && a.tw16 && policy != POL_F64 && ld != LD_FOLD && st != ST_SPLIT) {   // the split form
-        // inside one LDS pass where the sub-FFT fits (KBSRC_SPLIT_GLOBAL=1: always the global-memory route)
-        if (split_lds_ok(policy, ld, st, a) && !env_flag("KBSRC_SPLIT_GLOBAL")) {
+        // a specialised split pass where one is built, else inside one generic LDS pass where the sub-FFT fits
+        // (KBSRC_SPLIT_GLOBAL=1: always the global-memory route; =2: never the specialised passes)
+        const char* sg = std::getenv("KBSRC_SPLIT_GLOBAL");
+        const bool global = sg && sg[0] == '1', no_fixed = sg && sg[0] == '2';
+        const SplitFixedEntry* fe = policy == POL_DDS && a.pm == 0 && !global && !no_fixed
+                                        ? find_split_fixed(a.sd.n, ld, st, a.split) : nullptr;
+        if (fe) {
+            a.G = fe->G;   // the kernel's compile-time group; a partial last group is masked
+            a.ngrp = (a.nsub + a.G - 1) / a.G;
+            e = fe->launch(a, s);
+        } else if (split_lds_ok(policy, ld, st, a) && !global) {
             // up to 80 KB of LDS per workgroup: the split kernel's registers allow two workgroups per CU, and more
             // vectors per workgroup keep its four waves busy in the matrix stage (measured on 320-point rows:
             // G = 4 beat G = 2 by ~10% at 15 slices)

@@ -190,6 +190,13 @@ const FixedEntry* fixed_table_auto1(int* count);
 const FixedEntry* fixed_table_auto2(int* count);
 const FixedEntry* fixed_table_auto3(int* count);
 const FixedEntry* fixed_table_extreme(int* count);
+// specialised split-form passes (kernels_fixed_split.hip): exact-mode (POL_DDS) sub-FFT passes with the first two
+// radix-4 stages as the int8-MFMA radix-16 stage; nullptr when none is built for (n, ld, st, slices)
+struct SplitFixedEntry {
+    int n, ld, st, slices, G, T;
+    hipError_t (*launch)(const PassArgs&, hipStream_t);
+};
+const SplitFixedEntry* find_split_fixed(int n, int ld, int st, int slices);
 
 // variant: index among the registered shapes of (policy, n, ld, st); -1: the default (or KBSRC_TUNE's)
 bool launch_fixed(int policy, int ld, int st, PassArgs& a, hipStream_t s, hipError_t* err, int variant = -1);

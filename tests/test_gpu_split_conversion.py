@@ -107,8 +107,9 @@ def test_split_form_rejections():
 
 @pytest.mark.parametrize("fi,fo,block", [(96000, 44100, 630), (44100, 48000, 630), (384000, 44100, 1536)])
 def test_split_routes_give_the_same_bits(fi, fo, block, monkeypatch):
-    """The split form inside one LDS pass and through the global-memory route (gather, stage, tail pass, scatter)
-    compute the same operations: bit-identical outputs, for every pass that has a radix-16 stage."""
+    """The split form's three routes compute the same operations and give the same bits: the specialised split
+    passes (320- and 224-point sub-FFTs), the generic LDS pass with the matrix stage (KBSRC_SPLIT_GLOBAL=2), and
+    the global-memory route (gather, stage, tail pass, scatter; KBSRC_SPLIT_GLOBAL=1)."""
     import torch
     k = gpu()
     x = np.random.default_rng(fi + fo).uniform(-0.5, 0.5, (2 * fi, 2))
@@ -117,13 +118,16 @@ def test_split_routes_give_the_same_bits(fi, fo, block, monkeypatch):
     d_out = torch.empty(2 * plan.info["nout"], dtype=torch.float64, device="cuda")
     plan.stage(x, d_pad.data_ptr())
     for s in (6, 15):
-        say(f"{fi} -> {fo}: split S={s}, LDS and global routes")
-        monkeypatch.delenv("KBSRC_SPLIT_GLOBAL", raising=False)
-        plan.execute_split(d_pad.data_ptr(), d_out.data_ptr(), s)
-        torch.cuda.synchronize()
-        lds = d_out.cpu().numpy().copy()
-        monkeypatch.setenv("KBSRC_SPLIT_GLOBAL", "1")
-        plan.execute_split(d_pad.data_ptr(), d_out.data_ptr(), s)
-        torch.cuda.synchronize()
-        assert np.array_equal(d_out.cpu().numpy(), lds), s
+        outs = []
+        for route in (None, "2", "1"):
+            say(f"{fi} -> {fo}: split S={s}, route {route or 'specialised'}")
+            if route:
+                monkeypatch.setenv("KBSRC_SPLIT_GLOBAL", route)
+            else:
+                monkeypatch.delenv("KBSRC_SPLIT_GLOBAL", raising=False)
+            d_out.zero_()
+            plan.execute_split(d_pad.data_ptr(), d_out.data_ptr(), s)
+            torch.cuda.synchronize()
+            outs.append(d_out.cpu().numpy().copy())
+        assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2]), s
     plan.close()
