@@ -474,6 +474,12 @@ void run_cfft(const CFFTPlan& f, const RunCtx& c, int ld, const double* src, int
     const int pol_four = !c.dd ? POL_F64 : lane ? POL_DDS : POL_DDA;
     const int pol_one = c.dd ? POL_DDA : POL_F64;
     const long long ts = c.ts;
+    if (f.blue && c.proto.zkeep >= 0) {   // every output of the inner transforms is needed
+        RunCtx all = c;
+        all.proto.zkeep = -1;
+        run_cfft(f, all, ld, src, st, dst, scr, t_col, t_row);
+        return;
+    }
     if (f.blue) {   // X_k = conj(b_k) sum_j x_j conj(b_j) b_{k-j}, through the inner FFT twice (src/dsp/fft.hpp:268-278)
         BlueArgs b;
         b.n = f.n; b.m = f.m; b.ntask = c.proto.ntask; b.task0 = c.proto.task0; b.nblk = c.proto.nblk;
@@ -500,6 +506,7 @@ void run_cfft(const CFFTPlan& f, const RunCtx& c, int ld, const double* src, int
         a.dst = scr[0]; a.dst_tstride = ts; a.ste = f.N2; a.stg = 1;
         a.tw4 = f.tw4.d(); a.twmode = lane ? 1 : 2; a.renorm = lane && c.dd;
         a.f.M = f.N2;
+        a.zkeep = -1;   // the column pass's outputs all feed the rows
         timed(c, t_col, [&] { run_pass(pol_four, ld, ST_CPLX, a, c.s, "column pass"); });
         PassArgs b = c.proto;
         fill_sub(b, f.s2, *c.k, c.dd, pol_four, f.N1);
@@ -615,6 +622,16 @@ Engine::Engine(const Plan& p, const double* H, long long nH, int Jg, hipStream_t
     k_.init(s);
     fw_.init(p.Nb, dd_, k_, s);
     iv_.init(p.Mb, dd_, k_, s);
+    // the largest spectrum bin the bin map reads (src/resampler.cpp:53-78): down, bins up to min(Nb/2, Mb - 1);
+    // up, bins up to min(Nb/2, Mb/2); general fold, up to min(Nb/2, Jg). The real-FFT split pairs bin i with
+    // z[i] and z[h - i], so z[k] for zkeep < k < h - zkeep is never read; -1 when that range is empty.
+    {
+        const long long h = fw_.h;
+        const long long xmax = Jg_ ? std::min<long long>(h, Jg_)
+                             : p.up ? std::min<long long>(h, p.Mb / 2) : std::min<long long>(h, p.Mb - 1);
+        zkeep_ = xmax + 1 < h - xmax ? xmax : -1;
+        if (env_flag("KBSRC_STORE_ALL")) zkeep_ = -1;   // test switch: store every bin (same output)
+    }
     // H in the element format of the engine: (hi, lo) pairs or FP64
     H_.alloc((size_t)nH * (dd_ ? 16 : 8));
     if (dd_) hip_check(hipMemcpyAsync(H_.d(), H, (size_t)nH * 16, hipMemcpyDeviceToDevice, s), "kernel spectrum copy");
@@ -698,9 +715,12 @@ void Engine::execute(const double* xx, double* out, hipStream_t s, PassTimes* ti
             run_cfft(iv_.c, c, LD_FOLD, W0, ST_OUT, nullptr, sf, &local.inv_col, &local.inv_row);
             continue;
         }
-        // forward real FFT: padded input -> z (W0)
+        // forward real FFT: padded input -> z (W0); the last pass stores only the bins the bin map reads
         double* const sf[3] = {W1, W2, W3};
+        a.zkeep = zkeep_;
+        a.zh = fw_.h;
         run_cfft(fw_.c, c, LD_REAL_F64, xx, ST_CPLX, W0, sf, &local.fwd_col, &local.fwd_row);
+        a.zkeep = -1;
         // split, x H, bin map, inverse pre-split: z (W0) -> z' (W1)
         MidArgs m;
         m.z = W0; m.z_tstride = ts_; m.zi = W1; m.zi_tstride = ts_;

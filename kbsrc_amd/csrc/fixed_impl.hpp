@@ -61,17 +61,40 @@ template <int... R> struct Radices {
     }
 };
 
-// One Stockham stage over G sub-FFTs in place: compute all of this thread's butterflies, sync, write back.
+// Outputs of the last stage that no later reader needs (PassArgs::zkeep): sub-FFT j's output e goes to element
+// e * ste + j * stg, unread when keep < i < zh - keep. keep < 0: every output is needed.
+struct Prune {
+    long long keep = -1, zh = 0, ste = 0, stg = 0;
+    int j0 = 0;
+};
+
+// One Stockham stage over G sub-FFTs in place: compute all of this thread's butterflies, sync, write back. In the
+// last stage a butterfly whose r outputs are all unread is skipped (the store skips those elements too), which
+// changes no value that is stored.
 template <class A, class RL, int G, int T, int I>
-__device__ __forceinline__ void stage(const Lds<A, RL::N * G>& L, int tid, const double* tw_all, const double* oddc) {
+__device__ __forceinline__ void stage(const Lds<A, RL::N * G>& L, int tid, const double* tw_all, const double* oddc,
+                                      const Prune& pr) {
     constexpr int N = RL::N, r = RL::r[I], s = RL::stride(I), nbf = N / r, W = G * nbf, IT = (W + T - 1) / T;
+    constexpr bool kLast = I == RL::nst - 1;
     const double* tws = tw_all + (long long)RL::twoff(I) * kElt<A>;
+    // last stage: s = nbf, so butterfly b writes outputs b + u s
+    auto needed = [&](int g, int b) -> bool {
+        if (!kLast || pr.keep < 0) return true;
+        const long long j = pr.j0 + g;
+#pragma unroll
+        for (int u = 0; u < r; ++u) {
+            const long long i = (long long)(b + u * s) * pr.ste + j * pr.stg;
+            if (i <= pr.keep || i >= pr.zh - pr.keep) return true;
+        }
+        return false;
+    };
     Cx<A> held[IT][r];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const int w = tid + it * T;
         if (W % T == 0 || w < W) {
             const int g = w % G, b = w / G, p = b / s;
+            if (!needed(g, b)) continue;
             Cx<A> x[r];
 #pragma unroll
             for (int t = 0; t < r; ++t) x[t] = L.get((b + t * nbf) * G + g);
@@ -84,6 +107,7 @@ __device__ __forceinline__ void stage(const Lds<A, RL::N * G>& L, int tid, const
         const int w = tid + it * T;
         if (W % T == 0 || w < W) {
             const int g = w % G, b = w / G, p = b / s, q = b - p * s;
+            if (!needed(g, b)) continue;
             const int dbase = s * r * p + q;
 #pragma unroll
             for (int u = 0; u < r; ++u) L.put((dbase + u * s) * G + g, held[it][u]);
@@ -94,8 +118,8 @@ __device__ __forceinline__ void stage(const Lds<A, RL::N * G>& L, int tid, const
 
 template <class A, class RL, int G, int T, int... I>
 __device__ __forceinline__ void all_stages(const Lds<A, RL::N * G>& L, int tid, const double* tw, const double* oddc,
-                                           std::integer_sequence<int, I...>) {
-    (stage<A, RL, G, T, I>(L, tid, tw, oddc), ...);
+                                           const Prune& pr, std::integer_sequence<int, I...>) {
+    (stage<A, RL, G, T, I>(L, tid, tw, oddc, pr), ...);
 }
 
 // Workgroup index remapped so that each of the 8 XCDs (which take workgroups round-robin) works on a contiguous
@@ -278,7 +302,11 @@ __global__ __launch_bounds__(T, 4) void k_fixed(PassArgs a) {
         }
     }
     __syncthreads();
-    all_stages<A, RL, G, T>(L, tid, a.tw, a.oddc, std::make_integer_sequence<int, RL::nst>{});
+    Prune pr;
+    if constexpr (ST == ST_CPLX && !kPaired) {
+        if (a.zkeep >= 0) pr = Prune{a.zkeep, a.zh, a.ste, a.stg, j0};
+    }
+    all_stages<A, RL, G, T>(L, tid, a.tw, a.oddc, pr, std::make_integer_sequence<int, RL::nst>{});
 
     if constexpr (ST == ST_SPLIT) {
         // the sub-FFT outputs are z[M*e + j] (natural order); X[i] needs z[i] and z[h - i]
@@ -305,6 +333,7 @@ __global__ __launch_bounds__(T, 4) void k_fixed(PassArgs a) {
             if (j < 0) continue;
             const long long jg = j;
             const long long i = e * a.ste + jg * a.stg;
+            if (ST == ST_CPLX && z_unused(a, i)) continue;
             Cx<A> v = L.get(e * G + g);
             if (a.twmode && e != 0 && (a.twmode == 1 || jg != 0)) v = c_mul<A>(v, gld<A>(a.tw4, i));
             if constexpr (A::kDD) {

@@ -246,6 +246,7 @@ __global__ __launch_bounds__(256) void k_subfft(PassArgs a) {
             const int g = gfast ? w % G : w / n, e = gfast ? w / G : w % n;
             const long long jg = j0 + g;
             const long long i = e * a.ste + jg * a.stg;
+            if (ST == ST_CPLX && z_unused(a, i)) continue;
             Cx<A> v = cur.get(e * Gs + g);
             if (a.twmode && e != 0 && (a.twmode == 1 || jg != 0)) v = c_mul<A>(v, gload<A>(a.tw4, i));
             if (a.renorm) v = renorm<A>(v);

@@ -87,7 +87,13 @@ struct PassArgs {
     int split = 0;
     const double* tw16 = nullptr;
     const long* adig = nullptr;
+    // zkeep >= 0 (ST_CPLX, the last pass of a forward transform of length zh): outputs zkeep < i < zh - zkeep are
+    // never read (the bin map uses the spectrum only up to bin zkeep, and the real-FFT split pairs bin i with
+    // zh - i), so they are not stored
+    long long zkeep = -1, zh = 0;
 };
+// true when a pass with zkeep set may skip storing output element i
+__host__ __device__ inline bool z_unused(const PassArgs& a, long long i) { return a.zkeep >= 0 && i > a.zkeep && i < a.zh - a.zkeep; }
 
 // Spectrum x H, bin map and inverse pre-split for one batch of tasks (src/fft.hpp:361-409,411-458;
 // src/resampler.cpp:53-78): z (forward complex FFT output, h per task) -> zi (inverse complex FFT input, hi).

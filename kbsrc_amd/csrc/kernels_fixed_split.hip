@@ -13,8 +13,8 @@ namespace {
 
 template <class RL, int G, int T, int... I>
 __device__ __forceinline__ void tail_stages(const Lds<DDS, RL::N * G>& L, int tid, const double* tw,
-                                            const double* oddc, std::integer_sequence<int, I...>) {
-    (stage<DDS, RL, G, T, I + 2>(L, tid, tw, oddc), ...);
+                                            const double* oddc, const Prune& pr, std::integer_sequence<int, I...>) {
+    (stage<DDS, RL, G, T, I + 2>(L, tid, tw, oddc, pr), ...);
 }
 
 // two waves per SIMD: the matrix stage's slicing needs up to ~200 VGPRs at 15 slices
@@ -83,7 +83,11 @@ __global__ __launch_bounds__(T, 2) void k_fixed_split(PassArgs a) {
         }
     }
     __syncthreads();
-    tail_stages<RL, G, T>(L2, tid, a.tw, a.oddc, std::make_integer_sequence<int, RL::nst - 2>{});
+    Prune pr;
+    if constexpr (ST == ST_CPLX) {
+        if (a.zkeep >= 0) pr = Prune{a.zkeep, a.zh, a.ste, a.stg, j0};
+    }
+    tail_stages<RL, G, T>(L2, tid, a.tw, a.oddc, pr, std::make_integer_sequence<int, RL::nst - 2>{});
 
     // store (as the specialised pass): four-step twiddle, renormalisation, complex buffer or output samples
     for (int w = tid; w < G * N; w += T) {
@@ -92,6 +96,7 @@ __global__ __launch_bounds__(T, 2) void k_fixed_split(PassArgs a) {
         if (j < 0) continue;
         const long long jg = j;
         const long long i = e * a.ste + jg * a.stg;
+        if (ST == ST_CPLX && z_unused(a, i)) continue;
         Cx<A> v = L2.get(e * G + g);
         if (a.twmode && e != 0 && (a.twmode == 1 || jg != 0)) v = c_mul<A>(v, gld<A>(a.tw4, i));
         if (a.renorm) v = renorm_dd<A>(v);

@@ -72,6 +72,7 @@ template <class A, int ST> __global__ __launch_bounds__(256) void k_scatter(Pass
     if (idx >= total) return;
     const long long e = idx % n, r = idx / n, j = r % a.nsub, task = r / a.nsub;
     const long long i = e * a.ste + j * a.stg;
+    if (ST == ST_CPLX && z_unused(a, i)) return;
     Cx<A> v = gld<A>(buf, idx);
     if (a.twmode && e != 0 && (a.twmode == 1 || j != 0)) v = c_mul<A>(v, gld<A>(a.tw4, i));
     if constexpr (A::kDD) {
@@ -157,7 +158,7 @@ PassArgs tail_pass(const PassArgs& a, int k, int elt, const double* src, double*
     pb.src = src; pb.src_tstride = tdoubles; pb.dst = dst; pb.dst_tstride = tdoubles;
     pb.ntask = (int)((long long)a.ntask * a.nsub); pb.task0 = 0; pb.tdesc = nullptr;
     pb.nsub = R; pb.lde = R; pb.ldg = 1; pb.ste = R; pb.stg = 1;
-    pb.twmode = 0; pb.renorm = 0; pb.pm = 0; pb.split = 0;
+    pb.twmode = 0; pb.renorm = 0; pb.pm = 0; pb.split = 0; pb.zkeep = -1;
     return pb;
 }
 
@@ -181,7 +182,7 @@ template <class A> hipError_t run_global(int policy, int ld, int st, const PassA
         pa.src = a.gscr0; pa.src_tstride = tdoubles; pa.dst = a.gscr1; pa.dst_tstride = tdoubles;
         pa.ntask = (int)((long long)a.ntask * a.nsub); pa.task0 = 0; pa.tdesc = nullptr;
         pa.nsub = rest; pa.lde = rest; pa.ldg = 1; pa.ste = 1; pa.stg = R;
-        pa.twmode = 0; pa.renorm = 0; pa.pm = rest;
+        pa.twmode = 0; pa.renorm = 0; pa.pm = rest; pa.zkeep = -1;
         set_group(pa, policy);
         hipError_t e = launch_pass(policy, LD_CPLX, ST_CPLX, pa, 256, s);
         if (e != hipSuccess) return e;

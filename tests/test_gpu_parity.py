@@ -203,6 +203,21 @@ def test_384k_8ch_full_length_pin():
     assert hashlib.sha256(y.tobytes()).hexdigest() == H.PIN_384K_600S
 
 
+@pytest.mark.parametrize("rin,rout,mode,length", [(10000000, 48000, "exact", 0.5), (192000, 8000, "exact", 0.2),
+                                                   (48000, 8000, "fast", 0.2), (44100, 44101, "exact", 0.05),
+                                                   (88200, 96000, "exact", 0.2)])
+def test_unstored_bins_change_nothing(rin, rout, mode, length, monkeypatch):
+    """The forward pass leaves unstored the spectrum bins the bin map never reads (above its last bin, and their
+    split partners): same bits as storing every bin (KBSRC_STORE_ALL=1)."""
+    x = np.random.default_rng(rin + rout).uniform(-0.5, 0.5, (200000 if rin == 10000000 else 3 * rin, 2))
+    say(f"unstored bins {rin} -> {rout} {mode}")
+    monkeypatch.delenv("KBSRC_STORE_ALL", raising=False)
+    y = gpu().resample(x, rin, rout, mode, length=length)
+    monkeypatch.setenv("KBSRC_STORE_ALL", "1")
+    z = gpu().resample(x, rin, rout, mode, length=length)
+    assert y.tobytes() == z.tobytes()
+
+
 GLOBAL_CASES = [c for c in CASES if c[3] >= 1000 and c[2] <= 2][::2]
 
 
