@@ -467,7 +467,8 @@ def bench_extreme(args):
         torch.cuda.empty_cache()
         cases.append(row)
 
-    one("mono 10000000 -> 48000 Hz, 30 s, exact, --len 0.5", H.input_10m_30s().reshape(-1, 1), 10000000, 48000)
+    one("mono 10000000 -> 48000 Hz, 30 s, exact, --len 0.5", H.input_10m_30s().reshape(-1, 1), 10000000, 48000,
+        split_slices=(8, 15))
     one("8 ch 384000 -> 44100 Hz, 600 s, exact, --len 0.5", H.input_384k(H.FRAMES_384K_FULL), 384000, 44100,
         split_slices=(8, 15))
     cpu = None
