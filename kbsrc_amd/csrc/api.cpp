@@ -1,6 +1,7 @@
 // C-ABI entry points (include/kbsrc_gpu.h) and the C++ API (include/kbsrc/kbsrc.hpp) over the GPU engine.
 #include <xmmintrin.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <climits>
@@ -120,6 +121,7 @@ struct kbsrc_batch {
         Plan p;
         std::vector<int> jobs;
         long long ntask = 0;
+        int stream = 0;   // index into streams
         DevBuf tdesc;
         std::unique_ptr<Engine> eng;
     };
@@ -457,6 +459,21 @@ int kbsrc_batch_create(int njobs, const kbsrc_job* jobs, const kbsrc_options* op
             b->groups[g].ntask += (long long)p.ch * p.nblk;
         }
         const int nstreams = (int)std::min<size_t>(4, std::max<size_t>(1, b->groups.size()));
+        {   // groups to streams, largest first onto the least-loaded stream (work ~ tasks x n log n of both FFTs)
+            std::vector<std::pair<double, int>> w;
+            for (int g = 0; g < (int)b->groups.size(); ++g) {
+                const Plan& q = b->groups[g].p;
+                const double nb = q.Nb, mb = q.Mb;
+                w.push_back({(double)b->groups[g].ntask * (nb * std::log2(nb + 2) + mb * std::log2(mb + 2)), g});
+            }
+            std::sort(w.begin(), w.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+            std::vector<double> load(nstreams, 0.0);
+            for (const auto& [wt, g] : w) {
+                const int i = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+                b->groups[g].stream = i;
+                load[i] += wt;
+            }
+        }
         for (int i = 0; i < nstreams; ++i) {
             hipStream_t q = nullptr;
             hip_check(hipStreamCreateWithFlags(&q, hipStreamNonBlocking), "stream");
@@ -548,8 +565,8 @@ void batch_run(kbsrc_batch* b, const double* d_in, double* d_out, hipStream_t s)
     const int ns = (int)b->streams.size();
     hip_check(hipEventRecord(b->events[ns], s), "batch fork");
     for (int i = 0; i < ns; ++i) hip_check(hipStreamWaitEvent(b->streams[i], b->events[ns], 0), "batch fork");
-    for (size_t g = 0; g < b->groups.size(); ++g)   // groups round-robin over the batch's streams
-        b->groups[g].eng->execute(d_in, d_out, b->streams[g % ns]);
+    for (size_t g = 0; g < b->groups.size(); ++g)   // each group on the stream its weight was given
+        b->groups[g].eng->execute(d_in, d_out, b->streams[b->groups[g].stream % ns]);
     for (const kbsrc_batch::Job& J : b->jobs)
         if (J.p.equal && J.out_len > 0)
             hip_check(hipMemcpyAsync(d_out + J.out_off, d_in + J.in_off, sizeof(double) * J.out_len,
