@@ -22,6 +22,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 RATE_IN, RATE_OUT, SECONDS, CH = 96000, 44100, 180, 2
+INT8_PEAK_OPS = 5.0e15   # MI355X dense int8 matrix peak
 FP64_PEAK_TFLOPS = 78.6   # MI355X published FP64 peak
 HBM_PEAK_GBS = 8000.0
 
@@ -536,9 +537,13 @@ def bench_split(args):
         ms = timed(kbsrc_amd.split_dft16, s)
         e = max(errors(xs, y[: sample * 64].cpu().numpy().reshape(sample, 16, 4), sample))
         d = float(((y - ref).abs().max() / ref.abs().max()).item())
+        # int8 matrix work: per 16 vectors, S (S + 1) / 2 digit pairs x 2 M-tiles of 16 x 16 x 32 MACs (2 ops each)
+        ops = (nvec / 16) * (s * (s + 1) / 2) * 2 * 16 * 16 * 32 * 2
         rows[s] = {"ms": round(ms, 4), "ps_per_point": round(ms * 1e9 / (16 * nvec), 2),
                    "err_vs_256bit_log2": round(float(np.log2(e)), 1), "diff_vs_plain_dd": d,
-                   "speed_vs_plain": round(plain_ms / ms, 3)}
+                   "speed_vs_plain": round(plain_ms / ms, 3),
+                   "int8_tops": round(ops / (ms / 1e3) / 1e12, 1),
+                   "int8_frac_of_dense_peak": round(ops / (ms / 1e3) / INT8_PEAK_OPS, 3)}
         print(f"slices {s}: {ms:.3f} ms, error 2^{np.log2(e):.1f}, x{plain_ms / ms:.3f} vs plain", file=sys.stderr,
               flush=True)
     ok = [s for s in rows if rows[s]["err_vs_256bit_log2"] <= -100]
@@ -755,6 +760,10 @@ def main():
     plan.execute(xx.data_ptr(), out.data_ptr(), stream)
     host = out.cpu()
     e2e_ms = (time.perf_counter() - t0) * 1e3
+    # kbsrc_resample end to end from host memory to host memory: plan, design, upload, staging, engine, download
+    t0 = time.perf_counter()
+    kbsrc_amd.resample(x, RATE_IN, RATE_OUT, args.mode)
+    resample_s = time.perf_counter() - t0
     graph_ms, graph_same = graph_timing(args, plan, xx, out)
     print(f"captured graph replay: {graph_ms:.3f} ms/step; same output: {graph_same}", file=sys.stderr, flush=True)
 
@@ -796,6 +805,7 @@ def main():
         "design_s": round(t_design, 3),
         "stage_s": round(t_stage, 3),
         "e2e_engine_plus_d2h_ms": round(e2e_ms, 3),
+        "resample_host_to_host_s": round(resample_s, 3),
         "graph_ms_per_step": round(graph_ms, 4),
         "graph_outputs_identical": graph_same,
         "tuned": bool(args.tune),
