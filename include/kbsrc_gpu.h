@@ -114,7 +114,9 @@ int kbsrc_batch_create(int njobs, const kbsrc_job* jobs, const kbsrc_options* op
 
 /* job >= 0: in_offset, in_doubles, out_offset, out_doubles, nout, group (-1 for an equal-rate copy); offsets in
  * doubles into the batch's device buffers; job's output is planar, channel c at out_offset + c * nout.
- * job == -1: 0, input buffer doubles, 0, output buffer doubles, jobs, groups. */
+ * job == -1: fused, input buffer doubles, 0, output buffer doubles, jobs, groups; fused = 1 when executes run one
+ * launch per step over every group (asked for with KBSRC_BATCH_FUSE=1 when the batch is created, and only when
+ * every group fits the generic pass kernel). */
 #define KBSRC_BATCH_FIELDS 6
 int kbsrc_batch_query(const kbsrc_batch* batch, int job, long long* values);
 

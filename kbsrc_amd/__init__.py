@@ -235,6 +235,7 @@ class Batch:
         v = (ctypes.c_longlong * 6)()
         _check(_lib.kbsrc_batch_query(self._h, -1, v))
         self.in_doubles, self.out_doubles, self.groups = v[1], v[3], v[5]
+        self.fused = bool(v[0])   # executes run one launch per step over every group
         self.layout = []
         keys = ["in_offset", "in_doubles", "out_offset", "out_doubles", "nout", "group"]
         for j in range(len(jobs)):

@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <cstring>
 #include <functional>
+#include <tuple>
 #include <memory>
 #include <new>
 #include <stdexcept>
@@ -132,6 +133,22 @@ struct kbsrc_batch {
     double design_seconds = 0;
     std::vector<hipStream_t> streams;
     std::vector<hipEvent_t> events;   // one per stream, plus the fork event
+    // One launch per step over every group (for batches of short jobs, whose per-shape launches leave the GPU
+    // idle): the generic pass kernel and the split/multiply/fold kernel over descriptor tables, made per pair of
+    // device buffers.
+    struct StepLaunch {
+        bool mid = false;
+        int policy = 0, ld = 0, st = 0, n = 0, total = 0;
+        size_t lds = 0;
+        DevBuf descs, starts;
+    };
+    struct Fused {
+        const double* in = nullptr;
+        double* out = nullptr;
+        std::vector<StepLaunch> launches;
+    };
+    bool fuse = false;
+    std::vector<std::unique_ptr<Fused>> fused;   // by buffers, at most four kept
     ~kbsrc_batch() {
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
         for (hipStream_t q : streams) (void)hipStreamDestroy(q);
@@ -511,6 +528,18 @@ int kbsrc_batch_create(int njobs, const kbsrc_job* jobs, const kbsrc_options* op
             G.eng = std::make_unique<Engine>(G.p, d.H.d(), d.nH, d.Jg, s, G.ntask,
                                              reinterpret_cast<const long long*>(G.tdesc.d()), per_group);
         }
+        // one launch per step over every group: opt-in (KBSRC_BATCH_FUSE=1). Measured slower than the per-group
+        // specialised launches on four streams for every batch tried (256 jobs of 0.5-2 s: 14.06 against 9.62 ms;
+        // 16 such jobs: 1.01 against 0.85 ms; 32 jobs of 10-30 s: 19.17 against 13.25 ms): the generic kernel's
+        // runtime stage list costs more than the launches it saves.
+        {
+            const char* v = std::getenv("KBSRC_BATCH_FUSE");
+            b->fuse = v && v[0] == '1';
+            for (kbsrc_batch::Group& G : b->groups) {   // every group must fit the generic-kernel form
+                EngineSteps probe;
+                if (b->fuse && !G.eng->describe(nullptr, nullptr, probe)) b->fuse = false;
+            }
+        }
         b->design_seconds = now() - t0;
         *batch = b.release();
     });
@@ -520,7 +549,7 @@ int kbsrc_batch_query(const kbsrc_batch* b, int job, long long* v) {
     return guarded([&] {
         if (!b || !v || job < -1 || job >= (int)b->jobs.size()) throw std::invalid_argument("bad input buffer");
         if (job < 0) {
-            v[0] = 0; v[1] = b->in_total; v[2] = 0; v[3] = b->out_total; v[4] = (long long)b->jobs.size();
+            v[0] = b->fuse; v[1] = b->in_total; v[2] = 0; v[3] = b->out_total; v[4] = (long long)b->jobs.size();
             v[5] = (long long)b->groups.size();
             return;
         }
@@ -560,8 +589,92 @@ int kbsrc_batch_stage(kbsrc_batch* b, int job, const double* interleaved, int on
 
 namespace {
 
+// The batch's steps as one launch each over every group (nullptr when a group does not fit that form)
+const kbsrc_batch::Fused* fused_steps(kbsrc_batch* b, const double* d_in, double* d_out, hipStream_t s) {
+    for (const auto& f : b->fused)
+        if (f->in == d_in && f->out == d_out) return f.get();
+    std::vector<EngineSteps> all(b->groups.size());
+    for (size_t g = 0; g < b->groups.size(); ++g)
+        if (!b->groups[g].eng->describe(d_in, d_out, all[g])) return nullptr;
+    auto F = std::make_unique<kbsrc_batch::Fused>();
+    F->in = d_in;
+    F->out = d_out;
+    // passes at position k of every group, one launch per (policy, load, store)
+    auto passes = [&](bool forward, size_t k) {
+        std::vector<std::tuple<int, int, int>> keys;
+        for (const EngineSteps& e : all) {
+            const std::vector<PassStep>& v = forward ? e.fwd : e.inv;
+            if (k < v.size()) {
+                const auto key = std::make_tuple(v[k].policy, v[k].ld, v[k].st);
+                if (std::find(keys.begin(), keys.end(), key) == keys.end()) keys.push_back(key);
+            }
+        }
+        for (const auto& key : keys) {
+            std::vector<PassArgs> d;
+            std::vector<int> starts;
+            kbsrc_batch::StepLaunch L;
+            std::tie(L.policy, L.ld, L.st) = key;
+            for (const EngineSteps& e : all) {
+                const std::vector<PassStep>& v = forward ? e.fwd : e.inv;
+                if (k >= v.size() || std::make_tuple(v[k].policy, v[k].ld, v[k].st) != key) continue;
+                starts.push_back(L.total);
+                d.push_back(v[k].a);
+                L.total += v[k].a.ngrp * v[k].a.ntask;
+                L.lds = std::max(L.lds, pass_lds_bytes(L.policy, v[k].a.sd.n, v[k].a.G));
+            }
+            L.n = (int)d.size();
+            L.descs.upload(d.data(), d.size() * sizeof(PassArgs), s);
+            L.starts.upload(starts.data(), starts.size() * sizeof(int), s);
+            F->launches.push_back(std::move(L));
+        }
+    };
+    passes(true, 0);
+    passes(true, 1);
+    for (int pol : {POL_DDA, POL_F64}) {
+        std::vector<MidArgs> d;
+        std::vector<int> starts;
+        kbsrc_batch::StepLaunch L;
+        L.mid = true;
+        L.policy = pol;
+        for (const EngineSteps& e : all) {
+            if (e.mid_policy != pol) continue;
+            starts.push_back(L.total);
+            d.push_back(e.mid);
+            L.total += mid_workgroups(e.mid);
+        }
+        if (d.empty()) continue;
+        L.n = (int)d.size();
+        L.descs.upload(d.data(), d.size() * sizeof(MidArgs), s);
+        L.starts.upload(starts.data(), starts.size() * sizeof(int), s);
+        F->launches.push_back(std::move(L));
+    }
+    passes(false, 0);
+    passes(false, 1);
+    hip_check(hipStreamSynchronize(s), "batch step tables");
+    if (b->fused.size() >= 4) b->fused.erase(b->fused.begin());
+    b->fused.push_back(std::move(F));
+    return b->fused.back().get();
+}
+
 // every group's engine on the batch's streams, forked from and joined back into s
 void batch_run(kbsrc_batch* b, const double* d_in, double* d_out, hipStream_t s) {
+    if (b->fuse) {
+        if (const kbsrc_batch::Fused* F = fused_steps(b, d_in, d_out, s)) {
+            for (const kbsrc_batch::StepLaunch& L : F->launches) {
+                const hipError_t e = L.mid
+                    ? launch_mid_multi(L.policy, reinterpret_cast<const MidArgs*>(L.descs.d()),
+                                       reinterpret_cast<const int*>(L.starts.d()), L.n, L.total, s)
+                    : launch_pass_multi(L.policy, L.ld, L.st, reinterpret_cast<const PassArgs*>(L.descs.d()),
+                                        reinterpret_cast<const int*>(L.starts.d()), L.n, L.total, L.lds, s);
+                hip_check(e, "batch step");
+            }
+            for (const kbsrc_batch::Job& J : b->jobs)
+                if (J.p.equal && J.out_len > 0)
+                    hip_check(hipMemcpyAsync(d_out + J.out_off, d_in + J.in_off, sizeof(double) * J.out_len,
+                                             hipMemcpyDeviceToDevice, s), "copy");
+            return;
+        }
+    }
     const int ns = (int)b->streams.size();
     hip_check(hipEventRecord(b->events[ns], s), "batch fork");
     for (int i = 0; i < ns; ++i) hip_check(hipStreamWaitEvent(b->streams[i], b->events[ns], 0), "batch fork");
@@ -620,6 +733,7 @@ int kbsrc_plan_capture(kbsrc_plan* pl, const double* d_padded, double* d_out, kb
 int kbsrc_batch_capture(kbsrc_batch* b, const double* d_in, double* d_out, kbsrc_graph** graph) {
     return guarded([&] {
         if (!b || !d_in || !d_out || !graph) throw std::invalid_argument("bad input buffer");
+        if (b->fuse) fused_steps(b, d_in, d_out, b->streams[0]);   // tables made outside the capture
         *graph = capture([&](hipStream_t cs) { batch_run(b, d_in, d_out, cs); });
     });
 }

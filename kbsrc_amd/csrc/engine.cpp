@@ -468,6 +468,26 @@ template <class F> void timed(const RunCtx& c, double* acc, F&& fn) {
 
 }  // namespace
 
+namespace {
+
+// launch one pass, or describe it for the generic kernel when the context collects
+void do_pass(const RunCtx& c, double* t, int policy, int ld, int st, PassArgs a, const char* what) {
+    if (c.collect) {
+        const bool io = (ld == LD_REAL_F64 || ld == LD_CPLX) && (st == ST_CPLX || st == ST_OUT);
+        if (!io || pass_lds_bytes(policy, a.sd.n, 1) > kLdsMax || a.split) {
+            *c.collect_ok = false;
+            return;
+        }
+        a.G = pick_group(policy, a.sd.n, a.nsub);
+        a.ngrp = (a.nsub + a.G - 1) / a.G;
+        c.collect->push_back(PassStep{policy, ld, st, a});
+        return;
+    }
+    timed(c, t, [&] { run_pass(policy, ld, st, a, c.s, what); });
+}
+
+}  // namespace
+
 void run_cfft(const CFFTPlan& f, const RunCtx& c, int ld, const double* src, int st, double* dst,
               double* const scr[3], double* t_col, double* t_row) {
     const bool lane = c.lane;
@@ -478,6 +498,10 @@ void run_cfft(const CFFTPlan& f, const RunCtx& c, int ld, const double* src, int
         RunCtx all = c;
         all.proto.zkeep = -1;
         run_cfft(f, all, ld, src, st, dst, scr, t_col, t_row);
+        return;
+    }
+    if (f.blue && c.collect) {
+        *c.collect_ok = false;
         return;
     }
     if (f.blue) {   // X_k = conj(b_k) sum_j x_j conj(b_j) b_{k-j}, through the inner FFT twice (src/dsp/fft.hpp:268-278)
@@ -507,19 +531,19 @@ void run_cfft(const CFFTPlan& f, const RunCtx& c, int ld, const double* src, int
         a.tw4 = f.tw4.d(); a.twmode = lane ? 1 : 2; a.renorm = lane && c.dd;
         a.f.M = f.N2;
         a.zkeep = -1;   // the column pass's outputs all feed the rows
-        timed(c, t_col, [&] { run_pass(pol_four, ld, ST_CPLX, a, c.s, "column pass"); });
+        do_pass(c, t_col, pol_four, ld, ST_CPLX, a, "column pass");
         PassArgs b = c.proto;
         fill_sub(b, f.s2, *c.k, c.dd, pol_four, f.N1);
         b.src = scr[0]; b.src_tstride = ts; b.lde = 1; b.ldg = f.N2;
         b.dst = dst; b.dst_tstride = ts; b.ste = f.N1; b.stg = 1; b.renorm = lane && c.dd;
         b.f.M = f.N1;
-        timed(c, t_row, [&] { run_pass(pol_four, LD_CPLX, st, b, c.s, "row pass"); });
+        do_pass(c, t_row, pol_four, LD_CPLX, st, b, "row pass");
     } else {
         PassArgs a = c.proto;
         fill_sub(a, f.sd, *c.k, c.dd, pol_one, 1);
         a.src = src; a.src_tstride = src_ts; a.lde = 1; a.ldg = 0;
         a.dst = dst; a.dst_tstride = ts; a.ste = 1; a.stg = 0;
-        timed(c, t_row, [&] { run_pass(pol_one, ld, st, a, c.s, "single pass"); });
+        do_pass(c, t_row, pol_one, ld, st, a, "single pass");
     }
 }
 
@@ -680,26 +704,9 @@ void Engine::execute(const double* xx, double* out, hipStream_t s, PassTimes* ti
     }
     const bool fused = !split && use_fixed() && env_flag("KBSRC_FUSED") && !Jg_ && fw_.c.four && iv_.c.four &&
                        has_fixed(pol_four, fw_.c.N2, LD_CPLX, ST_SPLIT) && has_fixed(pol_four, iv_.c.N1, LD_FOLD, ST_CPLX);
-    RunCtx c;
-    c.ts = ts_; c.k = &k_; c.dd = dd_; c.lane = true; c.s = s; c.times = times ? &local : nullptr;
+    RunCtx c = context(xx, out, s, split);
+    c.times = times ? &local : nullptr;
     PassArgs& a = c.proto;
-    a.gscr0 = g_[0].d();
-    a.gscr1 = g_[1].d();
-    a.tdesc = tdesc_;
-    a.split = split;
-    a.adig = split ? reinterpret_cast<const long*>(adig_[split].d()) : nullptr;
-    a.nblk = p_.nblk;
-    a.in_cstride = p_.xstride;
-    a.in_bstride = (long long)p_.hop * p_.Q;
-    a.out = out;
-    a.nout = p_.nout;
-    a.o = p_.o;
-    a.margP = (long long)p_.marg * p_.P;
-    a.hopP = (long long)p_.hop * p_.P;
-    FuseArgs& fa = a.f;
-    fa.H = H_.d(); fa.wf = fw_.w.d(); fa.wi = iv_.w.d();
-    fa.h = fw_.h; fa.hinv = iv_.h; fa.Nb = p_.Nb; fa.Mb = p_.Mb; fa.K = p_.Mb / 2 + 1;
-    fa.jmax = std::min(p_.Nb / 2, p_.Mb - 1); fa.up = p_.up; fa.xmax = p_.up ? fw_.h : fa.jmax;
     double* W0 = w_[0].d();
     double* W1 = w_[1].d();
     double* W2 = w_[2].d();
@@ -733,6 +740,61 @@ void Engine::execute(const double* xx, double* out, hipStream_t s, PassTimes* ti
         run_cfft(iv_.c, c, LD_CPLX, W1, ST_OUT, nullptr, si, &local.inv_col, &local.inv_row);
     }
     if (times) *times = local;
+}
+
+RunCtx Engine::context(const double* xx, double* out, hipStream_t s, int split) {
+    (void)xx;
+    RunCtx c;
+    c.ts = ts_; c.k = &k_; c.dd = dd_; c.lane = true; c.s = s;
+    PassArgs& a = c.proto;
+    a.gscr0 = g_[0].d();
+    a.gscr1 = g_[1].d();
+    a.tdesc = tdesc_;
+    a.split = split;
+    a.adig = split ? reinterpret_cast<const long*>(adig_[split].d()) : nullptr;
+    a.nblk = p_.nblk;
+    a.in_cstride = p_.xstride;
+    a.in_bstride = (long long)p_.hop * p_.Q;
+    a.out = out;
+    a.nout = p_.nout;
+    a.o = p_.o;
+    a.margP = (long long)p_.marg * p_.P;
+    a.hopP = (long long)p_.hop * p_.P;
+    FuseArgs& fa = a.f;
+    fa.H = H_.d(); fa.wf = fw_.w.d(); fa.wi = iv_.w.d();
+    fa.h = fw_.h; fa.hinv = iv_.h; fa.Nb = p_.Nb; fa.Mb = p_.Mb; fa.K = p_.Mb / 2 + 1;
+    fa.jmax = std::min(p_.Nb / 2, p_.Mb - 1); fa.up = p_.up; fa.xmax = p_.up ? fw_.h : fa.jmax;
+    return c;
+}
+
+bool Engine::describe(const double* xx, double* out, EngineSteps& steps) {
+    if (ntask_ > tb_) return false;
+    bool ok = true;
+    RunCtx c = context(xx, out, nullptr, 0);
+    c.collect_ok = &ok;
+    PassArgs& a = c.proto;
+    a.ntask = (int)ntask_;
+    a.task0 = 0;
+    double* W0 = w_[0].d();
+    double* W1 = w_[1].d();
+    double* W2 = w_[2].d();
+    double* W3 = w_[3].d();
+    double* const sf[3] = {W1, W2, W3};
+    c.collect = &steps.fwd;
+    a.zkeep = zkeep_;
+    a.zh = fw_.h;
+    run_cfft(fw_.c, c, LD_REAL_F64, xx, ST_CPLX, W0, sf, nullptr, nullptr);
+    a.zkeep = -1;
+    MidArgs& m = steps.mid;
+    m.z = W0; m.z_tstride = ts_; m.zi = W1; m.zi_tstride = ts_;
+    m.H = H_.d(); m.wf = fw_.w.d(); m.wi = iv_.w.d();
+    m.h = fw_.h; m.hinv = iv_.h; m.Nb = p_.Nb; m.Mb = p_.Mb; m.K = p_.Mb / 2 + 1;
+    m.jmax = std::min(p_.Nb / 2, p_.Mb - 1); m.up = p_.up; m.ntask = (int)ntask_; m.Jg = Jg_;
+    steps.mid_policy = dd_ ? POL_DDA : POL_F64;
+    double* const si[3] = {W0, W2, W3};
+    c.collect = &steps.inv;
+    run_cfft(iv_.c, c, LD_CPLX, W1, ST_OUT, nullptr, si, nullptr, nullptr);
+    return ok && mid_workgroups(m) > 0;
 }
 
 }  // namespace kbsrc::gpu

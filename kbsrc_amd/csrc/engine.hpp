@@ -98,6 +98,11 @@ struct PassTimes {   // milliseconds per kernel kind, summed over one execute()
 // How a complex FFT runs: lane = the reference's engine form (sub-FFTs on its 4-lane types: one-TwoSum dd adds,
 // renormalised stores, four-step twiddle on every k1 != 0), otherwise its scalar form (accurate dd adds, no twiddle
 // on column 0), which the kernel design and the Bluestein chirp spectrum use.
+// A sub-FFT pass described for the generic kernel (G and ngrp set for 256 threads) instead of launched.
+struct PassStep {
+    int policy, ld, st;
+    PassArgs a;
+};
 struct RunCtx {
     PassArgs proto;        // tasks, block addressing, output window, fused-pass arguments
     long long ts = 0;      // doubles per task in every work buffer
@@ -105,6 +110,14 @@ struct RunCtx {
     bool dd = true, lane = true;
     hipStream_t s = nullptr;
     PassTimes* times = nullptr;
+    std::vector<PassStep>* collect = nullptr;   // set: passes are described here, not launched
+    bool* collect_ok = nullptr;                 // cleared when a pass cannot run on the generic kernel
+};
+// An engine's whole execute as generic-kernel steps: forward passes, the split/multiply/fold, inverse passes.
+struct EngineSteps {
+    std::vector<PassStep> fwd, inv;
+    MidArgs mid;
+    int mid_policy = 0;
 };
 
 // Runs f over proto.ntask tasks: load kind ld from src, store kind st to dst (work buffers, stride ctx.ts).
@@ -148,11 +161,16 @@ public:
     // split > 0: the split-precision form (exact-mode plans only; not bit-exact): sub-FFTs whose first two stages
     // are radix 4 run them as one int8-MFMA radix-16 stage with `split` base-128 slices per operand.
     void execute(const double* xx, double* out, hipStream_t s, PassTimes* times = nullptr, int split = 0);
+    // The same execute described as generic-kernel steps on these buffers (for one launch per step over several
+    // engines); false when it does not fit that form (Bluestein lengths, sub-FFTs longer than an LDS tile, more
+    // tasks than one batch of the work buffers).
+    bool describe(const double* xx, double* out, EngineSteps& steps);
     long long tasks() const { return ntask_; }
     int batch() const { return tb_; }
     const Plan& plan() const { return p_; }
 
 private:
+    RunCtx context(const double* xx, double* out, hipStream_t s, int split);
     Plan p_;
     bool dd_;
     int Jg_ = 0;

@@ -118,11 +118,11 @@ __device__ __forceinline__ void bfly_odd(const Planes<A>& cur, const Planes<A>& 
 // SPL > 0 (split form, dd policies, PassArgs::pm == 0): the first two radix-4 stages run as one int8-MFMA
 // radix-16 stage with SPL slices (split_core.hpp); y[16 p + k] = w_n^{p k} DFT16(x[p + (n / 16) m])[k], the
 // twiddle in dd with the accurate add, as the global-memory route's stage computes it.
-template <class A, int LD, int ST, int SPL = 0>
-__global__ __launch_bounds__(256) void k_subfft(PassArgs a) {
+template <class A, int LD, int ST, int SPL>
+__device__ __forceinline__ void subfft_body(const PassArgs& a, int bid) {
     extern __shared__ double lds[];
     const int tid = threadIdx.x, nthr = blockDim.x;
-    const int task = blockIdx.x / a.ngrp, grp = blockIdx.x % a.ngrp;
+    const int task = bid / a.ngrp, grp = bid % a.ngrp;
     const int Gs = a.G, j0 = grp * Gs;
     const int G = min(Gs, a.nsub - j0);
     const int n = a.sd.n;
@@ -270,6 +270,22 @@ __global__ __launch_bounds__(256) void k_subfft(PassArgs a) {
     }
 }
 
+template <class A, int LD, int ST, int SPL = 0>
+__global__ __launch_bounds__(256) void k_subfft(PassArgs a) {
+    subfft_body<A, LD, ST, SPL>(a, blockIdx.x);
+}
+
+// One launch over the passes of several plan shapes (a batch of short jobs): workgroups [starts[s], starts[s + 1])
+// run shape s's pass d[s] as k_subfft would.
+template <class A, int LD, int ST>
+__global__ __launch_bounds__(256) void k_subfft_multi(const PassArgs* __restrict__ d, const int* __restrict__ starts,
+                                                      int n) {
+    const int bid = blockIdx.x;
+    int s = 0;
+    while (s + 1 < n && bid >= starts[s + 1]) ++s;
+    subfft_body<A, LD, ST, 0>(d[s], bid - starts[s]);
+}
+
 // ---- forward split X[i] from the complex FFT output z (src/dsp/fft.hpp:365-407)
 template <class A> __device__ __forceinline__ Cx<A> fwd_split(const double* z, const double* w, int h, int i) {
     if (i == 0 || i == h) {
@@ -360,11 +376,11 @@ template <class A> __device__ __forceinline__ Cx<A> pre_v(Cx<A> Y, Cx<A> Yp, Cx<
     return c_conj<A>(Cx<A>{A::sub(E.re, D.im), A::add(E.im, D.re)});
 }
 
-template <class A> __global__ __launch_bounds__(256) void k_mid(MidArgs a) {
+template <class A> __device__ __forceinline__ void mid_body(const MidArgs& a, unsigned bid) {
     // task-fastest workgroup order: the tasks of one bin range run together, so the H and split-twiddle slices
     // they share stay in L2
-    const int task = (int)(blockIdx.x % (unsigned)a.ntask);
-    const int k = (int)(blockIdx.x / (unsigned)a.ntask) * blockDim.x + threadIdx.x;
+    const int task = (int)(bid % (unsigned)a.ntask);
+    const int k = (int)(bid / (unsigned)a.ntask) * blockDim.x + threadIdx.x;
     if (k > a.hinv / 2) return;
     const double* z = a.z + task * a.z_tstride;
     double* zi = a.zi + task * a.zi_tstride;
@@ -382,6 +398,16 @@ template <class A> __global__ __launch_bounds__(256) void k_mid(MidArgs a) {
             gstore<A>(zi, j, c_conj<A>(Cx<A>{A::sub(Ej.re, Dj.im), A::add(Ej.im, Dj.re)}));
         }
     }
+}
+
+template <class A> __global__ __launch_bounds__(256) void k_mid(MidArgs a) { mid_body<A>(a, blockIdx.x); }
+
+template <class A>
+__global__ __launch_bounds__(256) void k_mid_multi(const MidArgs* __restrict__ d, const int* __restrict__ starts, int n) {
+    const int bid = blockIdx.x;
+    int s = 0;
+    while (s + 1 < n && bid >= starts[s + 1]) ++s;
+    mid_body<A>(d[s], (unsigned)(bid - starts[s]));
 }
 
 __global__ __launch_bounds__(256) void k_stage(StageArgs a) {
@@ -527,6 +553,42 @@ hipError_t launch_pass(int policy, int ld, int st, const PassArgs& a, int thread
     KB_CASE(POL_F64, F64, LD_CPLX, ST_CPLX)
     KB_CASE(POL_F64, F64, LD_CPLX, ST_OUT)
 #undef KB_CASE
+    return hipErrorInvalidValue;
+}
+
+int mid_workgroups(const MidArgs& a) {
+    const unsigned long long n = (unsigned long long)((a.hinv / 2 + 1 + 255) / 256) * (unsigned long long)a.ntask;
+    return a.ntask < 1 || n > 0x7fffffffULL ? -1 : (int)n;
+}
+
+hipError_t launch_mid_multi(int policy, const MidArgs* d, const int* starts, int n, int total, hipStream_t s) {
+    if (n < 1 || total < 1) return hipErrorInvalidValue;
+    if (policy == POL_F64) hipLaunchKernelGGL(k_mid_multi<F64>, dim3(total), dim3(256), 0, s, d, starts, n);
+    else hipLaunchKernelGGL(k_mid_multi<DDA>, dim3(total), dim3(256), 0, s, d, starts, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_pass_multi(int policy, int ld, int st, const PassArgs* d, const int* starts, int n, int total,
+                             size_t lds, hipStream_t s) {
+    if (n < 1 || total < 1 || lds > 160 * 1024) return hipErrorInvalidValue;
+#define KB_MULTI(P, POL, L, S)                                                                                     \
+    if (policy == P && ld == L && st == S) {                                                                       \
+        static const hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_subfft_multi<POL, L, S>), \
+                                                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024); \
+        if (attr != hipSuccess) return attr;                                                                       \
+        hipLaunchKernelGGL((k_subfft_multi<POL, L, S>), dim3(total), dim3(256), lds, s, d, starts, n);              \
+        return hipGetLastError();                                                                                   \
+    }
+    KB_MULTI(POL_DDS, DDS, LD_REAL_F64, ST_CPLX)
+    KB_MULTI(POL_DDS, DDS, LD_CPLX, ST_CPLX)
+    KB_MULTI(POL_DDS, DDS, LD_CPLX, ST_OUT)
+    KB_MULTI(POL_DDA, DDA, LD_REAL_F64, ST_CPLX)
+    KB_MULTI(POL_DDA, DDA, LD_CPLX, ST_CPLX)
+    KB_MULTI(POL_DDA, DDA, LD_CPLX, ST_OUT)
+    KB_MULTI(POL_F64, F64, LD_REAL_F64, ST_CPLX)
+    KB_MULTI(POL_F64, F64, LD_CPLX, ST_CPLX)
+    KB_MULTI(POL_F64, F64, LD_CPLX, ST_OUT)
+#undef KB_MULTI
     return hipErrorInvalidValue;
 }
 

@@ -211,6 +211,13 @@ bool has_fixed(int policy, int n, int ld, int st);
 // Sub-FFT pass through global memory, one launch per stage (lengths whose tile does not fit LDS).
 hipError_t launch_global(int policy, int ld, int st, const PassArgs& a, hipStream_t s);
 hipError_t launch_mid(int policy, const MidArgs& a, hipStream_t s);
+// One launch over several plan shapes' passes (device arrays: descriptors d[0..n), workgroup starts[0..n) with shape s
+// taking [starts[s], starts[s + 1]) of total): the generic sub-FFT pass (each d[s] with its G, ngrp set for 256
+// threads; lds = the largest tile) and the split/multiply/fold kernel (mid_workgroups each).
+hipError_t launch_pass_multi(int policy, int ld, int st, const PassArgs* d, const int* starts, int n, int total,
+                             size_t lds, hipStream_t s);
+hipError_t launch_mid_multi(int policy, const MidArgs* d, const int* starts, int n, int total, hipStream_t s);
+int mid_workgroups(const MidArgs& a);   // workgroups of launch_mid for a, -1 if too many
 hipError_t launch_stage(const StageArgs& a, hipStream_t s);
 // policy: POL_DDA or POL_F64; kind: the LD kind (pre) or ST kind (post)
 hipError_t launch_blue(int policy, int phase, int kind, const BlueArgs& a, hipStream_t s);

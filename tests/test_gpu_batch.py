@@ -42,6 +42,45 @@ def signals(jobs, seed):
     return [rng.uniform(-0.5, 0.5, (f, c)) for f, c, _, _ in jobs]
 
 
+# no Bluestein lengths (32001 and 44101 have prime factors above 7); 48000 -> 48020 (2401/2400) is a general fold
+FUSABLE = [j for j in JOBS if j[3] not in (32001, 44101)] + [(7000, 2, 48000, 96000), (3000, 1, 48000, 48020)]
+
+
+@pytest.mark.parametrize("mode", ["exact", "fast"])
+@pytest.mark.parametrize("fuse", ["1", "0"])
+def test_batch_one_launch_per_step(mode, fuse, monkeypatch):
+    """The batch run as one launch per step over every group (opt-in: the generic pass kernel and the
+    split/multiply/fold kernel over per-group descriptor tables) gives each job's own bits, as the per-group
+    launches do."""
+    import torch
+    monkeypatch.setenv("KBSRC_BATCH_FUSE", fuse)
+    say(f"batch {mode}, one launch per step = {fuse}")
+    xs = signals(FUSABLE, 22)
+    b = gpu().Batch([(x.shape[0], x.shape[1], ri, ro) for x, (_, _, ri, ro) in zip(xs, FUSABLE)], mode, length=0.05)
+    assert b.fused == (fuse == "1")
+    d_in = torch.zeros(b.in_doubles, dtype=torch.float64, device="cuda")
+    d_out = torch.zeros(b.out_doubles, dtype=torch.float64, device="cuda")
+    for j, x in enumerate(xs):
+        b.stage(j, x, d_in.data_ptr())
+    for rep in range(2):   # the descriptor tables are made on the first execute and reused
+        d_out.zero_()
+        b.execute(d_in.data_ptr(), d_out.data_ptr())
+        torch.cuda.synchronize()
+        out = d_out.cpu().numpy()
+        for (f, c, ri, ro), x, L in zip(FUSABLE, xs, b.layout):
+            y = out[L["out_offset"]:L["out_offset"] + L["out_doubles"]].reshape(c, L["nout"])
+            want = gpu().resample(x, ri, ro, mode, length=0.05)
+            assert y.tobytes() == want.tobytes(), (rep, f, c, ri, ro)
+    g = b.capture(d_in.data_ptr(), d_out.data_ptr())   # and as a graph
+    d_out.zero_()
+    torch.cuda.synchronize()
+    g.launch()
+    torch.cuda.synchronize()
+    assert d_out.cpu().numpy().tobytes() == out.tobytes()
+    g.close()
+    b.close()
+
+
 @pytest.mark.parametrize("mode", ["exact", "fast"])
 def test_batch_equals_single_jobs(mode):
     say(f"batch {mode}: {len(JOBS)} jobs")
