@@ -655,6 +655,10 @@ Engine::Engine(const Plan& p, const double* H, long long nH, int Jg, hipStream_t
                              : p.up ? std::min<long long>(h, p.Mb / 2) : std::min<long long>(h, p.Mb - 1);
         zkeep_ = xmax + 1 < h - xmax ? xmax : -1;
         if (env_flag("KBSRC_STORE_ALL")) zkeep_ = -1;   // test switch: store every bin (same output)
+        // down, plain bin map: X[i] reads z[i] and z[h - i], so the split/fold threads k and k +- |h - Mb| share
+        // their z reads; their bin blocks (256 bins each) run next to each other
+        if (!p.up && !Jg_) kchain_ = (int)((std::llabs(h - (long long)p.Mb) + 128) / 256);
+        if (const char* v = std::getenv("KBSRC_MID_ORDER"); v && std::atoi(v) == 0) kchain_ = 0;   // test switch
     }
     // H in the element format of the engine: (hi, lo) pairs or FP64
     H_.alloc((size_t)nH * (dd_ ? 16 : 8));
@@ -734,6 +738,7 @@ void Engine::execute(const double* xx, double* out, hipStream_t s, PassTimes* ti
         m.H = H_.d(); m.wf = fw_.w.d(); m.wi = iv_.w.d();
         m.h = fw_.h; m.hinv = iv_.h; m.Nb = p_.Nb; m.Mb = p_.Mb; m.K = p_.Mb / 2 + 1;
         m.jmax = std::min(p_.Nb / 2, p_.Mb - 1); m.up = p_.up; m.ntask = nt; m.Jg = Jg_;
+        m.kchain = kchain_;
         timed(c, &local.mid, [&] { hip_check(launch_mid(dd_ ? POL_DDA : POL_F64, m, s), "mid kernel"); });
         // inverse: z' (W1) -> output samples
         double* const si[3] = {W0, W2, W3};
@@ -790,6 +795,7 @@ bool Engine::describe(const double* xx, double* out, EngineSteps& steps) {
     m.H = H_.d(); m.wf = fw_.w.d(); m.wi = iv_.w.d();
     m.h = fw_.h; m.hinv = iv_.h; m.Nb = p_.Nb; m.Mb = p_.Mb; m.K = p_.Mb / 2 + 1;
     m.jmax = std::min(p_.Nb / 2, p_.Mb - 1); m.up = p_.up; m.ntask = (int)ntask_; m.Jg = Jg_;
+    m.kchain = kchain_;
     steps.mid_policy = dd_ ? POL_DDA : POL_F64;
     double* const si[3] = {W0, W2, W3};
     c.collect = &steps.inv;

@@ -176,6 +176,7 @@ private:
     int Jg_ = 0;
     long long ntask_ = 0, ts_ = 0;   // ts_: doubles per task in the work buffers
     long long zkeep_ = -1;           // forward spectrum bins above this (and their split partners) go unstored
+    int kchain_ = 0;                 // MidArgs::kchain
     const long long* tdesc_ = nullptr;
     int tb_ = 1;
     DeviceConsts k_;

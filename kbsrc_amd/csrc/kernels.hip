@@ -376,11 +376,29 @@ template <class A> __device__ __forceinline__ Cx<A> pre_v(Cx<A> Y, Cx<A> Yp, Cx<
     return c_conj<A>(Cx<A>{A::sub(E.re, D.im), A::add(E.im, D.re)});
 }
 
+// Bin block of the q-th block in the launch order. With c = kchain > 0 the nk blocks run as the columns of a
+// matrix with rows of c blocks (b, b + c, b + 2c, ..., then b + 1, ...): a thread's eight z reads are shared
+// with the threads |h - Mb| bins away, so both reads of a z bin fall close together in time and the second one
+// finds it in cache. A bijection of [0, nk).
+__device__ __forceinline__ int mid_block(int q, int nk, int c) {
+    if (c <= 0 || c >= nk) return q;
+    const int R = (nk + c - 1) / c, full = nk - (R - 1) * c;   // columns 0..full-1 hold R blocks, the rest R - 1
+    int j, i;
+    if (q < full * R) {
+        j = q / R; i = q - j * R;
+    } else {
+        const int r = q - full * R;
+        j = full + r / (R - 1); i = r - (j - full) * (R - 1);
+    }
+    return j + i * c;
+}
+
 template <class A> __device__ __forceinline__ void mid_body(const MidArgs& a, unsigned bid) {
     // task-fastest workgroup order: the tasks of one bin range run together, so the H and split-twiddle slices
     // they share stay in L2
     const int task = (int)(bid % (unsigned)a.ntask);
-    const int k = (int)(bid / (unsigned)a.ntask) * blockDim.x + threadIdx.x;
+    const int nk = (a.hinv / 2 + 1 + (int)blockDim.x - 1) / (int)blockDim.x;
+    const int k = mid_block((int)(bid / (unsigned)a.ntask), nk, a.kchain) * blockDim.x + threadIdx.x;
     if (k > a.hinv / 2) return;
     const double* z = a.z + task * a.z_tstride;
     double* zi = a.zi + task * a.zi_tstride;
