@@ -122,6 +122,72 @@ __device__ __forceinline__ void all_stages(const Lds<A, RL::N * G>& L, int tid, 
     (stage<A, RL, G, T, I>(L, tid, tw, oddc, pr), ...);
 }
 
+// Stages 1 .. nst-2 (the first and last run with their global-memory side, below).
+template <class A, class RL, int G, int T, int... I>
+__device__ __forceinline__ void inner_stages(const Lds<A, RL::N * G>& L, int tid, const double* tw, const double* oddc,
+                                             std::integer_sequence<int, I...>) {
+    const Prune none;
+    (stage<A, RL, G, T, I + 1>(L, tid, tw, oddc, none), ...);
+}
+
+// Stage 0 with its inputs read from global memory: load(e, j) gives element e of sub-FFT j. It has stride 1, so
+// butterfly b reads elements b + t (N / r) and writes r b + u; nothing in LDS is read, so it writes its outputs
+// straight away, and one sync follows.
+template <class A, class RL, int G, int T, class Jof, class Load>
+__device__ __forceinline__ void first_stage(const Lds<A, RL::N * G>& L, int tid, const double* tw, const double* oddc,
+                                            Jof jof, Load load) {
+    constexpr int N = RL::N, r = RL::r[0], nbf = N / r, W = G * nbf, IT = (W + T - 1) / T;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int w = tid + it * T;
+        if (W % T == 0 || w < W) {
+            const int g = w % G, b = w / G, j = jof(g);
+            if (j < 0) continue;
+            Cx<A> x[r], y[r];
+#pragma unroll
+            for (int t = 0; t < r; ++t) x[t] = load(b + t * nbf, j);
+            butterfly<A, r>(x, y, b != 0, tw + (long long)b * (r - 1) * kElt<A>, oddc);
+#pragma unroll
+            for (int u = 0; u < r; ++u) L.put((r * b + u) * G + g, y[u]);
+        }
+    }
+    __syncthreads();
+}
+
+// The last stage with its outputs written to global memory: store(e, j, v) takes element e of sub-FFT j. Its
+// stride is N / r, so butterfly b (no twiddles) writes elements b + u (N / r). Butterflies whose outputs are all
+// unread are skipped as in stage().
+template <class A, class RL, int G, int T, class Jof, class Store>
+__device__ __forceinline__ void last_stage(const Lds<A, RL::N * G>& L, int tid, const double* tw, const double* oddc,
+                                           const Prune& pr, Jof jof, Store store) {
+    constexpr int I = RL::nst - 1, N = RL::N, r = RL::r[I], s = RL::stride(I), nbf = N / r, W = G * nbf,
+                  IT = (W + T - 1) / T;
+    static_assert(s == nbf, "the last stage has stride N / r");
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int w = tid + it * T;
+        if (W % T == 0 || w < W) {
+            const int g = w % G, b = w / G, j = jof(g);
+            if (j < 0) continue;
+            if (pr.keep >= 0) {
+                bool need = false;
+#pragma unroll
+                for (int u = 0; u < r; ++u) {
+                    const long long i = (long long)(b + u * s) * pr.ste + (long long)j * pr.stg;
+                    need = need || i <= pr.keep || i >= pr.zh - pr.keep;
+                }
+                if (!need) continue;
+            }
+            Cx<A> x[r], y[r];
+#pragma unroll
+            for (int t = 0; t < r; ++t) x[t] = L.get((b + t * nbf) * G + g);
+            butterfly<A, r>(x, y, false, tw, oddc);
+#pragma unroll
+            for (int u = 0; u < r; ++u) store(b + u * s, j, y[u]);
+        }
+    }
+}
+
 // Workgroup index remapped so that each of the 8 XCDs (which take workgroups round-robin) works on a contiguous
 // range of the grid: neighbouring sub-FFT groups share cache lines in that XCD's L2.
 __device__ __forceinline__ int xcd_wg() {
@@ -218,10 +284,13 @@ template <class A> __device__ __forceinline__ Cx<A> renorm_dd(Cx<A> v) {
 
 // __launch_bounds__'s second value is the minimum number of waves per SIMD the register allocation must allow:
 // 4 caps a thread at 128 VGPRs, so that LDS (not registers) limits occupancy.
-template <class A, class RL, int G, int T, int LD, int ST>
+// IO: the first stage reads global memory and the last stage writes it (first_stage, last_stage), instead of a
+// load into LDS, a sync, the first stage, ..., the last stage, a sync and a store from LDS; same operations.
+template <class A, class RL, int G, int T, int LD, int ST, bool IO = false>
 __global__ __launch_bounds__(T, 4) void k_fixed(PassArgs a) {
     constexpr int N = RL::N, CAP = N * G;
     constexpr bool kPaired = LD == LD_FOLD || ST == ST_SPLIT;
+    static_assert(!IO || (!kPaired && RL::nst >= 2), "global-memory stages need an unpaired pass of two stages");
     using S = SplitPol<A>;
     static_assert(std::is_same_v<Val<S>, Val<A>>);
     extern __shared__ double lds[];   // kPlanes * CAP doubles
@@ -235,6 +304,49 @@ __global__ __launch_bounds__(T, 4) void k_fixed(PassArgs a) {
     auto jof = [&](int g) -> int {
         if constexpr (kPaired) return slot_index<G>(g, grp, a.f.M);
         else return j0 + g < a.nsub ? j0 + g : -1;
+    };
+    // element e of sub-FFT j from the pass's source
+    auto load_at = [&](int e, int j) -> Cx<A> {
+        const long long i = e * a.lde + (long long)j * a.ldg;
+        if constexpr (LD == LD_REAL_F64) {
+            const double2 pr = reinterpret_cast<const double2*>(a.src + io.in_off)[i];
+            return {A::lift(pr.x), A::lift(pr.y)};
+        } else {
+            return gld<A>(a.src + task * a.src_tstride, i);
+        }
+    };
+    // element e of sub-FFT j's result to the pass's destination: four-step twiddle, renormalisation, then the
+    // complex store or the overlap discard (samples 2i and 2i + 1 of the block's output)
+    auto store_at = [&](int e, int j, Cx<A> v) {
+        const long long jg = j;
+        const long long i = e * a.ste + jg * a.stg;
+        if (ST == ST_CPLX && z_unused(a, i)) return;
+        if (a.twmode && e != 0 && (a.twmode == 1 || jg != 0)) v = c_mul<A>(v, gld<A>(a.tw4, i));
+        if constexpr (A::kDD) {
+            if (a.renorm) v = renorm_dd<A>(v);
+        }
+        if constexpr (ST == ST_CPLX) {
+            gst<A>(a.dst + task * a.dst_tstride, i, v);
+        } else if constexpr (ST == ST_OUT) {
+            double* o = a.out + io.out_off;
+            const long long x0 = 2 * i - a.margP;
+            if (x0 >= 0 && x0 < a.hopP) {
+                const long long d = io.blk * a.hopP + x0 - a.o;
+                if (d >= 0 && d < io.nout) {
+                    if constexpr (A::kDD) o[d] = v.re.hi + v.re.lo;
+                    else o[d] = v.re;
+                }
+            }
+            const long long x1 = x0 + 1;
+            if (x1 >= 0 && x1 < a.hopP) {
+                const long long d = io.blk * a.hopP + x1 - a.o;
+                if (d >= 0 && d < io.nout) {
+                    const Val<A> ni = A::neg(v.im);
+                    if constexpr (A::kDD) o[d] = ni.hi + ni.lo;
+                    else o[d] = ni;
+                }
+            }
+        }
     };
 
     if constexpr (LD == LD_FOLD) {
@@ -283,29 +395,27 @@ __global__ __launch_bounds__(T, 4) void k_fixed(PassArgs a) {
                 if (jof(g) >= 0) L.put(e * G + g, zs[it]);
             }
         }
-    } else {
+    } else if constexpr (!IO) {
         // load, lanes sub-FFT-fastest: a wave writes one contiguous LDS range (no bank conflicts) and reads 64/G
         // consecutive elements of each of its G sub-FFTs (or G neighbouring columns of 64/G rows)
         for (int w = tid; w < G * N; w += T) {
             const int g = w % G, e = w / G;
             const int j = jof(g);
             if (j < 0) continue;
-            const long long i = e * a.lde + (long long)j * a.ldg;
-            Cx<A> v;
-            if constexpr (LD == LD_REAL_F64) {
-                const double2 pr = reinterpret_cast<const double2*>(a.src + io.in_off)[i];
-                v = {A::lift(pr.x), A::lift(pr.y)};
-            } else {
-                v = gld<A>(a.src + task * a.src_tstride, i);
-            }
-            L.put(e * G + g, v);
+            L.put(e * G + g, load_at(e, j));
         }
     }
-    __syncthreads();
     Prune pr;
     if constexpr (ST == ST_CPLX && !kPaired) {
         if (a.zkeep >= 0) pr = Prune{a.zkeep, a.zh, a.ste, a.stg, j0};
     }
+    if constexpr (IO) {
+        first_stage<A, RL, G, T>(L, tid, a.tw, a.oddc, jof, load_at);
+        inner_stages<A, RL, G, T>(L, tid, a.tw, a.oddc, std::make_integer_sequence<int, RL::nst - 2>{});
+        last_stage<A, RL, G, T>(L, tid, a.tw, a.oddc, pr, jof, store_at);
+        return;
+    }
+    __syncthreads();
     all_stages<A, RL, G, T>(L, tid, a.tw, a.oddc, pr, std::make_integer_sequence<int, RL::nst>{});
 
     if constexpr (ST == ST_SPLIT) {
@@ -331,53 +441,27 @@ __global__ __launch_bounds__(T, 4) void k_fixed(PassArgs a) {
             const int g = w % G, e = w / G;
             const int j = jof(g);
             if (j < 0) continue;
-            const long long jg = j;
-            const long long i = e * a.ste + jg * a.stg;
-            if (ST == ST_CPLX && z_unused(a, i)) continue;
-            Cx<A> v = L.get(e * G + g);
-            if (a.twmode && e != 0 && (a.twmode == 1 || jg != 0)) v = c_mul<A>(v, gld<A>(a.tw4, i));
-            if constexpr (A::kDD) {
-                if (a.renorm) v = renorm_dd<A>(v);
-            }
-            if constexpr (ST == ST_CPLX) {
-                gst<A>(a.dst + task * a.dst_tstride, i, v);
-            } else {
-                double* o = a.out + io.out_off;
-                const long long x0 = 2 * i - a.margP;
-                if (x0 >= 0 && x0 < a.hopP) {
-                    const long long d = io.blk * a.hopP + x0 - a.o;
-                    if (d >= 0 && d < io.nout) {
-                        if constexpr (A::kDD) o[d] = v.re.hi + v.re.lo;
-                        else o[d] = v.re;
-                    }
-                }
-                const long long x1 = x0 + 1;
-                if (x1 >= 0 && x1 < a.hopP) {
-                    const long long d = io.blk * a.hopP + x1 - a.o;
-                    if (d >= 0 && d < io.nout) {
-                        const Val<A> ni = A::neg(v.im);
-                        if constexpr (A::kDD) o[d] = ni.hi + ni.lo;
-                        else o[d] = ni;
-                    }
-                }
-            }
+            store_at(e, j, L.get(e * G + g));
         }
     }
 }
 
 
-template <class A, class RL, int G, int T, int LD, int ST> hipError_t launch_fixed_one(const PassArgs& a, hipStream_t s) {
+// Unpaired passes of two or more stages run their first and last stages on global memory (k_fixed's IO).
+template <class A, class RL, int G, int T, int LD, int ST,
+          bool IO = LD != LD_FOLD && ST != ST_SPLIT && RL::nst >= 2>
+hipError_t launch_fixed_one(const PassArgs& a, hipStream_t s) {
     static_assert(RL::hold(G, T) <= (A::kDD ? 8 : 16), "too many values held per thread");
     static_assert(LD != LD_FOLD || (G * RL::N + T - 1) / T <= (A::kDD ? 8 : 16), "too many values held per thread");
     static_assert((LD != LD_FOLD && ST != ST_SPLIT) || G % 2 == 0, "paired passes need an even group");
     constexpr size_t bytes = (size_t)kPlanes<A> * RL::N * G * sizeof(double);
     static_assert(bytes <= 160 * 1024, "LDS tile too large");
     static const hipError_t attr = bytes > 64 * 1024
-        ? hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fixed<A, RL, G, T, LD, ST>),
+        ? hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fixed<A, RL, G, T, LD, ST, IO>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes)
         : hipSuccess;
     if (attr != hipSuccess) return attr;
-    hipLaunchKernelGGL((k_fixed<A, RL, G, T, LD, ST>), dim3(a.ngrp * a.ntask), dim3(T), bytes, s, a);
+    hipLaunchKernelGGL((k_fixed<A, RL, G, T, LD, ST, IO>), dim3(a.ngrp * a.ntask), dim3(T), bytes, s, a);
     return hipGetLastError();
 }
 

@@ -13,27 +13,36 @@ namespace {
 // and KBSRC_TUNE="n:k,..." selects the k-th for length n (the sweep that picks the defaults).
 const FixedEntry kEntries[] = {
     // exact (four-step sub-FFTs with the one-TwoSum add)
-    KB_FIXED(POL_DDS, 2, 128, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),    // 315: 96k->44.1k forward columns
+    KB_FIXED(POL_DDS, 8, 512, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),    // 315: 96k->44.1k forward columns
+    KB_FIXED(POL_DDS, 2, 128, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),
     KB_FIXED(POL_DDS, 4, 448, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),
     KB_FIXED(POL_DDS, 4, 512, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),
     KB_FIXED(POL_DDS, 7, 512, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),
-    KB_FIXED(POL_DDS, 8, 512, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),
-    KB_FIXED(POL_DDS, 8, 512, LD_CPLX, ST_CPLX, 4, 4, 4, 5),        // 320: 96k->44.1k forward rows
+    KB_FIXED(POL_DDS, 1, 64, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),
+    KB_FIXED(POL_DDS, 3, 192, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),
+    KB_FIXED(POL_DDS, 4, 256, LD_CPLX, ST_CPLX, 4, 4, 4, 5),        // 320: 96k->44.1k forward rows
+    KB_FIXED(POL_DDS, 8, 512, LD_CPLX, ST_CPLX, 4, 4, 4, 5),
     KB_FIXED(POL_DDS, 4, 320, LD_CPLX, ST_CPLX, 4, 4, 4, 5),
-    KB_FIXED(POL_DDS, 4, 256, LD_CPLX, ST_CPLX, 4, 4, 4, 5),
     KB_FIXED(POL_DDS, 4, 512, LD_CPLX, ST_CPLX, 4, 4, 4, 5),
     KB_FIXED(POL_DDS, 2, 128, LD_CPLX, ST_CPLX, 4, 4, 4, 5),
     KB_FIXED(POL_DDS, 6, 384, LD_CPLX, ST_CPLX, 4, 4, 4, 5),
-    KB_FIXED(POL_DDS, 8, 512, LD_CPLX, ST_CPLX, 3, 3, 3, 7),        // 189: 96k->44.1k inverse columns
+    KB_FIXED(POL_DDS, 8, 256, LD_CPLX, ST_CPLX, 3, 3, 3, 7),        // 189: 96k->44.1k inverse columns
+    KB_FIXED(POL_DDS, 8, 512, LD_CPLX, ST_CPLX, 3, 3, 3, 7),
     KB_FIXED(POL_DDS, 5, 192, LD_CPLX, ST_CPLX, 3, 3, 3, 7),
     KB_FIXED(POL_DDS, 7, 320, LD_CPLX, ST_CPLX, 3, 3, 3, 7),
     KB_FIXED(POL_DDS, 7, 512, LD_CPLX, ST_CPLX, 3, 3, 3, 7),
     KB_FIXED(POL_DDS, 4, 256, LD_CPLX, ST_CPLX, 3, 3, 3, 7),
-    KB_FIXED(POL_DDS, 7, 512, LD_CPLX, ST_OUT, 5, 7, 7),            // 245: 96k->44.1k inverse rows
+    KB_FIXED(POL_DDS, 4, 128, LD_CPLX, ST_CPLX, 3, 3, 3, 7),
+    KB_FIXED(POL_DDS, 1, 64, LD_CPLX, ST_CPLX, 3, 3, 3, 7),
+    KB_FIXED(POL_DDS, 5, 256, LD_CPLX, ST_OUT, 5, 7, 7),            // 245: 96k->44.1k inverse rows
+    KB_FIXED(POL_DDS, 7, 512, LD_CPLX, ST_OUT, 5, 7, 7),
     KB_FIXED(POL_DDS, 9, 448, LD_CPLX, ST_OUT, 5, 7, 7),
     KB_FIXED(POL_DDS, 9, 512, LD_CPLX, ST_OUT, 5, 7, 7),
     KB_FIXED(POL_DDS, 4, 256, LD_CPLX, ST_OUT, 5, 7, 7),
     KB_FIXED(POL_DDS, 2, 128, LD_CPLX, ST_OUT, 5, 7, 7),
+    KB_FIXED(POL_DDS, 10, 512, LD_CPLX, ST_OUT, 5, 7, 7),
+    KB_FIXED(POL_DDS, 7, 384, LD_CPLX, ST_OUT, 5, 7, 7),
+    KB_FIXED(POL_DDS, 1, 64, LD_CPLX, ST_OUT, 5, 7, 7),
     KB_FIXED(POL_DDS, 5, 192, LD_REAL_F64, ST_CPLX, 3, 3, 3, 7),    // 189: 44.1k->48k forward columns
     KB_FIXED(POL_DDS, 9, 448, LD_CPLX, ST_CPLX, 5, 7, 7),           // 245: 44.1k->48k forward rows
     KB_FIXED(POL_DDS, 8, 448, LD_CPLX, ST_CPLX, 4, 4, 2, 7),        // 224: 44.1k->48k inverse columns
