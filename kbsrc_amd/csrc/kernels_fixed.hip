@@ -65,10 +65,13 @@ const FixedEntry kEntries[] = {
     KB_FIXED(POL_F64, 8, 256, LD_REAL_F64, ST_CPLX, 4, 2, 5, 5),    // 200: 96k->44.1k fast forward columns
     KB_FIXED(POL_F64, 6, 128, LD_REAL_F64, ST_CPLX, 4, 2, 5, 5),
     KB_FIXED(POL_F64, 16, 320, LD_REAL_F64, ST_CPLX, 4, 2, 5, 5),
-    KB_FIXED(POL_F64, 8, 256, LD_CPLX, ST_CPLX, 4, 2, 5, 5),        // 200: 96k->44.1k fast forward rows
+    KB_FIXED(POL_F64, 16, 640, LD_REAL_F64, ST_CPLX, 4, 2, 5, 5),
+    KB_FIXED(POL_F64, 16, 640, LD_CPLX, ST_CPLX, 4, 2, 5, 5),       // 200: 96k->44.1k fast forward rows
+    KB_FIXED(POL_F64, 8, 256, LD_CPLX, ST_CPLX, 4, 2, 5, 5),
     KB_FIXED(POL_F64, 6, 128, LD_CPLX, ST_CPLX, 4, 2, 5, 5),
     KB_FIXED(POL_F64, 16, 320, LD_CPLX, ST_CPLX, 4, 2, 5, 5),
-    KB_FIXED(POL_F64, 10, 128, LD_CPLX, ST_CPLX, 5, 5, 5),          // 125: 96k->44.1k fast inverse columns
+    KB_FIXED(POL_F64, 20, 256, LD_CPLX, ST_CPLX, 5, 5, 5),          // 125: 96k->44.1k fast inverse columns
+    KB_FIXED(POL_F64, 10, 128, LD_CPLX, ST_CPLX, 5, 5, 5),
     KB_FIXED(POL_F64, 5, 128, LD_CPLX, ST_CPLX, 5, 5, 5),
     KB_FIXED(POL_F64, 9, 256, LD_CPLX, ST_OUT, 3, 7, 7),            // 147: 96k->44.1k fast inverse rows
     KB_FIXED(POL_F64, 6, 128, LD_CPLX, ST_OUT, 3, 7, 7),
