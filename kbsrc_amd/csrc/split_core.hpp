@@ -64,41 +64,48 @@ __device__ __forceinline__ void split_core(const double (&hi)[8], const double (
             b[j] |= (long)(uint8_t)(int8_t)(int)d << (8 * i);
         }
     }
-    // products a_i b_j, i + j <= S + 1 (1-based), summed per weight t = i + j, one M-tile (real, then imaginary
-    // rows) at a time; acc[t] weighs 2^{1 + e - 7 (t + 2)} (M / 2 and v carry 2^{-1} and 2^{-e})
+    // products a_i b_j, i + j <= S + 1 (1-based), summed per weight t = i + j (0-based: a_i b_{t - i}, i <= t),
+    // one M-tile (real, then imaginary rows) at a time; the sum of weight t weighs 2^{1 + e - 7 (t + 2)} (M / 2 and
+    // v carry 2^{-1} and 2^{-e}). Weights run in groups of 4, each group's four sums folded into the FP64 result
+    // before the next group starts, so only four accumulators are live.
 #pragma nounroll
     for (int mt = 0; mt < 2; ++mt) {
-        v4i acc[S];
+        long a[S];
 #pragma unroll
-        for (int t = 0; t < S; ++t) acc[t] = v4i{0, 0, 0, 0};
+        for (int i = 0; i < S; ++i) a[i] = adig[(i * 2 + mt) * 64 + lane];
+        double s[4], err[4];
 #pragma unroll
-        for (int i = 0; i < S; ++i) {
-            const long a = adig[(i * 2 + mt) * 64 + lane];
+        for (int t0 = 0; t0 < S; t0 += 4) {
+            const int tn = t0 + 4 < S ? t0 + 4 : S;
+            v4i acc[4];
 #pragma unroll
-            for (int j = 0; i + j + 2 <= S + 1; ++j)
-                acc[i + j] = __builtin_amdgcn_mfma_i32_16x16x32_i8(a, b[j], acc[i + j], 0, 0, 0);
-        }
-        // exact groups of 4 weights (|G| < 2^43, every Horner step exact in FP64), then an exact TwoSum chain with
-        // the error terms summed: the dd result is within ~2^-105 of the exact sum
+            for (int t = t0; t < tn; ++t) {
+                acc[t - t0] = v4i{0, 0, 0, 0};
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-            double s = 0.0, err = 0.0;
+                for (int i = 0; i <= t; ++i)
+                    acc[t - t0] = __builtin_amdgcn_mfma_i32_16x16x32_i8(a[i], b[t - i], acc[t - t0], 0, 0, 0);
+            }
+            // the group exactly (|G| < 2^43, every Horner step exact in FP64), then one step of an exact TwoSum
+            // chain with the error terms summed: the dd result is within ~2^-105 of the exact sum
 #pragma unroll
-            for (int t0 = 0; t0 < S; t0 += 4) {
-                const int tn = t0 + 4 < S ? t0 + 4 : S;
-                double G = (double)acc[t0][r];
+            for (int r = 0; r < 4; ++r) {
+                double G = (double)acc[0][r];
 #pragma unroll
-                for (int t = t0 + 1; t < tn; ++t) G = __builtin_fma(G, 128.0, (double)acc[t][r]);
+                for (int t = t0 + 1; t < tn; ++t) G = __builtin_fma(G, 128.0, (double)acc[t - t0][r]);
                 const double g = __builtin_ldexp(G, 1 + e - 7 * (tn + 1));
                 if (t0 == 0) {
-                    s = g;
+                    s[r] = g;
+                    err[r] = 0.0;
                 } else {
-                    const dd z = knuth2(s, g);
-                    s = z.hi;
-                    err += z.lo;
+                    const dd z = knuth2(s[r], g);
+                    s[r] = z.hi;
+                    err[r] += z.lo;
                 }
             }
-            const dd v = quick2(s, err);
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const dd v = quick2(s[r], err[r]);
             if (mt == 0) re[r] = v;
             else im[r] = v;
         }
