@@ -282,6 +282,54 @@ template <class A> __device__ __forceinline__ Cx<A> renorm_dd(Cx<A> v) {
     else return v;
 }
 
+// Element e of sub-FFT j from a pass's source (real pairs of the padded input, or complex elements).
+template <class A, int LD>
+__device__ __forceinline__ Cx<A> pass_load(const PassArgs& a, const TaskIO& io, int task, int e, int j) {
+    const long long i = e * a.lde + (long long)j * a.ldg;
+    if constexpr (LD == LD_REAL_F64) {
+        const double2 pr = reinterpret_cast<const double2*>(a.src + io.in_off)[i];
+        return {A::lift(pr.x), A::lift(pr.y)};
+    } else {
+        return gld<A>(a.src + task * a.src_tstride, i);
+    }
+}
+
+// Element e of sub-FFT j's result to a pass's destination: four-step twiddle, renormalisation, then the complex
+// store (bins no reader needs are skipped) or the overlap discard (samples 2i and 2i + 1 of the block's output,
+// src/dsp/fft.hpp:460 and src/resampler.cpp:80-84).
+template <class A, int ST>
+__device__ __forceinline__ void pass_store(const PassArgs& a, const TaskIO& io, int task, int e, int j, Cx<A> v) {
+    const long long jg = j;
+    const long long i = e * a.ste + jg * a.stg;
+    if (ST == ST_CPLX && z_unused(a, i)) return;
+    if (a.twmode && e != 0 && (a.twmode == 1 || jg != 0)) v = c_mul<A>(v, gld<A>(a.tw4, i));
+    if constexpr (A::kDD) {
+        if (a.renorm) v = renorm_dd<A>(v);
+    }
+    if constexpr (ST == ST_CPLX) {
+        gst<A>(a.dst + task * a.dst_tstride, i, v);
+    } else if constexpr (ST == ST_OUT) {
+        double* o = a.out + io.out_off;
+        const long long x0 = 2 * i - a.margP;
+        if (x0 >= 0 && x0 < a.hopP) {
+            const long long d = io.blk * a.hopP + x0 - a.o;
+            if (d >= 0 && d < io.nout) {
+                if constexpr (A::kDD) o[d] = v.re.hi + v.re.lo;
+                else o[d] = v.re;
+            }
+        }
+        const long long x1 = x0 + 1;
+        if (x1 >= 0 && x1 < a.hopP) {
+            const long long d = io.blk * a.hopP + x1 - a.o;
+            if (d >= 0 && d < io.nout) {
+                const Val<A> ni = A::neg(v.im);
+                if constexpr (A::kDD) o[d] = ni.hi + ni.lo;
+                else o[d] = ni;
+            }
+        }
+    }
+}
+
 // __launch_bounds__'s second value is the minimum number of waves per SIMD the register allocation must allow:
 // 4 caps a thread at 128 VGPRs, so that LDS (not registers) limits occupancy.
 // IO: the first stage reads global memory and the last stage writes it (first_stage, last_stage), instead of a
@@ -305,49 +353,8 @@ __global__ __launch_bounds__(T, 4) void k_fixed(PassArgs a) {
         if constexpr (kPaired) return slot_index<G>(g, grp, a.f.M);
         else return j0 + g < a.nsub ? j0 + g : -1;
     };
-    // element e of sub-FFT j from the pass's source
-    auto load_at = [&](int e, int j) -> Cx<A> {
-        const long long i = e * a.lde + (long long)j * a.ldg;
-        if constexpr (LD == LD_REAL_F64) {
-            const double2 pr = reinterpret_cast<const double2*>(a.src + io.in_off)[i];
-            return {A::lift(pr.x), A::lift(pr.y)};
-        } else {
-            return gld<A>(a.src + task * a.src_tstride, i);
-        }
-    };
-    // element e of sub-FFT j's result to the pass's destination: four-step twiddle, renormalisation, then the
-    // complex store or the overlap discard (samples 2i and 2i + 1 of the block's output)
-    auto store_at = [&](int e, int j, Cx<A> v) {
-        const long long jg = j;
-        const long long i = e * a.ste + jg * a.stg;
-        if (ST == ST_CPLX && z_unused(a, i)) return;
-        if (a.twmode && e != 0 && (a.twmode == 1 || jg != 0)) v = c_mul<A>(v, gld<A>(a.tw4, i));
-        if constexpr (A::kDD) {
-            if (a.renorm) v = renorm_dd<A>(v);
-        }
-        if constexpr (ST == ST_CPLX) {
-            gst<A>(a.dst + task * a.dst_tstride, i, v);
-        } else if constexpr (ST == ST_OUT) {
-            double* o = a.out + io.out_off;
-            const long long x0 = 2 * i - a.margP;
-            if (x0 >= 0 && x0 < a.hopP) {
-                const long long d = io.blk * a.hopP + x0 - a.o;
-                if (d >= 0 && d < io.nout) {
-                    if constexpr (A::kDD) o[d] = v.re.hi + v.re.lo;
-                    else o[d] = v.re;
-                }
-            }
-            const long long x1 = x0 + 1;
-            if (x1 >= 0 && x1 < a.hopP) {
-                const long long d = io.blk * a.hopP + x1 - a.o;
-                if (d >= 0 && d < io.nout) {
-                    const Val<A> ni = A::neg(v.im);
-                    if constexpr (A::kDD) o[d] = ni.hi + ni.lo;
-                    else o[d] = ni;
-                }
-            }
-        }
-    };
+    auto load_at = [&](int e, int j) -> Cx<A> { return pass_load<A, LD>(a, io, task, e, j); };
+    auto store_at = [&](int e, int j, Cx<A> v) { pass_store<A, ST>(a, io, task, e, j, v); };
 
     if constexpr (LD == LD_FOLD) {
         // Y = X H with the bin map for every element, then the pre-split of each with its mirror bin

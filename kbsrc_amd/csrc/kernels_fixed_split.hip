@@ -1,9 +1,9 @@
 // Specialised sub-FFT passes of the split-precision form: the compile-time pass of kernels_fixed.hip with its first
 // two radix-4 stages replaced by the int8-MFMA radix-16 stage (split_core.hpp), for the sub-FFT lengths of the
 // benchmark conversions (96 kHz -> 44.1 kHz forward rows, 320 points; 44.1 kHz -> 48 kHz inverse columns, 224).
-// The load and store are the specialised pass's; the matrix stage reads one LDS tile and writes a second, the
-// remaining Stockham stages run in place there. The same operations as the generic split pass and the
-// global-memory route, so the same bits.
+// The matrix stage reads its inputs from the pass's source and writes one LDS tile, the remaining Stockham stages
+// run in place there and the last one writes the pass's destination. The same operations as the generic split
+// pass and the global-memory route, so the same bits.
 #include "fixed_impl.hpp"
 #include "split_core.hpp"
 
@@ -11,19 +11,21 @@ namespace kbsrc::gpu {
 
 namespace {
 
+// Stages 2 .. nst-2 in place (the matrix stage took stages 0 and 1; the last stage writes global memory).
 template <class RL, int G, int T, int... I>
-__device__ __forceinline__ void tail_stages(const Lds<DDS, RL::N * G>& L, int tid, const double* tw,
-                                            const double* oddc, const Prune& pr, std::integer_sequence<int, I...>) {
-    (stage<DDS, RL, G, T, I + 2>(L, tid, tw, oddc, pr), ...);
+__device__ __forceinline__ void mid_stages(const Lds<DDS, RL::N * G>& L, int tid, const double* tw,
+                                           const double* oddc, std::integer_sequence<int, I...>) {
+    const Prune none;
+    (stage<DDS, RL, G, T, I + 2>(L, tid, tw, oddc, none), ...);
 }
 
-// two waves per SIMD: the matrix stage's slicing needs up to ~200 VGPRs at 15 slices
+// two waves per SIMD: the matrix stage's slicing needs up to ~150 VGPRs at 15 slices
 template <class RL, int G, int T, int LD, int ST, int SPL>
 __global__ __launch_bounds__(T, 2) void k_fixed_split(PassArgs a) {
     using A = DDS;
     constexpr int N = RL::N, CAP = N * G, NV = G * (N / 16);
-    extern __shared__ double lds[];   // two tiles of kPlanes * CAP doubles
-    const Lds<A, CAP> L{lds}, L2{lds + kPlanes<A> * CAP};
+    extern __shared__ double lds[];   // kPlanes * CAP doubles
+    const Lds<A, CAP> L{lds};
     const int tid = threadIdx.x;
     const int wg = xcd_wg();
     const int task = wg / a.ngrp, grp = wg - task * a.ngrp;
@@ -31,36 +33,21 @@ __global__ __launch_bounds__(T, 2) void k_fixed_split(PassArgs a) {
     const TaskIO io = task_io(a.tdesc, a.task0 + task, a.nblk, a.in_cstride, a.in_bstride, a.nout);
     auto jof = [&](int g) -> int { return j0 + g < a.nsub ? j0 + g : -1; };
 
-    // load, lanes sub-FFT-fastest (as the specialised pass)
-    for (int w = tid; w < G * N; w += T) {
-        const int g = w % G, e = w / G;
-        const int j = jof(g);
-        if (j < 0) continue;
-        const long long i = e * a.lde + (long long)j * a.ldg;
-        Cx<A> v;
-        if constexpr (LD == LD_REAL_F64) {
-            const double2 pr = reinterpret_cast<const double2*>(a.src + io.in_off)[i];
-            v = {A::lift(pr.x), A::lift(pr.y)};
-        } else {
-            v = gld<A>(a.src + task * a.src_tstride, i);
-        }
-        L.put(e * G + g, v);
-    }
-    __syncthreads();
-
-    // the radix-16 stage: vector (g, p) = elements p + (N / 16) m of local sub-FFT g; y[16 p + k] = w_N^{p k} DFT16
+    // the radix-16 stage, its inputs straight from the pass's source: vector (g, p) = elements p + (N / 16) m of
+    // local sub-FFT g; y[16 p + k] = w_N^{p k} DFT16, written to the LDS tile
     {
         constexpr int nv = N / 16;
         const int lane = tid & 63, q = lane >> 4, col = lane & 15;
         for (int base = (tid >> 6) * 16; base < NV; base += (T >> 6) * 16) {   // uniform over the wave
             const int v = base + col;
             const int g = v < NV ? v % G : 0, p = v < NV ? v / G : 0;
-            const bool live = v < NV && jof(g) >= 0;
+            const int j = v < NV ? jof(g) : -1;
+            const bool live = j >= 0;
             double hi[8], lo[8];
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 if (live) {
-                    const Cx<A> c = L.get((p + nv * (4 * q + i)) * G + g);
+                    const Cx<A> c = pass_load<A, LD>(a, io, task, p + nv * (4 * q + i), j);
                     hi[2 * i] = c.re.hi;
                     hi[2 * i + 1] = c.im.hi;
                     lo[2 * i] = c.re.lo;
@@ -77,7 +64,7 @@ __global__ __launch_bounds__(T, 2) void k_fixed_split(PassArgs a) {
                     const int k = 4 * q + r;
                     cx<dd> z{re[r], im[r]};
                     if (p != 0 && k != 0) z = c_mul<DDA>(z, gld<DDA>(a.tw16, (long long)p * 15 + k - 1));
-                    L2.put((16 * p + k) * G + g, z);
+                    L.put((16 * p + k) * G + g, z);
                 }
             }
         }
@@ -87,46 +74,19 @@ __global__ __launch_bounds__(T, 2) void k_fixed_split(PassArgs a) {
     if constexpr (ST == ST_CPLX) {
         if (a.zkeep >= 0) pr = Prune{a.zkeep, a.zh, a.ste, a.stg, j0};
     }
-    tail_stages<RL, G, T>(L2, tid, a.tw, a.oddc, pr, std::make_integer_sequence<int, RL::nst - 2>{});
-
-    // store (as the specialised pass): four-step twiddle, renormalisation, complex buffer or output samples
-    for (int w = tid; w < G * N; w += T) {
-        const int g = w % G, e = w / G;
-        const int j = jof(g);
-        if (j < 0) continue;
-        const long long jg = j;
-        const long long i = e * a.ste + jg * a.stg;
-        if (ST == ST_CPLX && z_unused(a, i)) continue;
-        Cx<A> v = L2.get(e * G + g);
-        if (a.twmode && e != 0 && (a.twmode == 1 || jg != 0)) v = c_mul<A>(v, gld<A>(a.tw4, i));
-        if (a.renorm) v = renorm_dd<A>(v);
-        if constexpr (ST == ST_CPLX) {
-            gst<A>(a.dst + task * a.dst_tstride, i, v);
-        } else {
-            double* o = a.out + io.out_off;
-            const long long x0 = 2 * i - a.margP;
-            if (x0 >= 0 && x0 < a.hopP) {
-                const long long d = io.blk * a.hopP + x0 - a.o;
-                if (d >= 0 && d < io.nout) o[d] = v.re.hi + v.re.lo;
-            }
-            const long long x1 = x0 + 1;
-            if (x1 >= 0 && x1 < a.hopP) {
-                const long long d = io.blk * a.hopP + x1 - a.o;
-                if (d >= 0 && d < io.nout) {
-                    const dd ni = A::neg(v.im);
-                    o[d] = ni.hi + ni.lo;
-                }
-            }
-        }
-    }
+    mid_stages<RL, G, T>(L, tid, a.tw, a.oddc, std::make_integer_sequence<int, RL::nst - 3>{});
+    // the last stage, its outputs through the specialised pass's store (four-step twiddle, renormalisation, complex
+    // buffer or output samples)
+    last_stage<A, RL, G, T>(L, tid, a.tw, a.oddc, pr, jof,
+                            [&](int e, int jj, Cx<A> v) { pass_store<A, ST>(a, io, task, e, jj, v); });
 }
 
 template <class RL, int G, int T, int LD, int ST, int SPL> hipError_t launch_split_one(const PassArgs& a, hipStream_t s) {
     static_assert(RL::nst >= 3 && RL::r[0] == 4 && RL::r[1] == 4, "the split stage replaces two radix-4 stages");
     static_assert(RL::hold(G, T) <= 8, "too many values held per thread");
     static_assert(LD != LD_FOLD && ST != ST_SPLIT, "no fused split passes");
-    constexpr size_t bytes = (size_t)2 * kPlanes<DDS> * RL::N * G * sizeof(double);
-    static_assert(bytes <= 160 * 1024, "LDS tiles too large");
+    constexpr size_t bytes = (size_t)kPlanes<DDS> * RL::N * G * sizeof(double);
+    static_assert(bytes <= 160 * 1024, "LDS tile too large");
     static const hipError_t attr = bytes > 64 * 1024
         ? hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fixed_split<RL, G, T, LD, ST, SPL>),
                               hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes)
@@ -140,15 +100,17 @@ template <class RL, int G, int T, int LD, int ST, int SPL> hipError_t launch_spl
     SplitFixedEntry{Radices<__VA_ARGS__>::N, LD, ST, SPL, G, T, &launch_split_one<Radices<__VA_ARGS__>, G, T, LD, ST, SPL>}
 
 // first matching entry wins; the others stay for KBSRC_SPLIT_VARIANT sweeps. Measured on the 180 s 96 kHz ->
-// 44.1 kHz conversion (ms per conversion, plain 1.90-2.00): G = 2, T = 192: 2.01-2.02 (6 slices), 2.14-2.15 (15);
-// G = 4, T = 256: 2.01, 2.21-2.22; G = 4, T = 320: 1.99-2.01, 2.50-2.51 (one workgroup per CU at 15 slices).
+// 44.1 kHz conversion (ms per conversion, plain 1.767 on the same device): G = 4, T = 256: 1.752 (6 slices), 1.965
+// (15); G = 2, T = 192: 1.817, 1.997; G = 4, T = 320: 1.789, 2.313.
 const SplitFixedEntry kSplit[] = {
+    KB_SPLIT(4, 256, LD_CPLX, ST_CPLX, 6, 4, 4, 4, 5),
+    KB_SPLIT(4, 256, LD_CPLX, ST_CPLX, 7, 4, 4, 4, 5),
+    KB_SPLIT(4, 256, LD_CPLX, ST_CPLX, 8, 4, 4, 4, 5),
+    KB_SPLIT(4, 256, LD_CPLX, ST_CPLX, 15, 4, 4, 4, 5),
     KB_SPLIT(2, 192, LD_CPLX, ST_CPLX, 6, 4, 4, 4, 5),
     KB_SPLIT(2, 192, LD_CPLX, ST_CPLX, 7, 4, 4, 4, 5),
     KB_SPLIT(2, 192, LD_CPLX, ST_CPLX, 8, 4, 4, 4, 5),
     KB_SPLIT(2, 192, LD_CPLX, ST_CPLX, 15, 4, 4, 4, 5),
-    KB_SPLIT(4, 256, LD_CPLX, ST_CPLX, 6, 4, 4, 4, 5),
-    KB_SPLIT(4, 256, LD_CPLX, ST_CPLX, 15, 4, 4, 4, 5),
     KB_SPLIT(4, 320, LD_CPLX, ST_CPLX, 6, 4, 4, 4, 5),
     KB_SPLIT(4, 320, LD_CPLX, ST_CPLX, 15, 4, 4, 4, 5),
     KB_SPLIT(4, 256, LD_CPLX, ST_CPLX, 6, 4, 4, 2, 7),
