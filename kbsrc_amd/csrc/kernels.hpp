@@ -204,6 +204,7 @@ struct SplitFixedEntry {
     hipError_t (*launch)(const PassArgs&, hipStream_t);
 };
 const SplitFixedEntry* find_split_fixed(int n, int ld, int st, int slices);
+const SplitFixedEntry* split_table_extreme(int* count);   // kernels_fixed_split_extreme.hip
 
 // variant: index among the registered shapes of (policy, n, ld, st); -1: the default (or KBSRC_TUNE's)
 bool launch_fixed(int policy, int ld, int st, PassArgs& a, hipStream_t s, hipError_t* err, int variant = -1);
