@@ -131,3 +131,36 @@ def test_split_routes_give_the_same_bits(fi, fo, block, monkeypatch):
             outs.append(d_out.cpu().numpy().copy())
         assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2]), s
     plan.close()
+
+
+def test_split_routes_agree_10mhz(monkeypatch):
+    """The 10 MHz -> 48 kHz plan (--len 0.5: a 20,160,000-point forward transform, 3150 x 3200): its 3200-point
+    forward rows run the matrix stage in the specialised split pass (one 768-thread workgroup per sub-FFT) and
+    through the global-memory route (KBSRC_SPLIT_GLOBAL=1); both give the same bits, at 8 and 15 slices."""
+    import torch
+    k = gpu()
+    x = np.random.default_rng(10).uniform(-0.5, 0.5, 1_000_000)
+    plan = k.Plan(len(x), 1, 10_000_000, 48000, "exact", length=0.5)
+    d_pad = torch.zeros(plan.info["xstride"], dtype=torch.float64, device="cuda")
+    d_out = torch.empty(plan.info["nout"], dtype=torch.float64, device="cuda")
+    plan.stage(x, d_pad.data_ptr())
+    plan.execute(d_pad.data_ptr(), d_out.data_ptr())
+    torch.cuda.synchronize()
+    plain = d_out.cpu().numpy().copy()
+    peak = np.abs(plain).max()
+    for s in (8, 15):
+        outs = []
+        for route in (None, "1"):
+            say(f"10 MHz -> 48 kHz: split S={s}, route {route or 'specialised'}")
+            if route:
+                monkeypatch.setenv("KBSRC_SPLIT_GLOBAL", route)
+            else:
+                monkeypatch.delenv("KBSRC_SPLIT_GLOBAL", raising=False)
+            d_out.zero_()
+            plan.execute_split(d_pad.data_ptr(), d_out.data_ptr(), s)
+            torch.cuda.synchronize()
+            outs.append(d_out.cpu().numpy().copy())
+        assert np.array_equal(outs[0], outs[1]), s
+        # and the split output stays within its slicing error of the plain one
+        assert np.abs(outs[0] - plain).max() <= (2.0 ** -40 if s == 8 else 2.0 ** -50) * peak, s
+    plan.close()
