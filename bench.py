@@ -193,6 +193,18 @@ def committed_traffic(mode, pass_name):
     return None
 
 
+def committed_valu_busy(mode, pass_name):
+    """Fraction of cycles a pass keeps the VALU issuing, from the committed rocprofv3 counter summary, or None."""
+    path = os.path.join(ROOT, "profiles", f"valu_counters_96k_to_44k1_180s_{mode}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 9 and f[0] == pass_name:
+            return float(f[6])
+    return None
+
+
 def cpu_baseline(x, threads, workdir):
     """The reference's engine (oracle/_ref/kbsrc, compiled from its sources) on the same input, on this host:
     its `-v` line reports the resample time without design and file I/O (app/main.cpp:144-145)."""
@@ -753,6 +765,9 @@ def main():
     roof["algorithmic_per_launch"] = (f"{fl[dom] * tasks / 1e9:.3f} GFLOP" if dd
                                       else f"{passes[dom]['algorithmic_gb']:.4f} GB")
     roof["traffic"] = committed_traffic(args.mode, dom)
+    # the same pass's VALU issue rate (SQ_ACTIVE_INST_VALU against one instruction per 4 cycles per SIMD): dd adds
+    # count one flop each against a peak that counts an FMA as two, so frac stays well under this
+    roof["valu_busy"] = committed_valu_busy(args.mode, dom)
 
     # end to end (excluding file I/O): output copied back to the host
     torch.cuda.synchronize()
