@@ -509,29 +509,34 @@ template <int N> using RadicesOf = typename RadOf<N, std::make_integer_sequence<
 struct Shape {
     int G, T;
 };
+// 256 threads (4 waves) and the G whose stages waste the fewest wave slots (a wave with no butterfly issues
+// nothing; a partly filled one costs a full one), within 80 KB of LDS and the held-value bound; ties go to the
+// larger G. Measured with the first and last stages on global memory, 4-wave workgroups beat 8-wave ones of the
+// same wave efficiency (320 points: 4 x 256 0.374 ms against 8 x 512 0.403; 189: 8 x 256 0.236 against 8 x 512
+// 0.252; 245: 5 x 256 0.215 against 9 x 512 0.229); the rule picks 4 x 256 and 5 x 256 for the first and last,
+// 7 x 256 for 189.
 constexpr Shape auto_shape(int n, bool dd) {
     const int planes = dd ? 4 : 2, hmax = dd ? 8 : 16, nst = radix_count(n);
-    Shape best{1, 128};
-    long long best_score = -1, best_elems = -1;
-    for (int T = 128; T <= 512; T += 64) {
-        for (int G = 1; G <= 16; ++G) {
-            const long long lds = (long long)planes * 8 * n * G;
-            if (lds > 80 * 1024) break;
-            int h = 0;
-            long long eff = 0;   // lane use in 1e-6 units, summed over stages
-            for (int i = 0; i < nst; ++i) {
-                const int r = radix_at(n, i), W = G * (n / r), it = (W + T - 1) / T;
-                h = it * r > h ? it * r : h;
-                eff += 1000000LL * W / ((long long)it * T);
-            }
-            if (h > hmax) continue;
-            const long long wgs = 160 * 1024 / lds, waves = wgs * T / 64 < 16 ? wgs * T / 64 : 16;
-            const long long score = eff / nst * waves / 16 / 100;   // coarse, so near-equal shapes tie
-            if (score > best_score || (score == best_score && (long long)G * n > best_elems)) {
-                best_score = score;
-                best_elems = (long long)G * n;
-                best = {G, T};
-            }
+    constexpr int T = 256;
+    Shape best{1, T};
+    long long best_num = -1, best_den = 1;
+    for (int G = 1; G <= 32; ++G) {
+        const long long lds = (long long)planes * 8 * n * G;
+        if (lds > 80 * 1024) break;
+        int h = 0;
+        long long work = 0, slots = 0;
+        for (int i = 0; i < nst; ++i) {
+            const int r = radix_at(n, i), W = G * (n / r), it = (W + T - 1) / T, rem = W - (it - 1) * T;
+            h = it * r > h ? it * r : h;
+            work += W;
+            slots += (long long)(it - 1) * (T / 64) + (rem + 63) / 64;
+        }
+        if (h > hmax) continue;
+        // efficiency work / (64 slots), compared exactly; ties (and better) take the larger G
+        if (best_num < 0 || work * 64 * best_den >= best_num * 64 * slots) {
+            best_num = work;
+            best_den = slots;
+            best = {G, T};
         }
     }
     return best;

@@ -58,6 +58,9 @@ const FixedEntry kEntries[] = {
     KB_FIXED(POL_DDS, 8, 448, LD_FOLD, ST_CPLX, 4, 4, 2, 7),        // 224: 44.1k->48k
     KB_FIXED(POL_DDS, 8, 320, LD_CPLX, ST_SPLIT, 4, 4, 2, 5),       // 160: 48k->44.1k
     KB_FIXED(POL_DDS, 4, 128, LD_FOLD, ST_CPLX, 5, 5, 5),           // 125: 48k->44.1k
+    // 336: 8-channel 384 kHz -> 44.1 kHz inverse columns and rows (ahead of the generated 4 x 256 shapes)
+    KB_FIXED(POL_DDS, 6, 512, LD_CPLX, ST_CPLX, 4, 4, 3, 7),
+    KB_FIXED(POL_DDS, 6, 512, LD_CPLX, ST_OUT, 4, 4, 3, 7),
     // 16 (radix 4 x 4): the plain dd baseline of the split-precision radix-16 comparison
     KB_FIXED(POL_DDS, 64, 256, LD_CPLX, ST_CPLX, 4, 4),
     KB_FIXED(POL_DDS, 32, 128, LD_CPLX, ST_CPLX, 4, 4),

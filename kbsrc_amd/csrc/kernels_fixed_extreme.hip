@@ -9,7 +9,10 @@ namespace kbsrc::gpu {
 namespace {
 
 const FixedEntry kExtreme[] = {
+    // 8-channel 384 kHz -> 44.1 kHz exact: 2 x 512 measured ahead of the generated 1 x 256 and 2 x 256 (second)
+    KB_FIXED(POL_DDS, 2, 512, LD_REAL_F64, ST_CPLX, 4, 4, 4, 3, 5),
     KB_AUTO(POL_DDS, 960, LD_REAL_F64, ST_CPLX),
+    KB_FIXED(POL_DDS, 2, 512, LD_CPLX, ST_CPLX, 4, 4, 4, 4, 4),
     KB_AUTO(POL_DDS, 1024, LD_CPLX, ST_CPLX),
     // 3150 and 3200 points, 16 waves per workgroup (10 waves, T = 640, the second entries: 28.6 against 27.5 ms
     // for the 30 s 10 MHz conversion)
