@@ -43,10 +43,13 @@ const FixedEntry kEntries[] = {
     KB_FIXED(POL_DDS, 10, 512, LD_CPLX, ST_OUT, 5, 7, 7),
     KB_FIXED(POL_DDS, 7, 384, LD_CPLX, ST_OUT, 5, 7, 7),
     KB_FIXED(POL_DDS, 1, 64, LD_CPLX, ST_OUT, 5, 7, 7),
-    KB_FIXED(POL_DDS, 5, 192, LD_REAL_F64, ST_CPLX, 3, 3, 3, 7),    // 189: 44.1k->48k forward columns
-    KB_FIXED(POL_DDS, 9, 448, LD_CPLX, ST_CPLX, 5, 7, 7),           // 245: 44.1k->48k forward rows
+    KB_FIXED(POL_DDS, 8, 256, LD_REAL_F64, ST_CPLX, 3, 3, 3, 7),    // 189: 44.1k->48k forward columns
+    KB_FIXED(POL_DDS, 5, 192, LD_REAL_F64, ST_CPLX, 3, 3, 3, 7),
+    KB_FIXED(POL_DDS, 5, 256, LD_CPLX, ST_CPLX, 5, 7, 7),           // 245: 44.1k->48k forward rows
+    KB_FIXED(POL_DDS, 9, 448, LD_CPLX, ST_CPLX, 5, 7, 7),
     KB_FIXED(POL_DDS, 8, 448, LD_CPLX, ST_CPLX, 4, 4, 2, 7),        // 224: 44.1k->48k inverse columns
-    KB_FIXED(POL_DDS, 7, 320, LD_CPLX, ST_OUT, 3, 3, 5, 5),         // 225: 44.1k->48k inverse rows
+    KB_FIXED(POL_DDS, 5, 256, LD_CPLX, ST_OUT, 3, 3, 5, 5),         // 225: 44.1k->48k inverse rows
+    KB_FIXED(POL_DDS, 7, 320, LD_CPLX, ST_OUT, 3, 3, 5, 5),
     KB_FIXED(POL_DDS, 5, 128, LD_REAL_F64, ST_CPLX, 5, 5, 5),       // 125: 48k->44.1k forward columns
     KB_FIXED(POL_DDS, 8, 320, LD_CPLX, ST_CPLX, 4, 4, 2, 5),        // 160: 48k->44.1k forward rows
     KB_FIXED(POL_DDS, 5, 128, LD_CPLX, ST_CPLX, 5, 5, 5),           // 125: 48k->44.1k inverse columns
