@@ -218,6 +218,21 @@ def test_unstored_bins_change_nothing(rin, rout, mode, length, monkeypatch):
     assert y.tobytes() == z.tobytes()
 
 
+@pytest.mark.parametrize("rin,rout,mode", [(96000, 44100, "exact"), (96000, 44100, "fast"), (48000, 44100, "exact"),
+                                           (192000, 44100, "exact")])
+def test_split_fold_block_order_changes_nothing(rin, rout, mode, monkeypatch):
+    """Down conversions run the split/fold kernel's bin blocks abs(h - Mb) apart back to back (the threads there
+    share their z reads); the launch order changes no value: same bits as the plain block order
+    (KBSRC_MID_ORDER=0)."""
+    x = np.random.default_rng(rin + 7 * rout).uniform(-0.5, 0.5, (3 * rin, 2))
+    say(f"split/fold block order {rin} -> {rout} {mode}")
+    monkeypatch.delenv("KBSRC_MID_ORDER", raising=False)
+    y = gpu().resample(x, rin, rout, mode)
+    monkeypatch.setenv("KBSRC_MID_ORDER", "0")
+    z = gpu().resample(x, rin, rout, mode)
+    assert y.tobytes() == z.tobytes()
+
+
 GLOBAL_CASES = [c for c in CASES if c[3] >= 1000 and c[2] <= 2][::2]
 
 
