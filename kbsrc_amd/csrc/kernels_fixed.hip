@@ -13,7 +13,8 @@ namespace {
 // and KBSRC_TUNE="n:k,..." selects the k-th for length n (the sweep that picks the defaults).
 const FixedEntry kEntries[] = {
     // exact (four-step sub-FFTs with the one-TwoSum add)
-    KB_FIXED(POL_DDS, 8, 512, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),    // 315: 96k->44.1k forward columns
+    KB_FIXED(POL_DDS, 4, 256, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),    // 315: 96k->44.1k forward columns
+    KB_FIXED(POL_DDS, 8, 512, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),
     KB_FIXED(POL_DDS, 2, 128, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),
     KB_FIXED(POL_DDS, 4, 448, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),
     KB_FIXED(POL_DDS, 4, 512, LD_REAL_F64, ST_CPLX, 3, 3, 5, 7),
