@@ -11,9 +11,11 @@
 #include <functional>
 #include <tuple>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "engine.hpp"
@@ -232,11 +234,94 @@ void stage_dev(const Plan& p, bool lpc, const double* d_in, const double* first,
     hip_check(hipStreamSynchronize(s), "staging sync");
 }
 
+// Host <-> device copies of caller buffers through two pinned chunks of 64 MB (made on first use, kept for the
+// process, one caller at a time): each chunk is copied on the host by up to 8 threads (page faults on a fresh
+// caller buffer are taken in parallel) while the DMA of the other chunk runs. Synchronous: returns when done.
+struct PinnedChunks {
+    static constexpr size_t kBytes = 64u << 20;
+    std::mutex m;
+    void* buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    void ensure() {
+        for (int i = 0; i < 2; ++i) {
+            if (!buf[i]) hip_check(hipHostMalloc(&buf[i], kBytes, hipHostMallocDefault), "pinned staging buffer");
+            if (!ev[i]) hip_check(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming), "staging event");
+        }
+    }
+};
+PinnedChunks& pinned_chunks() {
+    static PinnedChunks* c = new PinnedChunks;   // kept until the process ends
+    return *c;
+}
+
+void parallel_copy(void* dst, const void* src, size_t bytes) {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const unsigned nt = std::min(8u, std::max(1u, hw));
+    if (nt == 1 || bytes < (8u << 20)) {
+        std::memcpy(dst, src, bytes);
+        return;
+    }
+    const size_t per = ((bytes + nt - 1) / nt + 4095) & ~(size_t)4095;
+    auto part = [=](size_t o) {
+        std::memcpy(static_cast<char*>(dst) + o, static_cast<const char*>(src) + o, std::min(per, bytes - o));
+    };
+    std::vector<std::thread> th;
+    for (unsigned t = 1; t < nt && t * per < bytes; ++t) {
+        try {
+            th.emplace_back(part, t * per);
+        } catch (...) {   // no thread: this part here
+            part(t * per);
+        }
+    }
+    part(0);
+    for (auto& x : th) x.join();
+}
+
+void upload_host(void* d_dst, const void* h_src, size_t bytes, hipStream_t s) {
+    PinnedChunks& c = pinned_chunks();
+    std::lock_guard<std::mutex> lock(c.m);
+    c.ensure();
+    const size_t K = PinnedChunks::kBytes;
+    for (size_t off = 0, k = 0; off < bytes; off += K, ++k) {
+        const int b = (int)(k & 1);
+        const size_t n = std::min(K, bytes - off);
+        if (k >= 2) hip_check(hipEventSynchronize(c.ev[b]), "staging wait");   // the DMA that last read buf[b]
+        parallel_copy(c.buf[b], static_cast<const char*>(h_src) + off, n);
+        hip_check(hipMemcpyAsync(static_cast<char*>(d_dst) + off, c.buf[b], n, hipMemcpyHostToDevice, s), "upload");
+        hip_check(hipEventRecord(c.ev[b], s), "staging event");
+    }
+    hip_check(hipStreamSynchronize(s), "upload sync");
+}
+
+void download_host(void* h_dst, const void* d_src, size_t bytes, hipStream_t s) {
+    PinnedChunks& c = pinned_chunks();
+    std::lock_guard<std::mutex> lock(c.m);
+    c.ensure();
+    const size_t K = PinnedChunks::kBytes, chunks = (bytes + K - 1) / K;
+    auto issue = [&](size_t k) {
+        const size_t off = k * K, n = std::min(K, bytes - off);
+        hip_check(hipMemcpyAsync(c.buf[k & 1], static_cast<const char*>(d_src) + off, n, hipMemcpyDeviceToHost, s),
+                  "download");
+        hip_check(hipEventRecord(c.ev[k & 1], s), "staging event");
+    };
+    if (chunks > 0) issue(0);
+    for (size_t k = 0; k < chunks; ++k) {
+        if (k + 1 < chunks) issue(k + 1);   // into the other chunk, whose copy-out finished last iteration
+        hip_check(hipEventSynchronize(c.ev[k & 1]), "staging wait");
+        const size_t off = k * K, n = std::min(K, bytes - off);
+        parallel_copy(static_cast<char*>(h_dst) + off, c.buf[k & 1], n);
+    }
+}
+
 // Staging from host interleaved input: one upload, then as above.
 void stage(const Plan& p, bool lpc, const double* in, double* d_padded, hipStream_t s) {
     DevBuf din;
-    if (p.nin > 0) din.upload(in, sizeof(double) * (size_t)(p.nin * p.ch), s);
-    else din.alloc(sizeof(double));
+    if (p.nin > 0) {
+        din.alloc(sizeof(double) * (size_t)(p.nin * p.ch));
+        upload_host(din.d(), in, sizeof(double) * (size_t)(p.nin * p.ch), s);
+    } else {
+        din.alloc(sizeof(double));
+    }
     const long long m = edge_frames(p);
     stage_dev(p, lpc, din.d(), in, in + (p.nin - m) * p.ch, d_padded, s);
 }
@@ -306,8 +391,7 @@ void resample_host(const double* in, long long frames, int channels, int rate_in
     pl->design_seconds += now() - t1;
     const double t2 = now();
     pl->eng->execute(xx.d(), dout.d(), s);
-    if (p.ch * p.nout > 0)
-        hip_check(hipMemcpyAsync(out, dout.d(), sizeof(double) * p.ch * p.nout, hipMemcpyDeviceToHost, s), "download");
+    if (p.ch * p.nout > 0) download_host(out, dout.d(), sizeof(double) * p.ch * p.nout, s);
     hip_check(hipStreamSynchronize(s), "engine sync");
     fill_stats(*pl, now() - t2, st);
 }
