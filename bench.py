@@ -24,6 +24,8 @@ sys.path.insert(0, ROOT)
 RATE_IN, RATE_OUT, SECONDS, CH = 96000, 44100, 180, 2
 INT8_PEAK_OPS = 5.0e15   # MI355X dense int8 matrix peak
 FP64_PEAK_TFLOPS = 78.6   # MI355X published FP64 peak
+SPLIT_SLICES_2M100 = 15   # fewest base-128 slices whose radix-16 stage error is within 2^-100 (test_gpu_split)
+WARM_MIN = 30   # fewest untimed executes before a timed region
 HBM_PEAK_GBS = 8000.0
 
 
@@ -119,17 +121,21 @@ def _short_kernel(name):
 
 def _pass_of_dispatches(names):
     """Engine dispatch order is fwd_col, fwd_row, mid, inv_col, inv_row per execute (unfused, no Bluestein): label
-    each dispatch by its offset from the nearest k_mid."""
+    each dispatch by its offset from the nearest k_mid, where all five are the plain form's kernels (executes of the
+    split form, whose forward rows run another kernel, are left out)."""
     lab = [None] * len(names)
+
+    def plain_pass(n):
+        return "k_fixed<" in n and "k_fixed_split" not in n
+
     for i, n in enumerate(names):
-        if "k_mid<" in n:
+        if "k_mid<" in n and i >= 2 and i + 2 < len(names) and all(plain_pass(names[i + o]) for o in (-2, -1, 1, 2)):
             for off, p in zip((-2, -1, 0, 1, 2), PASSES):
-                if 0 <= i + off < len(names):
-                    lab[i + off] = p
+                lab[i + off] = p
     return lab
 
 
-def summarize_rocprof(d, mode):
+def summarize_rocprof(d, mode, steps=20, warmup=5):
     """Reads <d>/stats (--kernel-trace --stats), <d>/fetch (--pmc FETCH_SIZE) and <d>/write (--pmc WRITE_SIZE),
     all csv, and prints the per-pass summary kept under profiles/ (HBM bytes scaled by what each counter reads for
     the 256 MiB copy of --profile-copy)."""
@@ -141,6 +147,9 @@ def summarize_rocprof(d, mode):
         if p:
             dur[p].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
             kname[p] = _short_kernel(r["Kernel_Name"])
+    # the timed region of the profiled command: plain executes max(W, WARM_MIN) .. + K - 1, in dispatch order
+    t0 = max(warmup, WARM_MIN)
+    timed = {p: v[t0:t0 + steps] for p, v in dur.items()}
     copy_bytes = (1 << 25) * 8
     cnt, factor = {}, {}
     for c, sub in (("FETCH_SIZE", "fetch"), ("WRITE_SIZE", "write")):
@@ -154,24 +163,28 @@ def summarize_rocprof(d, mode):
     out.append(f"# Engine passes, stereo 96000 -> 44100 Hz, 180 s, {mode} mode: MI355X, rocprofv3 (csv output)")
     out.append("# commands, from /tmp with TMPDIR=/tmp, R = the repository root, D = an output folder:")
     out.append(f"#   rocprofv3 --kernel-trace --stats -d D/stats -o run --output-format csv -- python3 R/bench.py "
-               f"--mode {mode} --steps 5 --warmup 1 --no-cpu")
+               f"--mode {mode} --steps {steps} --warmup {warmup} --no-cpu")
     for c, sub in (("FETCH_SIZE", "fetch"), ("WRITE_SIZE", "write")):
         out.append(f"#   rocprofv3 --pmc {c} -d D/{sub} -o run --output-format csv -- python3 R/bench.py --mode {mode} "
-                   "--steps 2 --warmup 1 --no-cpu --profile-copy")
-    out.append(f"#   python3 R/bench.py --summarize-rocprof D --mode {mode}")
+                   "--steps 2 --warmup 1 --no-cpu --no-split --cpu-runs 1 --profile-copy")
+    out.append(f"#   python3 R/bench.py --summarize-rocprof D --mode {mode} --steps {steps} --warmup {warmup}")
     out.append(f"# calibration, 256 MiB device copy: FETCH_SIZE x {factor['FETCH_SIZE']:.3f}, "
                f"WRITE_SIZE x {factor['WRITE_SIZE']:.3f} (counter KiB -> bytes moved)")
+    out.append(f"# avg_us: the mean over the {steps} executes of the timed region (executes {t0}..{t0 + steps - 1} of "
+               f"the run, after {t0} untimed ones); all_avg_us: over every execute of the run (the warm-up and the "
+               f"executes after host-side pauses included)")
     out.append(f"# {'pass':8s} {'launches':>8s} {'avg_us':>9s} {'min_us':>9s} {'max_us':>9s} {'fetch_MB':>9s} "
-               f"{'write_MB':>9s} {'GB/s':>8s}  kernel")
+               f"{'write_MB':>9s} {'GB/s':>8s} {'all_avg_us':>10s}  kernel")
     for p in PASSES:
-        t = dur[p]
+        t, ta = timed[p], dur[p]
         if not t:
             continue
         fb = sum(cnt["FETCH_SIZE"][p]) / len(cnt["FETCH_SIZE"][p]) * 1024 * factor["FETCH_SIZE"]
         wb = sum(cnt["WRITE_SIZE"][p]) / len(cnt["WRITE_SIZE"][p]) * 1024 * factor["WRITE_SIZE"]
         avg = sum(t) / len(t)
         out.append(f"  {p:8s} {len(t):8d} {avg:9.1f} {min(t):9.1f} {max(t):9.1f} {fb / 1e6:9.1f} {wb / 1e6:9.1f} "
-                   f"{(fb + wb) / avg / 1e3:8.0f}  {kname[p]}")
+                   f"{(fb + wb) / avg / 1e3:8.0f} {sum(ta) / len(ta):10.1f}  {kname[p]}")
+    out.append(f"timed_sum_us {sum(sum(timed[p]) / len(timed[p]) for p in PASSES if timed[p]):.1f}")
     for p in PASSES:
         if cnt["FETCH_SIZE"][p]:
             fb = sum(cnt["FETCH_SIZE"][p]) / len(cnt["FETCH_SIZE"][p]) * 1024 * factor["FETCH_SIZE"]
@@ -193,6 +206,18 @@ def committed_traffic(mode, pass_name):
     return None
 
 
+def committed_avg_us(mode, pass_name):
+    """Average duration (us) of a pass's kernel over its launches in the committed rocprofv3 summary, or None."""
+    path = os.path.join(ROOT, "profiles", f"passes_96k_to_44k1_180s_{mode}.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) >= 9 and f[0] == pass_name and f[1].isdigit():
+            return float(f[2])
+    return None
+
+
 def committed_valu_busy(mode, pass_name):
     """Fraction of cycles a pass keeps the VALU issuing, from the committed rocprofv3 counter summary, or None."""
     path = os.path.join(ROOT, "profiles", f"valu_counters_96k_to_44k1_180s_{mode}.txt")
@@ -205,28 +230,40 @@ def committed_valu_busy(mode, pass_name):
     return None
 
 
-def cpu_baseline(x, threads, workdir):
-    """The reference's engine (oracle/_ref/kbsrc, compiled from its sources) on the same input, on this host:
-    its `-v` line reports the resample time without design and file I/O (app/main.cpp:144-145)."""
+def host_cpu():
+    """The host's CPU model and the logical CPUs this process may run on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    return {"model": model, "logical_cpus": os.cpu_count(), "usable_cpus": usable}
+
+
+def ref_engine_once(src, dst, threads, rate_out=RATE_OUT, extra=()):
+    """One run of the reference's CLI (oracle/_ref/kbsrc, compiled from its sources): (resample s, design s) from
+    its `-v` line, which times the engine without design and file I/O (app/main.cpp:144-145); None on failure."""
     ref = os.path.join(ROOT, "oracle", "_ref", "kbsrc")
-    if not os.path.exists(ref):
+    r = subprocess.run([ref, src, dst, "--rate", str(rate_out), "-t", str(threads), "-v", *extra],
+                       capture_output=True, text=True, timeout=300)
+    m = re.search(r"design ([0-9.]+)s\s+resample ([0-9.]+)s", r.stderr)
+    if r.returncode or not m:
         return None
-    from tests.helpers import wav_f64
-    src, dst = os.path.join(workdir, "in.wav"), os.path.join(workdir, "out.wav")
-    with open(src, "wb") as f:
-        f.write(wav_f64(x, CH, RATE_IN))
-    times = []
-    for _ in range(3):
-        r = subprocess.run([ref, src, dst, "--rate", str(RATE_OUT), "-t", str(threads), "-v"], capture_output=True,
-                           text=True, timeout=300)
-        m = re.search(r"design ([0-9.]+)s\s+resample ([0-9.]+)s", r.stderr)
-        if r.returncode or not m:
-            return None
-        times.append((float(m.group(2)), float(m.group(1))))
-        print(f"cpu reference run: resample {times[-1][0]:.3f} s (threads {threads})", file=sys.stderr, flush=True)
-    times.sort()
-    eng, design = times[len(times) // 2]
-    return eng, design
+    return float(m.group(2)), float(m.group(1))
+
+
+def median(v):
+    v = sorted(v)
+    n = len(v)
+    return v[n // 2] if n % 2 else 0.5 * (v[n // 2 - 1] + v[n // 2])
 
 
 # ---- batched form: many files, channel counts and rate pairs in one batch ------------------------------------------
@@ -275,7 +312,7 @@ def graph_timing(args, owner, d_src, d_out):
     import torch
     alt = torch.empty_like(d_out)
     g = owner.capture(d_src.data_ptr(), alt.data_ptr())
-    for _ in range(max(1, args.warmup)):
+    for _ in range(max(args.warmup, WARM_MIN)):
         g.launch()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -656,8 +693,11 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--mode", default="exact", choices=["exact", "fast"])
-    ap.add_argument("--cpu-threads", type=int, default=0, help="reference CPU engine threads (0: min(16, cores))")
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="reference CPU engine threads beside its default (0: min(16, usable cores))")
+    ap.add_argument("--cpu-runs", type=int, default=5, help="alternating CPU/GPU runs of the baseline protocol")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-split", action="store_true", help="skip timing the split-precision form")
     ap.add_argument("--profile-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (calibrates HBM byte counters under rocprofv3 --pmc)")
     ap.add_argument("--tune", action="store_true",
@@ -670,7 +710,7 @@ def main():
                     help="print the per-pass summary of rocprofv3 csv output under DIR/{stats,fetch,write} and exit")
     args = ap.parse_args()
     if args.summarize_rocprof:
-        return summarize_rocprof(args.summarize_rocprof, args.mode)
+        return summarize_rocprof(args.summarize_rocprof, args.mode, args.steps, args.warmup)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}): the path would shard by block ranges with "
               "no exchange step, and nothing here drives a second GPU; run with --gpus 1", file=sys.stderr)
@@ -718,7 +758,10 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         del a, b
-    for _ in range(args.warmup):
+    # untimed warm-up: W executes, and at least WARM_MIN in all, so that the GPU's clocks have left their idle state
+    # (after a pause of the GPU the first ~5 executes of this workload run 10-25% slower; rocprofv3 trace)
+    warm = max(args.warmup, WARM_MIN)
+    for _ in range(warm):
         plan.execute(xx.data_ptr(), out.data_ptr(), stream)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -728,15 +771,25 @@ def main():
     ms = (time.perf_counter() - t0) * 1e3 / args.steps
     nout_total = CH * info["nout"]
     value = nout_total / (ms / 1e3) / 1e6
-    print(f"engine {ms:.3f} ms/step -> {value:.1f} output Msamples/s", file=sys.stderr, flush=True)
+    print(f"engine {ms:.3f} ms/step -> {value:.1f} output Msamples/s ({warm} warm-up executes)", file=sys.stderr,
+          flush=True)
 
-    # per-kernel HIP-event times on the launch stream, averaged over a few timed executes
-    reps = 3
+    # per-kernel HIP-event times on the launch stream right after the timed executes (the five launches queued back
+    # to back, timing-only events read after), averaged over a few executes
+    reps = 5
     acc = None
     for _ in range(reps):
         t = plan.execute_timed(xx.data_ptr(), out.data_ptr(), stream)
         acc = t if acc is None else {k: acc[k] + t[k] for k in acc}
     kt = {k: v / reps for k, v in acc.items()}
+
+    # what was timed is checked: the output of the timed executes (the timed-pass executes rewrite the same bits)
+    # against the reference's pinned SHA-256 of the f64 WAV it writes for this input
+    from tests.helpers import PINS_96K_180S, wav_sha_planar
+    host = out.cpu().numpy().reshape(CH, info["nout"])
+    sha_ok = wav_sha_planar(host, RATE_OUT) == PINS_96K_180S[f"{args.mode}_f64"]
+    del host
+    print(f"timed output sha256 equals the reference pin: {sha_ok}", file=sys.stderr, flush=True)
     tasks = info["tasks"]
     fl = block_flops(info["Nb"], info["Mb"], args.mode == "exact", bool(info["up"]))
     passes = {}
@@ -750,18 +803,22 @@ def main():
         nbytes = by[k] * tasks + (k == "mid") * (info["Nb"] // 2 + 1) * (16 if dd else 8)
         passes[k]["algorithmic_gb"] = round(nbytes / 1e9, 4)
         passes[k]["gbs"] = round(nbytes / (kt[k] / 1e3) / 1e9, 1) if kt[k] > 0 else 0.0
+    prof_us = committed_avg_us(args.mode, dom)
     if dd:
         # dd arithmetic: ~12 flops per compulsory byte in the dominant pass, above the FP64 ridge (78.6 TFLOP/s over
-        # 8 TB/s = 9.8 flop/B): FP64-issue bound. MI355X's vector and matrix FP64 peaks are both 78.6 TFLOP/s.
+        # 8 TB/s = 9.8 flop/B): bound by FP64 instruction issue on the vector ALUs
         achieved = fl[dom] * tasks / (kt[dom] / 1e3) / 1e12
-        roof = {"bound": "mfma", "pipe": "fp64 (VALU; vector and matrix FP64 peaks are equal)", "kernel": dom,
-                "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP64_PEAK_TFLOPS, 4)}
+        roof = {"bound": "fp64-valu", "kernel": dom, "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4)}
+        if prof_us:
+            roof["frac_from_profile"] = round(fl[dom] * tasks / (prof_us / 1e6) / 1e12 / FP64_PEAK_TFLOPS, 4)
     else:
         # FP64 arithmetic: ~1.4 flops per compulsory byte, far below the ridge: HBM bound
         achieved = passes[dom]["gbs"]
         roof = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4)}
+    roof["avg_launch_us_events"] = round(kt[dom] * 1e3, 1)
+    roof["avg_launch_us_profile"] = prof_us
     roof["algorithmic_per_launch"] = (f"{fl[dom] * tasks / 1e9:.3f} GFLOP" if dd
                                       else f"{passes[dom]['algorithmic_gb']:.4f} GB")
     roof["traffic"] = committed_traffic(args.mode, dom)
@@ -769,30 +826,105 @@ def main():
     # count one flop each against a peak that counts an FMA as two, so frac stays well under this
     roof["valu_busy"] = committed_valu_busy(args.mode, dom)
 
-    # end to end (excluding file I/O): output copied back to the host
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    plan.execute(xx.data_ptr(), out.data_ptr(), stream)
-    host = out.cpu()
-    e2e_ms = (time.perf_counter() - t0) * 1e3
-    # kbsrc_resample end to end from host memory to host memory: plan, design, upload, staging, engine, download
-    t0 = time.perf_counter()
-    kbsrc_amd.resample(x, RATE_IN, RATE_OUT, args.mode)
-    resample_s = time.perf_counter() - t0
+    # split-precision form of the same conversion at the first slice count whose radix-16 stage error is within
+    # 2^-100 (15 slices: 2^-103.8 against a 256-bit DFT), timed like the plain form, outside its timed region
+    split = None
+    if dd and not args.no_split:
+        alt = torch.empty_like(out)
+        for _ in range(warm):
+            plan.execute_split(xx.data_ptr(), alt.data_ptr(), SPLIT_SLICES_2M100, stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            plan.execute_split(xx.data_ptr(), alt.data_ptr(), SPLIT_SLICES_2M100, stream)
+        torch.cuda.synchronize()
+        split_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        diff = float((alt - out).abs().max().item())
+        split = {"slices": SPLIT_SLICES_2M100, "ms_per_step": round(split_ms, 4),
+                 "ratio": round(ms / split_ms, 3), "max_abs_diff_vs_plain": diff}
+        print(f"split form, {SPLIT_SLICES_2M100} slices: {split_ms:.3f} ms/step (x{ms / split_ms:.3f} of plain), "
+              f"max |diff| vs plain {diff:.3g}", file=sys.stderr, flush=True)
+        del alt
+
     graph_ms, graph_same = graph_timing(args, plan, xx, out)
     print(f"captured graph replay: {graph_ms:.3f} ms/step; same output: {graph_same}", file=sys.stderr, flush=True)
 
+    # CPU baseline protocol: the reference's own engine (oracle/_ref/kbsrc, built from its sources) on the same input
+    # on this host, at its default thread count for the mode (8 exact, 4 fast: src/resampler.cpp:139) and at the
+    # CPU share one GPU gets on this host (16 threads; the reference's useful cap here is ch * ceil(nblk / 4) = 54,
+    # src/resampler.cpp:208), alternating with the GPU's three measured forms: (a) device kernels (the engine
+    # over HBM-resident input), (b) the engine with the input's upload and staging and the output's download to
+    # pinned host memory, (c) kbsrc_resample host memory to host memory (plan, design, upload, staging, engine,
+    # download). Median of the runs each.
     cpu = None
-    if not args.no_cpu and args.mode == "exact":
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        with tempfile.TemporaryDirectory(prefix="kbsrc-bench-") as td:
-            r = cpu_baseline(x, threads, td)
-        if r:
-            eng, design = r
-            cpu = {"value": round(nout_total / eng / 1e6, 2), "unit": "output Msamples/s", "cores": threads,
-                   "kind": "reference",
-                   "sample": f"full workload (2 x {frames} frames), reference engine resample time {eng:.3f} s "
-                             f"(median of 3, design {design:.3f} s excluded), {threads} threads"}
+    ref = os.path.join(ROOT, "oracle", "_ref", "kbsrc")
+    runs = args.cpu_runs
+    gpu_runs = {"a_kernels_ms": [], "b_engine_h2d_d2h_ms": [], "c_host_to_host_ms": []}
+    settings = {}
+    have_ref = not args.no_cpu and os.path.exists(ref)
+    default_threads = 8 if dd else 4
+    share = args.cpu_threads or min(16, host_cpu()["usable_cpus"])
+    thread_list = sorted({default_threads, share})
+    pinned = torch.empty(nout_total, dtype=torch.float64, pin_memory=True)
+    from tests.helpers import wav_f64
+    with tempfile.TemporaryDirectory(prefix="kbsrc-bench-") as td:
+        src, dst = os.path.join(td, "in.wav"), os.path.join(td, "out.wav")
+        if have_ref:
+            with open(src, "wb") as f:
+                f.write(wav_f64(x, CH, RATE_IN))
+            ref_engine_once(src, dst, share, extra=("--mode", args.mode))   # warm-up (page cache, first run)
+        for i in range(runs):
+            if have_ref:
+                for th in thread_list:
+                    r = ref_engine_once(src, dst, th, extra=("--mode", args.mode))
+                    if r is None:
+                        have_ref = False
+                        break
+                    settings.setdefault(th, []).append(r)
+                    print(f"cpu reference run {i + 1}/{runs}: {th} threads, engine {r[0]:.3f} s, design {r[1]:.3f} s",
+                          file=sys.stderr, flush=True)
+            for _ in range(WARM_MIN):   # the GPU idled during the CPU runs: clocks up again first
+                plan.execute(xx.data_ptr(), out.data_ptr(), stream)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(5):
+                plan.execute(xx.data_ptr(), out.data_ptr(), stream)
+            torch.cuda.synchronize()
+            gpu_runs["a_kernels_ms"].append((time.perf_counter() - t0) * 1e3 / 5)
+            t0 = time.perf_counter()
+            plan.stage(x, xx.data_ptr(), stream)
+            plan.execute(xx.data_ptr(), out.data_ptr(), stream)
+            pinned.copy_(out, non_blocking=True)
+            torch.cuda.synchronize()
+            gpu_runs["b_engine_h2d_d2h_ms"].append((time.perf_counter() - t0) * 1e3)
+            t0 = time.perf_counter()
+            kbsrc_amd.resample(x, RATE_IN, RATE_OUT, args.mode)
+            gpu_runs["c_host_to_host_ms"].append((time.perf_counter() - t0) * 1e3)
+            print(f"gpu run {i + 1}/{runs}: (a) {gpu_runs['a_kernels_ms'][-1]:.3f} ms, (b) "
+                  f"{gpu_runs['b_engine_h2d_d2h_ms'][-1]:.2f} ms, (c) {gpu_runs['c_host_to_host_ms'][-1]:.2f} ms",
+                  file=sys.stderr, flush=True)
+    del pinned
+    gpu_med = {k: round(median(v), 3) for k, v in gpu_runs.items()}
+    if have_ref and settings:
+        hc = host_cpu()
+        per = {}
+        for th, rs in sorted(settings.items()):
+            eng = median([r[0] for r in rs])
+            per[str(th)] = {"engine_s": round(eng, 4), "design_s": round(median([r[1] for r in rs]), 4),
+                            "engine_output_msamples_s": round(nout_total / eng / 1e6, 2),
+                            "speedup_a_kernels": round(eng * 1e3 / gpu_med["a_kernels_ms"], 1),
+                            "speedup_b_engine_h2d_d2h": round(eng * 1e3 / gpu_med["b_engine_h2d_d2h_ms"], 1),
+                            "speedup_c_host_to_host": round(eng * 1e3 / gpu_med["c_host_to_host_ms"], 1),
+                            "runs": len(rs)}
+        best = min(settings, key=lambda th: median([r[0] for r in settings[th]]))
+        eng = median([r[0] for r in settings[best]])
+        cpu = {"value": round(nout_total / eng / 1e6, 2), "unit": "output Msamples/s", "cores": best,
+               "kind": "reference",
+               "sample": f"full workload (2 x {frames} frames), the reference's engine (resample time of its -v line, "
+                         f"design and file I/O excluded), median of {runs} runs alternating with the GPU runs, at "
+                         f"{' and '.join(str(t) for t in thread_list)} threads; value at {best}",
+               "cpu_model": hc["model"], "host_logical_cpus": hc["logical_cpus"], "usable_cpus": hc["usable_cpus"],
+               "threads": per}
 
     algo_gflop = sum(fl.values()) * tasks / 1e9
     line = {
@@ -803,6 +935,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
+        "warmup_executes_run": warm,
         "ms_per_step": round(ms, 4),
         "higher_is_better": True,
         "scaling": "weak",
@@ -814,23 +947,45 @@ def main():
                    "nb": info["nb"], "tasks": tasks, "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "output_sha_ok": bool(sha_ok),
         "passes": passes,
+        "pass_sum_ms": round(sum(kt.values()), 4),
         "algorithmic_gflop_per_step": round(algo_gflop, 2),
         "step_tflops": round(algo_gflop / ms, 3),
         "design_s": round(t_design, 3),
         "stage_s": round(t_stage, 3),
-        "e2e_engine_plus_d2h_ms": round(e2e_ms, 3),
-        "resample_host_to_host_s": round(resample_s, 3),
+        "gpu_forms_median_ms": gpu_med,
         "graph_ms_per_step": round(graph_ms, 4),
         "graph_outputs_identical": graph_same,
         "tuned": bool(args.tune),
     }
+    if split:
+        line["split_vs_plain_at_2^-100"] = split["ratio"]
+        line["split"] = split
     if cpu:
         line["speedup_vs_cpu_engine"] = round(value / cpu["value"], 1)
-    print(json.dumps(line), flush=True)
-    del host
     plan.close()
+    report_children()
+    print(json.dumps(line), flush=True)
     return 0
+
+
+def report_children():
+    """Names any child process of this one still alive at the end (a bench must leave none)."""
+    kids = []
+    try:
+        for t in os.listdir(f"/proc/{os.getpid()}/task"):
+            with open(f"/proc/{os.getpid()}/task/{t}/children") as f:
+                kids += f.read().split()
+    except OSError:
+        return
+    for k in kids:
+        try:
+            with open(f"/proc/{k}/cmdline", "rb") as f:
+                cmd = f.read().replace(b"\0", b" ").decode(errors="replace")
+        except OSError:
+            cmd = "?"
+        print(f"child process still alive at exit: {k} {cmd}", file=sys.stderr, flush=True)
 
 
 if __name__ == "__main__":

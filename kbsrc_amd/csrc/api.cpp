@@ -515,6 +515,7 @@ int kbsrc_plan_execute_timed(kbsrc_plan* pl, const double* d_padded, double* d_o
         if (!pl || !pl->eng || !d_padded || !d_out || !ms) throw std::invalid_argument("bad input buffer");
         PassTimes t;
         pl->eng->execute(d_padded, d_out, static_cast<hipStream_t>(stream), &t);
+        hip_check(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "timed execute sync");
         ms[0] = t.fwd_col; ms[1] = t.fwd_row; ms[2] = t.mid; ms[3] = t.inv_col; ms[4] = t.inv_row;
     });
 }

@@ -445,28 +445,50 @@ bool CFFTPlan::needs_global() const {
 
 namespace {
 
-// brackets one launch with HIP events on the launch stream when timing is asked for
+// brackets one launch with HIP events on the launch stream when timing is asked for; the host does not wait here
+// (PassTimes::collect reads the events after the whole execute is queued)
 template <class F> void timed(const RunCtx& c, double* acc, F&& fn) {
     if (!c.times) {
         fn();
         return;
     }
+    // timing-only events: no system-scope fence when recorded (a fence writes back and invalidates the caches
+    // after the kernel, which the interval would then include)
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    hip_check(hipEventCreate(&e0), "event");
-    hip_check(hipEventCreate(&e1), "event");
+    hip_check(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence), "event");
+    if (hipEventCreateWithFlags(&e1, hipEventDisableSystemFence) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        throw HipError("event");
+    }
+    c.times->pending.push_back({e0, e1, acc});
     hip_check(hipEventRecord(e0, c.s), "event record");
     fn();
     hip_check(hipEventRecord(e1, c.s), "event record");
-    hip_check(hipEventSynchronize(e1), "event sync");
-    float ms = 0;
-    hip_check(hipEventElapsedTime(&ms, e0, e1), "event time");
-    *acc += ms;
     c.times->launches++;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
 }
 
 }  // namespace
+
+void PassTimes::collect() {
+    hipError_t err = hipSuccess;
+    for (const Pending& p : pending) {
+        float ms = 0;
+        if (err == hipSuccess) err = hipEventSynchronize(p.e1);
+        if (err == hipSuccess) err = hipEventElapsedTime(&ms, p.e0, p.e1);
+        if (err == hipSuccess) *p.acc += ms;
+        (void)hipEventDestroy(p.e0);
+        (void)hipEventDestroy(p.e1);
+    }
+    pending.clear();
+    hip_check(err, "pass timing");
+}
+
+PassTimes::~PassTimes() {
+    for (const Pending& p : pending) {
+        (void)hipEventDestroy(p.e0);
+        (void)hipEventDestroy(p.e1);
+    }
+}
 
 namespace {
 
@@ -744,7 +766,15 @@ void Engine::execute(const double* xx, double* out, hipStream_t s, PassTimes* ti
         double* const si[3] = {W0, W2, W3};
         run_cfft(iv_.c, c, LD_CPLX, W1, ST_OUT, nullptr, si, &local.inv_col, &local.inv_row);
     }
-    if (times) *times = local;
+    if (times) {
+        local.collect();
+        times->fwd_col += local.fwd_col;
+        times->fwd_row += local.fwd_row;
+        times->mid += local.mid;
+        times->inv_col += local.inv_col;
+        times->inv_row += local.inv_row;
+        times->launches += local.launches;
+    }
 }
 
 RunCtx Engine::context(const double* xx, double* out, hipStream_t s, int split) {

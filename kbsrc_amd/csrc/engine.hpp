@@ -90,9 +90,22 @@ struct RFFTPlan {
     void init(int N, bool as_dd, const DeviceConsts& k, hipStream_t s);
 };
 
-struct PassTimes {   // milliseconds per kernel kind, summed over one execute()
+// Milliseconds per kernel kind, summed over one execute(). Each launch is bracketed by two HIP events on its stream
+// and the launches are queued back to back (no host wait between them), so an interval is the kernel's own time
+// plus the queue's few-microsecond dispatch gap; the events are read once the execute has been queued.
+struct PassTimes {
     double fwd_col = 0, fwd_row = 0, mid = 0, inv_col = 0, inv_row = 0;
     int launches = 0;
+    struct Pending {
+        hipEvent_t e0, e1;
+        double* acc;
+    };
+    std::vector<Pending> pending;
+    void collect();   // waits for the pending events and adds their intervals (destroys the events)
+    PassTimes() = default;
+    PassTimes(const PassTimes&) = delete;
+    PassTimes& operator=(const PassTimes&) = delete;
+    ~PassTimes();     // events still pending (an execute that threw) are destroyed unread
 };
 
 // How a complex FFT runs: lane = the reference's engine form (sub-FFTs on its 4-lane types: one-TwoSum dd adds,
