@@ -16,6 +16,7 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <condition_variable>
 #include <vector>
 
 #include "engine.hpp"
@@ -93,6 +94,7 @@ template <class F> int guarded(F&& f) {
 struct kbsrc_graph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    std::shared_ptr<const void> keep;   // device tables the graph's kernels read (a batch's one-launch step tables)
     kbsrc_graph() = default;
     kbsrc_graph(const kbsrc_graph&) = delete;
     kbsrc_graph& operator=(const kbsrc_graph&) = delete;
@@ -150,7 +152,9 @@ struct kbsrc_batch {
         std::vector<StepLaunch> launches;
     };
     bool fuse = false;
-    std::vector<std::unique_ptr<Fused>> fused;   // by buffers, at most four kept
+    // by buffers, at most four kept here; a captured graph holds its own reference, so an entry evicted here stays
+    // alive while a graph that baked in its tables exists
+    std::vector<std::shared_ptr<Fused>> fused;
     ~kbsrc_batch() {
         for (hipEvent_t e : events) (void)hipEventDestroy(e);
         for (hipStream_t q : streams) (void)hipStreamDestroy(q);
@@ -234,25 +238,103 @@ void stage_dev(const Plan& p, bool lpc, const double* d_in, const double* first,
     hip_check(hipStreamSynchronize(s), "staging sync");
 }
 
-// Host <-> device copies of caller buffers through two pinned chunks of 64 MB (made on first use, kept for the
-// process, one caller at a time): each chunk is copied on the host by up to 8 threads (page faults on a fresh
-// caller buffer are taken in parallel) while the DMA of the other chunk runs. Synchronous: returns when done.
-struct PinnedChunks {
+// Host <-> device copies of caller buffers through pairs of pinned 64 MB chunks: each chunk is filled or drained on
+// the host by up to 8 threads (page faults on a fresh caller buffer are taken in parallel) while the DMA of the
+// other chunk runs. A pair has its own stream, which first waits (on the device) for the work already queued on the
+// caller's stream, so the copies keep the caller's order without any thread blocking on the caller's stream. Pairs
+// come from a small pool made on first use and kept for the process; a lease returns its pair only once the pair's
+// stream has drained, whatever the exit path. Without pinned memory the copies go straight from and to the caller's
+// buffer on the caller's stream. Synchronous: returns when done.
+struct ChunkPair {
     static constexpr size_t kBytes = 64u << 20;
-    std::mutex m;
     void* buf[2] = {nullptr, nullptr};
-    hipEvent_t ev[2] = {nullptr, nullptr};
-    void ensure() {
-        for (int i = 0; i < 2; ++i) {
-            if (!buf[i]) hip_check(hipHostMalloc(&buf[i], kBytes, hipHostMallocDefault), "pinned staging buffer");
-            if (!ev[i]) hip_check(hipEventCreateWithFlags(&ev[i], hipEventDisableTiming), "staging event");
+    hipEvent_t ev[2] = {nullptr, nullptr};   // the last DMA issued from / into buf[i]
+    hipEvent_t order = nullptr;              // recorded on the caller's stream
+    hipStream_t st = nullptr;
+};
+
+class ChunkPool {
+public:
+    // a free pair, a new one (at most kMax), or nullptr when pinned memory or the pair's resources cannot be had
+    ChunkPair* acquire() {
+        std::unique_lock<std::mutex> lock(m_);
+        for (;;) {
+            if (!free_.empty()) {
+                ChunkPair* c = free_.back();
+                free_.pop_back();
+                return c;
+            }
+            if (failed_) return nullptr;
+            if (made_ < kMax) {
+                ++made_;
+                lock.unlock();
+                ChunkPair* c = make();
+                lock.lock();
+                if (!c) {
+                    --made_;
+                    failed_ = true;
+                    cv_.notify_all();
+                }
+                return c;
+            }
+            cv_.wait(lock);
         }
     }
+    void release(ChunkPair* c) {
+        {
+            std::lock_guard<std::mutex> lock(m_);
+            free_.push_back(c);
+        }
+        cv_.notify_one();
+    }
+
+private:
+    static constexpr int kMax = 4;
+    static ChunkPair* make() {
+        auto c = std::make_unique<ChunkPair>();
+        bool ok = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) == hipSuccess &&
+                  hipEventCreateWithFlags(&c->order, hipEventDisableTiming) == hipSuccess;
+        for (int i = 0; i < 2 && ok; ++i)
+            ok = hipHostMalloc(&c->buf[i], ChunkPair::kBytes, hipHostMallocDefault) == hipSuccess &&
+                 hipEventCreateWithFlags(&c->ev[i], hipEventDisableTiming) == hipSuccess;
+        if (ok) return c.release();
+        (void)hipGetLastError();
+        for (int i = 0; i < 2; ++i) {
+            if (c->buf[i]) (void)hipHostFree(c->buf[i]);
+            if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
+        }
+        if (c->order) (void)hipEventDestroy(c->order);
+        if (c->st) (void)hipStreamDestroy(c->st);
+        return nullptr;
+    }
+    std::mutex m_;
+    std::condition_variable cv_;
+    std::vector<ChunkPair*> free_;
+    int made_ = 0;
+    bool failed_ = false;
 };
-PinnedChunks& pinned_chunks() {
-    static PinnedChunks* c = new PinnedChunks;   // kept until the process ends
-    return *c;
+ChunkPool& chunk_pool() {
+    static ChunkPool* p = new ChunkPool;   // kept until the process ends
+    return *p;
 }
+
+// A pair on loan: on every exit path its stream is drained (no DMA still reads or fills its chunks) before the pair
+// goes back to the pool.
+class ChunkLease {
+public:
+    ChunkLease() : c_(chunk_pool().acquire()) {}
+    ~ChunkLease() {
+        if (!c_) return;
+        (void)hipStreamSynchronize(c_->st);
+        chunk_pool().release(c_);
+    }
+    ChunkLease(const ChunkLease&) = delete;
+    ChunkLease& operator=(const ChunkLease&) = delete;
+    ChunkPair* get() const { return c_; }
+
+private:
+    ChunkPair* c_;
+};
 
 void parallel_copy(void* dst, const void* src, size_t bytes) {
     const unsigned hw = std::thread::hardware_concurrency();
@@ -278,38 +360,51 @@ void parallel_copy(void* dst, const void* src, size_t bytes) {
 }
 
 void upload_host(void* d_dst, const void* h_src, size_t bytes, hipStream_t s) {
-    PinnedChunks& c = pinned_chunks();
-    std::lock_guard<std::mutex> lock(c.m);
-    c.ensure();
-    const size_t K = PinnedChunks::kBytes;
+    ChunkLease lease;
+    ChunkPair* c = lease.get();
+    if (!c) {   // no pinned memory: straight from the caller's buffer
+        hip_check(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, s), "upload");
+        hip_check(hipStreamSynchronize(s), "upload sync");
+        return;
+    }
+    hip_check(hipEventRecord(c->order, s), "staging order");
+    hip_check(hipStreamWaitEvent(c->st, c->order, 0), "staging order");
+    const size_t K = ChunkPair::kBytes;
     for (size_t off = 0, k = 0; off < bytes; off += K, ++k) {
         const int b = (int)(k & 1);
         const size_t n = std::min(K, bytes - off);
-        if (k >= 2) hip_check(hipEventSynchronize(c.ev[b]), "staging wait");   // the DMA that last read buf[b]
-        parallel_copy(c.buf[b], static_cast<const char*>(h_src) + off, n);
-        hip_check(hipMemcpyAsync(static_cast<char*>(d_dst) + off, c.buf[b], n, hipMemcpyHostToDevice, s), "upload");
-        hip_check(hipEventRecord(c.ev[b], s), "staging event");
+        if (k >= 2) hip_check(hipEventSynchronize(c->ev[b]), "staging wait");   // the DMA that last read buf[b]
+        parallel_copy(c->buf[b], static_cast<const char*>(h_src) + off, n);
+        hip_check(hipMemcpyAsync(static_cast<char*>(d_dst) + off, c->buf[b], n, hipMemcpyHostToDevice, c->st),
+                  "upload");
+        hip_check(hipEventRecord(c->ev[b], c->st), "staging event");
     }
-    hip_check(hipStreamSynchronize(s), "upload sync");
+    hip_check(hipStreamSynchronize(c->st), "upload sync");
 }
 
 void download_host(void* h_dst, const void* d_src, size_t bytes, hipStream_t s) {
-    PinnedChunks& c = pinned_chunks();
-    std::lock_guard<std::mutex> lock(c.m);
-    c.ensure();
-    const size_t K = PinnedChunks::kBytes, chunks = (bytes + K - 1) / K;
+    ChunkLease lease;
+    ChunkPair* c = lease.get();
+    if (!c) {   // no pinned memory: straight into the caller's buffer
+        hip_check(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, s), "download");
+        hip_check(hipStreamSynchronize(s), "download sync");
+        return;
+    }
+    hip_check(hipEventRecord(c->order, s), "staging order");   // after the work that writes d_src
+    hip_check(hipStreamWaitEvent(c->st, c->order, 0), "staging order");
+    const size_t K = ChunkPair::kBytes, chunks = (bytes + K - 1) / K;
     auto issue = [&](size_t k) {
         const size_t off = k * K, n = std::min(K, bytes - off);
-        hip_check(hipMemcpyAsync(c.buf[k & 1], static_cast<const char*>(d_src) + off, n, hipMemcpyDeviceToHost, s),
-                  "download");
-        hip_check(hipEventRecord(c.ev[k & 1], s), "staging event");
+        hip_check(hipMemcpyAsync(c->buf[k & 1], static_cast<const char*>(d_src) + off, n, hipMemcpyDeviceToHost,
+                                 c->st), "download");
+        hip_check(hipEventRecord(c->ev[k & 1], c->st), "staging event");
     };
     if (chunks > 0) issue(0);
     for (size_t k = 0; k < chunks; ++k) {
         if (k + 1 < chunks) issue(k + 1);   // into the other chunk, whose copy-out finished last iteration
-        hip_check(hipEventSynchronize(c.ev[k & 1]), "staging wait");
+        hip_check(hipEventSynchronize(c->ev[k & 1]), "staging wait");
         const size_t off = k * K, n = std::min(K, bytes - off);
-        parallel_copy(static_cast<char*>(h_dst) + off, c.buf[k & 1], n);
+        parallel_copy(static_cast<char*>(h_dst) + off, c->buf[k & 1], n);
     }
 }
 
@@ -675,13 +770,13 @@ int kbsrc_batch_stage(kbsrc_batch* b, int job, const double* interleaved, int on
 namespace {
 
 // The batch's steps as one launch each over every group (nullptr when a group does not fit that form)
-const kbsrc_batch::Fused* fused_steps(kbsrc_batch* b, const double* d_in, double* d_out, hipStream_t s) {
+std::shared_ptr<kbsrc_batch::Fused> fused_steps(kbsrc_batch* b, const double* d_in, double* d_out, hipStream_t s) {
     for (const auto& f : b->fused)
-        if (f->in == d_in && f->out == d_out) return f.get();
+        if (f->in == d_in && f->out == d_out) return f;
     std::vector<EngineSteps> all(b->groups.size());
     for (size_t g = 0; g < b->groups.size(); ++g)
         if (!b->groups[g].eng->describe(d_in, d_out, all[g])) return nullptr;
-    auto F = std::make_unique<kbsrc_batch::Fused>();
+    auto F = std::make_shared<kbsrc_batch::Fused>();
     F->in = d_in;
     F->out = d_out;
     // passes at position k of every group, one launch per (policy, load, store)
@@ -737,14 +832,14 @@ const kbsrc_batch::Fused* fused_steps(kbsrc_batch* b, const double* d_in, double
     passes(false, 1);
     hip_check(hipStreamSynchronize(s), "batch step tables");
     if (b->fused.size() >= 4) b->fused.erase(b->fused.begin());
-    b->fused.push_back(std::move(F));
-    return b->fused.back().get();
+    b->fused.push_back(F);
+    return F;
 }
 
 // every group's engine on the batch's streams, forked from and joined back into s
 void batch_run(kbsrc_batch* b, const double* d_in, double* d_out, hipStream_t s) {
     if (b->fuse) {
-        if (const kbsrc_batch::Fused* F = fused_steps(b, d_in, d_out, s)) {
+        if (const std::shared_ptr<kbsrc_batch::Fused> F = fused_steps(b, d_in, d_out, s)) {
             for (const kbsrc_batch::StepLaunch& L : F->launches) {
                 const hipError_t e = L.mid
                     ? launch_mid_multi(L.policy, reinterpret_cast<const MidArgs*>(L.descs.d()),
@@ -818,8 +913,10 @@ int kbsrc_plan_capture(kbsrc_plan* pl, const double* d_padded, double* d_out, kb
 int kbsrc_batch_capture(kbsrc_batch* b, const double* d_in, double* d_out, kbsrc_graph** graph) {
     return guarded([&] {
         if (!b || !d_in || !d_out || !graph) throw std::invalid_argument("bad input buffer");
-        if (b->fuse) fused_steps(b, d_in, d_out, b->streams[0]);   // tables made outside the capture
+        std::shared_ptr<kbsrc_batch::Fused> F;
+        if (b->fuse) F = fused_steps(b, d_in, d_out, b->streams[0]);   // tables made outside the capture
         *graph = capture([&](hipStream_t cs) { batch_run(b, d_in, d_out, cs); });
+        (*graph)->keep = F;
     });
 }
 

@@ -127,3 +127,34 @@ def test_batch_rejections():
         gpu().Batch([(1000, 1, 48000, 0)])
     with pytest.raises(ValueError, match="bad input buffer"):
         gpu().Batch([(1000, 0, 48000, 44100)])
+
+
+def test_batch_graph_keeps_its_step_tables(monkeypatch):
+    """A graph captured from a batch that runs one launch per step keeps the step tables it was captured with,
+    even after executes on four other pairs of buffers have pushed them out of the batch's own cache."""
+    import torch
+    monkeypatch.setenv("KBSRC_BATCH_FUSE", "1")
+    say("batch graph after step-table eviction")
+    xs = signals(FUSABLE, 23)
+    b = gpu().Batch([(x.shape[0], x.shape[1], ri, ro) for x, (_, _, ri, ro) in zip(xs, FUSABLE)], "exact", length=0.05)
+    assert b.fused
+    d_in = torch.zeros(b.in_doubles, dtype=torch.float64, device="cuda")
+    for j, x in enumerate(xs):
+        b.stage(j, x, d_in.data_ptr())
+    outs = [torch.zeros(b.out_doubles, dtype=torch.float64, device="cuda") for _ in range(5)]
+    b.execute(d_in.data_ptr(), outs[0].data_ptr())
+    torch.cuda.synchronize()
+    want = outs[0].clone()
+    g = b.capture(d_in.data_ptr(), outs[0].data_ptr())
+    for o in outs[1:]:   # four other buffer pairs: the captured pair's tables leave the batch's cache
+        b.execute(d_in.data_ptr(), o.data_ptr())
+    torch.cuda.synchronize()
+    for o in outs[1:]:
+        assert torch.equal(o, want)
+    outs[0].zero_()
+    torch.cuda.synchronize()
+    g.launch()
+    torch.cuda.synchronize()
+    assert torch.equal(outs[0], want)
+    g.close()
+    b.close()

@@ -186,3 +186,19 @@ def test_cli_reports_errors_in_reference_order(tmp_path):
     r = subprocess.run([H.GPU_CLI, "--rate", "44100", str(tmp_path / "missing.wav"), str(out)], capture_output=True,
                        text=True, timeout=60)
     assert r.returncode == 1 and "cannot open input" in r.stderr
+
+
+def test_host_buffer_conversions_from_two_threads():
+    """kbsrc_resample from two threads at once, each input larger than the pinned staging pair (2 x 64 MB), gives
+    each thread the bits it gets alone."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import kbsrc_amd
+    frames = 96000 * 90   # stereo: 138 MB of input per conversion
+    xs = [np.random.default_rng(40 + k).uniform(-0.5, 0.5, (frames, 2)) for k in range(2)]
+    print("two-thread host conversions: 2 x 138 MB", flush=True)
+    alone = [kbsrc_amd.resample(x, 96000, 44100, "exact") for x in xs]
+    with ThreadPoolExecutor(2) as ex:
+        both = list(ex.map(lambda x: kbsrc_amd.resample(x, 96000, 44100, "exact"), xs))
+    for a, b in zip(alone, both):
+        assert a.tobytes() == b.tobytes()
