@@ -87,6 +87,16 @@ int kbsrc_plan_stats(const kbsrc_plan* plan, kbsrc_stats* stats);
  * d_out: channels x nout doubles, planar. Asynchronous on stream. */
 int kbsrc_plan_execute(kbsrc_plan* plan, const double* d_padded, double* d_out, void* stream);
 
+/* The same execute with no device allocation (no counterpart in the reference; for callers that cannot allocate:
+ * code under HIP graph capture, tight serving loops). The work buffers come from one static device array of
+ * KBSRC_NOALLOC_SCRATCH_BYTES per GPU, held from the first such call on that GPU until the process ends; tasks run
+ * in as many launches as that array needs, with the bits of kbsrc_plan_execute. A plan whose work buffers for one
+ * task exceed the array is rejected (KBSRC_EINVAL). Every call on a GPU shares that GPU's array: calls that overlap
+ * in time on one GPU (other streams, other threads) race and must be serialised by the caller. Plans with equal
+ * rates have no engine and are rejected as by kbsrc_plan_execute. Asynchronous on stream. */
+#define KBSRC_NOALLOC_SCRATCH_BYTES (128LL << 20)
+int kbsrc_plan_execute_noalloc(kbsrc_plan* plan, const double* d_padded, double* d_out, void* stream);
+
 /* Same, synchronous, with HIP-event times in milliseconds of the five kernel passes summed over the launch:
  * ms[0] forward column, ms[1] forward row, ms[2] split/multiply/fold, ms[3] inverse column, ms[4] inverse row. */
 int kbsrc_plan_execute_timed(kbsrc_plan* plan, const double* d_padded, double* d_out, void* stream, double* ms);

@@ -68,6 +68,7 @@ def _load():
         "kbsrc_plan_stage": (I, [P, P, P, P]),
         "kbsrc_plan_execute": (I, [P, P, P, P]),
         "kbsrc_plan_execute_split": (I, [P, P, P, I, P]),
+        "kbsrc_plan_execute_noalloc": (I, [P, P, P, P]),
         "kbsrc_plan_execute_timed": (I, [P, P, P, P, ctypes.POINTER(D)]),
         "kbsrc_plan_destroy": (None, [P]),
         "kbsrc_plan_stage_device": (I, [P, P, P, P]),
@@ -332,6 +333,12 @@ class Plan:
     def execute(self, d_padded: int, d_out: int, stream: int = 0):
         _check(_lib.kbsrc_plan_execute(self._h, ctypes.c_void_p(d_padded), ctypes.c_void_p(d_out),
                                        ctypes.c_void_p(stream)))
+
+    def execute_noalloc(self, d_padded: int, d_out: int, stream: int = 0):
+        """The execute with its work buffers in the library's static device array (no device allocation); same
+        bits. Calls on one GPU share the array and must not overlap."""
+        _check(_lib.kbsrc_plan_execute_noalloc(self._h, ctypes.c_void_p(d_padded), ctypes.c_void_p(d_out),
+                                               ctypes.c_void_p(stream)))
 
     def capture(self, d_padded: int, d_out: int) -> Graph:
         """The execute on these buffers as a replayable HIP graph."""

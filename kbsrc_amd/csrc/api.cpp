@@ -25,6 +25,8 @@
 
 #pragma clang fp contract(off)
 
+static_assert(kbsrc::gpu::kScratchBytes == (size_t)KBSRC_NOALLOC_SCRATCH_BYTES, "static scratch size");
+
 namespace {
 
 using namespace kbsrc::gpu;
@@ -594,6 +596,14 @@ int kbsrc_plan_execute(kbsrc_plan* pl, const double* d_padded, double* d_out, vo
     return guarded([&] {
         if (!pl || !pl->eng || !d_padded || !d_out) throw std::invalid_argument("bad input buffer");
         pl->eng->execute(d_padded, d_out, static_cast<hipStream_t>(stream));
+    });
+}
+
+int kbsrc_plan_execute_noalloc(kbsrc_plan* pl, const double* d_padded, double* d_out, void* stream) {
+    return guarded([&] {
+        if (!pl || !pl->eng || !d_padded || !d_out) throw std::invalid_argument("bad input buffer");
+        // the work buffers come from the library's static device array (its address is looked up once per device)
+        pl->eng->execute_in(d_padded, d_out, static_cast<hipStream_t>(stream), scratch_memory(), kScratchBytes);
     });
 }
 

@@ -712,8 +712,6 @@ Engine::Engine(const Plan& p, const double* H, long long nH, int Jg, hipStream_t
 }
 
 void Engine::execute(const double* xx, double* out, hipStream_t s, PassTimes* times, int split) {
-    PassTimes local;
-    const int pol_four = dd_ ? POL_DDS : POL_F64;
     if (split) {   // tables of the split form, made on first use
         if (!dd_ || !split_slices_ok(split)) throw std::invalid_argument("option out of range");
         fw_.c.init_split(s);
@@ -728,17 +726,59 @@ void Engine::execute(const double* xx, double* out, hipStream_t s, PassTimes* ti
             g_[1].alloc((size_t)ts_ * sizeof(double) * tb_);
         }
     }
+    const Work w{{w_[0].d(), w_[1].d(), w_[2].d(), w_[3].d()}, {g_[0].d(), g_[1].d()}, tb_};
+    run(xx, out, s, times, split, w);
+}
+
+namespace {
+constexpr size_t kCarveAlign = 256;
+size_t carve_bytes(size_t tasks, size_t per_task) { return (tasks * per_task + kCarveAlign - 1) / kCarveAlign * kCarveAlign; }
+}  // namespace
+
+size_t Engine::task_bytes() const {
+    const bool blue = fw_.c.blue || iv_.c.blue;
+    const bool glob = fw_.c.needs_global() || iv_.c.needs_global() || global_forced();
+    return (size_t)ts_ * sizeof(double) * ((blue ? 4 : 2) + (glob ? 2 : 0));
+}
+
+void Engine::execute_in(const double* xx, double* out, hipStream_t s, double* mem, size_t bytes) {
+    const bool blue = fw_.c.blue || iv_.c.blue;
+    const bool glob = fw_.c.needs_global() || iv_.c.needs_global() || global_forced();
+    const int nw = blue ? 4 : 2, nbuf = nw + (glob ? 2 : 0);
+    const size_t per = (size_t)ts_ * sizeof(double);   // one task in one buffer
+    if (!mem || reinterpret_cast<uintptr_t>(mem) % kCarveAlign)
+        throw std::invalid_argument("scratch memory must be 256-byte aligned");
+    size_t tb = bytes / nbuf / per;   // tasks per launch, each buffer's start kept aligned
+    while (tb > 0 && nbuf * carve_bytes(tb, per) > bytes) --tb;
+    if (tb == 0)
+        throw std::invalid_argument("plan too large for the static scratch: one task needs " +
+                                    std::to_string(nbuf * carve_bytes(1, per)) + " bytes of work buffers, " +
+                                    std::to_string(bytes) + " available");
+    tb = std::min<size_t>(tb, (size_t)ntask_);
+    Work w{{nullptr, nullptr, nullptr, nullptr}, {nullptr, nullptr}, (int)std::min<size_t>(tb, 0x7fffffff)};
+    char* p = reinterpret_cast<char*>(mem);
+    for (int i = 0; i < nbuf; ++i, p += carve_bytes(tb, per)) {
+        double* b = reinterpret_cast<double*>(p);
+        if (i < nw) w.w[i] = b;
+        else w.g[i - nw] = b;
+    }
+    run(xx, out, s, nullptr, 0, w);
+}
+
+void Engine::run(const double* xx, double* out, hipStream_t s, PassTimes* times, int split, const Work& w) {
+    PassTimes local;
+    const int pol_four = dd_ ? POL_DDS : POL_F64;
     const bool fused = !split && use_fixed() && env_flag("KBSRC_FUSED") && !Jg_ && fw_.c.four && iv_.c.four &&
                        has_fixed(pol_four, fw_.c.N2, LD_CPLX, ST_SPLIT) && has_fixed(pol_four, iv_.c.N1, LD_FOLD, ST_CPLX);
-    RunCtx c = context(xx, out, s, split);
+    RunCtx c = context(xx, out, s, split, w);
     c.times = times ? &local : nullptr;
     PassArgs& a = c.proto;
-    double* W0 = w_[0].d();
-    double* W1 = w_[1].d();
-    double* W2 = w_[2].d();
-    double* W3 = w_[3].d();
-    for (long long t0 = 0; t0 < ntask_; t0 += tb_) {
-        const int nt = (int)std::min<long long>(tb_, ntask_ - t0);
+    double* W0 = w.w[0];
+    double* W1 = w.w[1];
+    double* W2 = w.w[2];
+    double* W3 = w.w[3];
+    for (long long t0 = 0; t0 < ntask_; t0 += w.tb) {
+        const int nt = (int)std::min<long long>(w.tb, ntask_ - t0);
         a.ntask = nt;
         a.task0 = t0;
         if (fused) {
@@ -777,16 +817,16 @@ void Engine::execute(const double* xx, double* out, hipStream_t s, PassTimes* ti
     }
 }
 
-RunCtx Engine::context(const double* xx, double* out, hipStream_t s, int split) {
+RunCtx Engine::context(const double* xx, double* out, hipStream_t s, int split, const Work& w) {
     (void)xx;
     RunCtx c;
     c.ts = ts_; c.k = &k_; c.dd = dd_; c.lane = true; c.s = s;
     PassArgs& a = c.proto;
-    a.gscr0 = g_[0].d();
-    a.gscr1 = g_[1].d();
+    a.gscr0 = w.g[0];
+    a.gscr1 = w.g[1];
     a.tdesc = tdesc_;
     a.split = split;
-    a.adig = split ? reinterpret_cast<const long*>(adig_[split].d()) : nullptr;
+    a.adig = split ? reinterpret_cast<const long*>(adig_.at(split).d()) : nullptr;
     a.nblk = p_.nblk;
     a.in_cstride = p_.xstride;
     a.in_bstride = (long long)p_.hop * p_.Q;
@@ -805,7 +845,8 @@ RunCtx Engine::context(const double* xx, double* out, hipStream_t s, int split) 
 bool Engine::describe(const double* xx, double* out, EngineSteps& steps) {
     if (ntask_ > tb_) return false;
     bool ok = true;
-    RunCtx c = context(xx, out, nullptr, 0);
+    const Work w{{w_[0].d(), w_[1].d(), w_[2].d(), w_[3].d()}, {g_[0].d(), g_[1].d()}, tb_};
+    RunCtx c = context(xx, out, nullptr, 0, w);
     c.collect_ok = &ok;
     PassArgs& a = c.proto;
     a.ntask = (int)ntask_;

@@ -174,6 +174,11 @@ public:
     // split > 0: the split-precision form (exact-mode plans only; not bit-exact): sub-FFTs whose first two stages
     // are radix 4 run them as one int8-MFMA radix-16 stage with `split` base-128 slices per operand.
     void execute(const double* xx, double* out, hipStream_t s, PassTimes* times = nullptr, int split = 0);
+    // The plain execute with every work buffer carved from the caller's device memory [mem, mem + bytes) instead of
+    // the engine's own: tasks run in as many launches as that memory needs, with the same bits. Allocates nothing.
+    // Throws std::invalid_argument when one task's buffers do not fit.
+    void execute_in(const double* xx, double* out, hipStream_t s, double* mem, size_t bytes);
+    size_t task_bytes() const;   // bytes one task needs over all its work buffers
     // The same execute described as generic-kernel steps on these buffers (for one launch per step over several
     // engines); false when it does not fit that form (Bluestein lengths, sub-FFTs longer than an LDS tile, more
     // tasks than one batch of the work buffers).
@@ -183,7 +188,13 @@ public:
     const Plan& plan() const { return p_; }
 
 private:
-    RunCtx context(const double* xx, double* out, hipStream_t s, int split);
+    struct Work {
+        double* w[4];   // the work buffers (two, four with Bluestein)
+        double* g[2];   // global-pass scratch (sub-FFTs longer than an LDS tile), or null
+        int tb;         // tasks per launch
+    };
+    void run(const double* xx, double* out, hipStream_t s, PassTimes* times, int split, const Work& w);
+    RunCtx context(const double* xx, double* out, hipStream_t s, int split, const Work& w);
     Plan p_;
     bool dd_;
     int Jg_ = 0;
@@ -197,6 +208,11 @@ private:
     DevBuf H_, w_[4], g_[2];
     std::map<int, DevBuf> adig_;   // split form: the DFT matrix's slices per slice count
 };
+
+// Bytes of the device array kept for executes that must not allocate (scratch.hip), and its address on the
+// current device (taken once per device from the symbol; the array lives until the process ends).
+constexpr size_t kScratchBytes = (size_t)128 << 20;
+double* scratch_memory();
 
 // The split form's 16-point DFT matrix as S base-128 slices in the MFMA A layout: M / 2, M the real form of
 // F = e^{-2 pi i k n / 16} with rows (Re y[0..15], Im y[0..15]) and columns (Re x[0], Im x[0], Re x[1], ...),
