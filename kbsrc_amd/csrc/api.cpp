@@ -35,25 +35,25 @@ thread_local std::string g_error;
 
 double now() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
 
-// Round-to-nearest for the host-side dd work of Exact; the caller's MXCSR rounding bits come back on exit
-// (the reference's guard: src/dispatch.cpp:37-67). Exception flags and FTZ/DAZ are left alone.
-class NearestRounding {
-public:
-    explicit NearestRounding(bool active) {
-        if (!active) return;
-        saved_ = _mm_getcsr() & kRC;
-        if (saved_) _mm_setcsr(_mm_getcsr() & ~kRC);
+// Runs f with the SSE unit, where the host's double-double arithmetic runs, rounding to nearest when `exact` is set,
+// and gives the caller's rounding mode back afterwards, also when f throws (the reference computes Exact in
+// round-to-nearest whatever the caller's mode: src/dispatch.cpp:31-36,73). Only the MXCSR rounding field changes:
+// exception flags raised meanwhile stay raised, and FTZ/DAZ and the x87 control word are left as they are.
+template <class F> void in_nearest_mode(bool exact, F&& f) {
+    const unsigned caller = _MM_GET_ROUNDING_MODE();
+    if (!exact || caller == _MM_ROUND_NEAREST) {
+        f();
+        return;
     }
-    ~NearestRounding() {
-        if (saved_) _mm_setcsr((_mm_getcsr() & ~kRC) | saved_);
+    _MM_SET_ROUNDING_MODE(_MM_ROUND_NEAREST);
+    try {
+        f();
+    } catch (...) {
+        _MM_SET_ROUNDING_MODE(caller);
+        throw;
     }
-    NearestRounding(const NearestRounding&) = delete;
-    NearestRounding& operator=(const NearestRounding&) = delete;
-
-private:
-    static constexpr unsigned kRC = 0x6000;
-    unsigned saved_ = 0;
-};
+    _MM_SET_ROUNDING_MODE(caller);
+}
 
 kbsrc_options defaults() {
     kbsrc_options o;
@@ -457,40 +457,42 @@ void fill_stats(const kbsrc_plan& pl, double process_seconds, kbsrc_stats* st) {
 void resample_host(const double* in, long long frames, int channels, int rate_in, int rate_out,
                    const kbsrc_options& o, double* out, kbsrc_stats* st) {
     if (channels < 1 || frames < 0 || (frames > 0 && !in)) throw std::invalid_argument("bad input buffer");
-    NearestRounding rn(o.precision != KBSRC_FAST);
-    const double t0 = now();
-    if (channels >= 1 && rate_in >= 1 && rate_out >= 1 && rate_in == rate_out) {
-        // equal rates: the input is copied (src/resampler.cpp:106-111); options are still validated
-        check_options(o);
-        make_plan(frames, channels, rate_in, rate_out, o.precision == KBSRC_FAST, o.length, o.beta, o.block);
-        for (int c = 0; c < channels; ++c)
-            for (long long i = 0; i < frames; ++i) out[(size_t)c * frames + i] = in[i * channels + c];
-        if (st) {
-            kbsrc_plan pl;
-            pl.p.P = pl.p.Q = 1;
-            pl.p.equal = true;
-            fill_stats(pl, 0, st);
-            st->design_seconds = now() - t0;
+    in_nearest_mode(o.precision != KBSRC_FAST, [&] {
+        const double t0 = now();
+        if (channels >= 1 && rate_in >= 1 && rate_out >= 1 && rate_in == rate_out) {
+            // equal rates: the input is copied (src/resampler.cpp:106-111); options are still validated
+            check_options(o);
+            make_plan(frames, channels, rate_in, rate_out, o.precision == KBSRC_FAST, o.length, o.beta, o.block);
+            for (int c = 0; c < channels; ++c)
+                for (long long i = 0; i < frames; ++i) out[(size_t)c * frames + i] = in[i * channels + c];
+            if (st) {
+                kbsrc_plan pl;
+                pl.p.P = pl.p.Q = 1;
+                pl.p.equal = true;
+                fill_stats(pl, 0, st);
+                st->design_seconds = now() - t0;
+            }
+            return;
         }
-        return;
-    }
-    hipStream_t s = nullptr;
-    hip_check(hipStreamCreate(&s), "stream");
-    struct StreamGuard {
-        hipStream_t s;
-        ~StreamGuard() { (void)hipStreamDestroy(s); }
-    } sg{s};
-    auto pl = make(frames, channels, rate_in, rate_out, o, s);
-    const Plan& p = pl->p;
-    const double t1 = now();
-    DevBuf xx((size_t)p.ch * p.xstride * sizeof(double)), dout((size_t)std::max(1LL, p.ch * p.nout) * sizeof(double));
-    stage(p, o.lpc_edges, in, xx.d(), s);
-    pl->design_seconds += now() - t1;
-    const double t2 = now();
-    pl->eng->execute(xx.d(), dout.d(), s);
-    if (p.ch * p.nout > 0) download_host(out, dout.d(), sizeof(double) * p.ch * p.nout, s);
-    hip_check(hipStreamSynchronize(s), "engine sync");
-    fill_stats(*pl, now() - t2, st);
+        hipStream_t s = nullptr;
+        hip_check(hipStreamCreate(&s), "stream");
+        struct StreamGuard {
+            hipStream_t s;
+            ~StreamGuard() { (void)hipStreamDestroy(s); }
+        } sg{s};
+        auto pl = make(frames, channels, rate_in, rate_out, o, s);
+        const Plan& p = pl->p;
+        const double t1 = now();
+        DevBuf xx((size_t)p.ch * p.xstride * sizeof(double));
+        DevBuf dout((size_t)std::max(1LL, p.ch * p.nout) * sizeof(double));
+        stage(p, o.lpc_edges, in, xx.d(), s);
+        pl->design_seconds += now() - t1;
+        const double t2 = now();
+        pl->eng->execute(xx.d(), dout.d(), s);
+        if (p.ch * p.nout > 0) download_host(out, dout.d(), sizeof(double) * p.ch * p.nout, s);
+        hip_check(hipStreamSynchronize(s), "engine sync");
+        fill_stats(*pl, now() - t2, st);
+    });
 }
 
 }  // namespace
@@ -516,7 +518,8 @@ int kbsrc_output_frames(long long frames, int rate_in, int rate_out, long long* 
 
 int kbsrc_resample(const double* in, long long frames, int channels, int rate_in, int rate_out,
                    const kbsrc_options* opt, double* out, kbsrc_stats* stats) {
-    return guarded([&] { resample_host(in, frames, channels, rate_in, rate_out, opt ? *opt : defaults(), out, stats); });
+    return guarded(
+        [&] { resample_host(in, frames, channels, rate_in, rate_out, opt ? *opt : defaults(), out, stats); });
 }
 
 int kbsrc_plan_create(long long frames, int channels, int rate_in, int rate_out, const kbsrc_options* opt,
@@ -524,20 +527,21 @@ int kbsrc_plan_create(long long frames, int channels, int rate_in, int rate_out,
     return guarded([&] {
         if (!plan) throw std::invalid_argument("bad input buffer");
         const kbsrc_options o = opt ? *opt : defaults();
-        NearestRounding rn(o.precision != KBSRC_FAST);
-        check_options(o);   // argument errors before any device call
-        (void)make_plan(frames, channels, rate_in, rate_out, o.precision == KBSRC_FAST, o.length, o.beta, o.block);
-        // the design runs on a stream of its own that does not wait for the legacy default stream, so a caller
-        // may copy input on other streams meanwhile
-        hipStream_t s = nullptr;
-        hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream");
-        struct StreamGuard {
-            hipStream_t s;
-            ~StreamGuard() { (void)hipStreamDestroy(s); }
-        } sg{s};
-        auto pl = make(frames, channels, rate_in, rate_out, o, s);
-        hip_check(hipStreamSynchronize(s), "plan sync");
-        *plan = pl.release();
+        in_nearest_mode(o.precision != KBSRC_FAST, [&] {
+            check_options(o);   // argument errors before any device call
+            (void)make_plan(frames, channels, rate_in, rate_out, o.precision == KBSRC_FAST, o.length, o.beta, o.block);
+            // the design runs on a stream of its own that does not wait for the legacy default stream, so a caller
+            // may copy input on other streams meanwhile
+            hipStream_t s = nullptr;
+            hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "stream");
+            struct StreamGuard {
+                hipStream_t s;
+                ~StreamGuard() { (void)hipStreamDestroy(s); }
+            } sg{s};
+            auto pl = make(frames, channels, rate_in, rate_out, o, s);
+            hip_check(hipStreamSynchronize(s), "plan sync");
+            *plan = pl.release();
+        });
     });
 }
 
@@ -557,8 +561,9 @@ int kbsrc_plan_stage_device(kbsrc_plan* pl, const double* d_interleaved, double*
     return guarded([&] {
         if (!pl || !d_padded || (pl->p.nin > 0 && !d_interleaved)) throw std::invalid_argument("bad input buffer");
         if (!pl->eng) throw std::invalid_argument("plan has no engine (equal rates)");
-        NearestRounding rn(pl->opt.precision != KBSRC_FAST);
-        stage_device(pl->p, pl->opt.lpc_edges, d_interleaved, d_padded, static_cast<hipStream_t>(stream));
+        in_nearest_mode(pl->opt.precision != KBSRC_FAST, [&] {
+            stage_device(pl->p, pl->opt.lpc_edges, d_interleaved, d_padded, static_cast<hipStream_t>(stream));
+        });
     });
 }
 
@@ -587,8 +592,9 @@ int kbsrc_plan_stage(kbsrc_plan* pl, const double* in, double* d_padded, void* s
     return guarded([&] {
         if (!pl || !d_padded || (pl->p.nin > 0 && !in)) throw std::invalid_argument("bad input buffer");
         if (!pl->eng) throw std::invalid_argument("plan has no engine (equal rates)");
-        NearestRounding rn(pl->opt.precision != KBSRC_FAST);
-        stage(pl->p, pl->opt.lpc_edges, in, d_padded, static_cast<hipStream_t>(stream));
+        in_nearest_mode(pl->opt.precision != KBSRC_FAST, [&] {
+            stage(pl->p, pl->opt.lpc_edges, in, d_padded, static_cast<hipStream_t>(stream));
+        });
     });
 }
 
@@ -610,8 +616,10 @@ int kbsrc_plan_execute_noalloc(kbsrc_plan* pl, const double* d_padded, double* d
 int kbsrc_plan_execute_split(kbsrc_plan* pl, const double* d_padded, double* d_out, int slices, void* stream) {
     return guarded([&] {
         if (!pl || !pl->eng || !d_padded || !d_out) throw std::invalid_argument("bad input buffer");
-        NearestRounding rn(true);   // the split tables are made in host dd arithmetic on first use
-        pl->eng->execute(d_padded, d_out, static_cast<hipStream_t>(stream), nullptr, slices);
+        // the split tables are made in host dd arithmetic on first use
+        in_nearest_mode(true, [&] {
+            pl->eng->execute(d_padded, d_out, static_cast<hipStream_t>(stream), nullptr, slices);
+        });
     });
 }
 
@@ -632,106 +640,107 @@ int kbsrc_batch_create(int njobs, const kbsrc_job* jobs, const kbsrc_options* op
         if (njobs < 1 || !jobs || !batch) throw std::invalid_argument("bad input buffer");
         const kbsrc_options o = opt ? *opt : defaults();
         check_options(o);
-        NearestRounding rn(o.precision != KBSRC_FAST);
-        const double t0 = now();
-        auto b = std::make_unique<kbsrc_batch>();
-        b->opt = o;
-        b->jobs.resize(njobs);
-        auto align = [](long long v) { return (v + 31) / 32 * 32; };
-        for (int j = 0; j < njobs; ++j) {
-            const kbsrc_job& J = jobs[j];
-            if (J.channels < 1 || J.frames < 0) throw std::invalid_argument("bad input buffer");
-            kbsrc_batch::Job& job = b->jobs[j];
-            job.p = make_plan(J.frames, J.channels, J.rate_in, J.rate_out, o.precision == KBSRC_FAST, o.length, o.beta,
-                              o.block);
-            const Plan& p = job.p;
-            job.in_off = b->in_total;
-            job.in_len = p.equal ? p.nin * p.ch : p.ch * p.xstride;
-            b->in_total += align(job.in_len);
-            job.out_off = b->out_total;
-            job.out_len = p.ch * p.nout;
-            b->out_total += align(job.out_len);
-            if (p.equal) continue;
-            int g = 0;   // jobs with the same rates and block size share a group
-            for (; g < (int)b->groups.size(); ++g) {
-                const Plan& q = b->groups[g].p;
-                if (q.fi == p.fi && q.fo == p.fo && q.nb == p.nb) break;
-            }
-            if (g == (int)b->groups.size()) {
-                b->groups.emplace_back();
-                b->groups.back().p = p;
-            }
-            job.group = g;
-            b->groups[g].jobs.push_back(j);
-            b->groups[g].ntask += (long long)p.ch * p.nblk;
-        }
-        const int nstreams = (int)std::min<size_t>(4, std::max<size_t>(1, b->groups.size()));
-        {   // groups to streams, largest first onto the least-loaded stream (work ~ tasks x n log n of both FFTs)
-            std::vector<std::pair<double, int>> w;
-            for (int g = 0; g < (int)b->groups.size(); ++g) {
-                const Plan& q = b->groups[g].p;
-                const double nb = q.Nb, mb = q.Mb;
-                w.push_back({(double)b->groups[g].ntask * (nb * std::log2(nb + 2) + mb * std::log2(mb + 2)), g});
-            }
-            std::sort(w.begin(), w.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
-            std::vector<double> load(nstreams, 0.0);
-            for (const auto& [wt, g] : w) {
-                const int i = (int)(std::min_element(load.begin(), load.end()) - load.begin());
-                b->groups[g].stream = i;
-                load[i] += wt;
-            }
-        }
-        for (int i = 0; i < nstreams; ++i) {
-            hipStream_t q = nullptr;
-            hip_check(hipStreamCreateWithFlags(&q, hipStreamNonBlocking), "stream");
-            b->streams.push_back(q);
-        }
-        for (int i = 0; i <= nstreams; ++i) {
-            hipEvent_t e = nullptr;
-            hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
-            b->events.push_back(e);
-        }
-        hipStream_t s = b->streams[0];
-        size_t free_b = 0, total_b = 0;
-        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
-        const size_t per_group = std::clamp<size_t>(free_b / (16 * std::max<size_t>(1, b->groups.size())),
-                                                    (size_t)256 << 20, (size_t)4 << 30);
-        DeviceConsts k;
-        k.init(s);
-        for (kbsrc_batch::Group& G : b->groups) {
-            // task table: (input offset of the block, output offset of the channel, block, channel length)
-            std::vector<long long> t;
-            t.reserve(4 * G.ntask);
-            for (int j : G.jobs) {
-                const kbsrc_batch::Job& job = b->jobs[j];
+        in_nearest_mode(o.precision != KBSRC_FAST, [&] {
+            const double t0 = now();
+            auto b = std::make_unique<kbsrc_batch>();
+            b->opt = o;
+            b->jobs.resize(njobs);
+            auto align = [](long long v) { return (v + 31) / 32 * 32; };
+            for (int j = 0; j < njobs; ++j) {
+                const kbsrc_job& J = jobs[j];
+                if (J.channels < 1 || J.frames < 0) throw std::invalid_argument("bad input buffer");
+                kbsrc_batch::Job& job = b->jobs[j];
+                job.p = make_plan(J.frames, J.channels, J.rate_in, J.rate_out, o.precision == KBSRC_FAST, o.length,
+                                  o.beta, o.block);
                 const Plan& p = job.p;
-                for (int c = 0; c < p.ch; ++c)
-                    for (long long blk = 0; blk < p.nblk; ++blk) {
-                        t.push_back(job.in_off + c * p.xstride + blk * (long long)p.hop * p.Q);
-                        t.push_back(job.out_off + c * p.nout);
-                        t.push_back(blk);
-                        t.push_back(p.nout);
-                    }
+                job.in_off = b->in_total;
+                job.in_len = p.equal ? p.nin * p.ch : p.ch * p.xstride;
+                b->in_total += align(job.in_len);
+                job.out_off = b->out_total;
+                job.out_len = p.ch * p.nout;
+                b->out_total += align(job.out_len);
+                if (p.equal) continue;
+                int g = 0;   // jobs with the same rates and block size share a group
+                for (; g < (int)b->groups.size(); ++g) {
+                    const Plan& q = b->groups[g].p;
+                    if (q.fi == p.fi && q.fo == p.fo && q.nb == p.nb) break;
+                }
+                if (g == (int)b->groups.size()) {
+                    b->groups.emplace_back();
+                    b->groups.back().p = p;
+                }
+                job.group = g;
+                b->groups[g].jobs.push_back(j);
+                b->groups[g].ntask += (long long)p.ch * p.nblk;
             }
-            G.tdesc.upload(t.data(), t.size() * sizeof(long long), s);
-            const Design d = design_kernel(G.p, o, k, s);
-            G.eng = std::make_unique<Engine>(G.p, d.H.d(), d.nH, d.Jg, s, G.ntask,
-                                             reinterpret_cast<const long long*>(G.tdesc.d()), per_group);
-        }
-        // one launch per step over every group: opt-in (KBSRC_BATCH_FUSE=1). Measured slower than the per-group
-        // specialised launches on four streams for every batch tried (256 jobs of 0.5-2 s: 14.06 against 9.62 ms;
-        // 16 such jobs: 1.01 against 0.85 ms; 32 jobs of 10-30 s: 19.17 against 13.25 ms): the generic kernel's
-        // runtime stage list costs more than the launches it saves.
-        {
-            const char* v = std::getenv("KBSRC_BATCH_FUSE");
-            b->fuse = v && v[0] == '1';
-            for (kbsrc_batch::Group& G : b->groups) {   // every group must fit the generic-kernel form
-                EngineSteps probe;
-                if (b->fuse && !G.eng->describe(nullptr, nullptr, probe)) b->fuse = false;
+            const int nstreams = (int)std::min<size_t>(4, std::max<size_t>(1, b->groups.size()));
+            {   // groups to streams, largest first onto the least-loaded stream (work ~ tasks x n log n of both FFTs)
+                std::vector<std::pair<double, int>> w;
+                for (int g = 0; g < (int)b->groups.size(); ++g) {
+                    const Plan& q = b->groups[g].p;
+                    const double nb = q.Nb, mb = q.Mb;
+                    w.push_back({(double)b->groups[g].ntask * (nb * std::log2(nb + 2) + mb * std::log2(mb + 2)), g});
+                }
+                std::sort(w.begin(), w.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+                std::vector<double> load(nstreams, 0.0);
+                for (const auto& [wt, g] : w) {
+                    const int i = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+                    b->groups[g].stream = i;
+                    load[i] += wt;
+                }
             }
-        }
-        b->design_seconds = now() - t0;
-        *batch = b.release();
+            for (int i = 0; i < nstreams; ++i) {
+                hipStream_t q = nullptr;
+                hip_check(hipStreamCreateWithFlags(&q, hipStreamNonBlocking), "stream");
+                b->streams.push_back(q);
+            }
+            for (int i = 0; i <= nstreams; ++i) {
+                hipEvent_t e = nullptr;
+                hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+                b->events.push_back(e);
+            }
+            hipStream_t s = b->streams[0];
+            size_t free_b = 0, total_b = 0;
+            if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
+            const size_t per_group = std::clamp<size_t>(free_b / (16 * std::max<size_t>(1, b->groups.size())),
+                                                        (size_t)256 << 20, (size_t)4 << 30);
+            DeviceConsts k;
+            k.init(s);
+            for (kbsrc_batch::Group& G : b->groups) {
+                // task table: (input offset of the block, output offset of the channel, block, channel length)
+                std::vector<long long> t;
+                t.reserve(4 * G.ntask);
+                for (int j : G.jobs) {
+                    const kbsrc_batch::Job& job = b->jobs[j];
+                    const Plan& p = job.p;
+                    for (int c = 0; c < p.ch; ++c)
+                        for (long long blk = 0; blk < p.nblk; ++blk) {
+                            t.push_back(job.in_off + c * p.xstride + blk * (long long)p.hop * p.Q);
+                            t.push_back(job.out_off + c * p.nout);
+                            t.push_back(blk);
+                            t.push_back(p.nout);
+                        }
+                }
+                G.tdesc.upload(t.data(), t.size() * sizeof(long long), s);
+                const Design d = design_kernel(G.p, o, k, s);
+                G.eng = std::make_unique<Engine>(G.p, d.H.d(), d.nH, d.Jg, s, G.ntask,
+                                                 reinterpret_cast<const long long*>(G.tdesc.d()), per_group);
+            }
+            // one launch per step over every group: opt-in (KBSRC_BATCH_FUSE=1). Measured slower than the per-group
+            // specialised launches on four streams for every batch tried (256 jobs of 0.5-2 s: 14.06 against 9.62 ms;
+            // 16 such jobs: 1.01 against 0.85 ms; 32 jobs of 10-30 s: 19.17 against 13.25 ms): the generic kernel's
+            // runtime stage list costs more than the launches it saves.
+            {
+                const char* v = std::getenv("KBSRC_BATCH_FUSE");
+                b->fuse = v && v[0] == '1';
+                for (kbsrc_batch::Group& G : b->groups) {   // every group must fit the generic-kernel form
+                    EngineSteps probe;
+                    if (b->fuse && !G.eng->describe(nullptr, nullptr, probe)) b->fuse = false;
+                }
+            }
+            b->design_seconds = now() - t0;
+            *batch = b.release();
+        });
     });
 }
 
@@ -755,27 +764,30 @@ int kbsrc_batch_stage(kbsrc_batch* b, int job, const double* interleaved, int on
         const Plan& p = J.p;
         if (p.nin > 0 && !interleaved) throw std::invalid_argument("bad input buffer");
         hipStream_t s = static_cast<hipStream_t>(stream);
-        NearestRounding rn(b->opt.precision != KBSRC_FAST);
-        if (p.equal) {   // a planar copy: channel c at in_off + c * frames
-            DevBuf tmp;
-            const double* src = interleaved;
-            if (!on_device && p.nin > 0) {
-                tmp.upload(interleaved, sizeof(double) * (size_t)(p.nin * p.ch), s);
-                src = tmp.d();
+        in_nearest_mode(b->opt.precision != KBSRC_FAST, [&] {
+            if (p.equal) {   // a planar copy: channel c at in_off + c * frames
+                DevBuf tmp;
+                const double* src = interleaved;
+                if (!on_device && p.nin > 0) {
+                    tmp.upload(interleaved, sizeof(double) * (size_t)(p.nin * p.ch), s);
+                    src = tmp.d();
+                }
+                if (p.nin > 0) {
+                    StageArgs a;
+                    a.in = src; a.head = src; a.tail = src; a.xx = d_in + J.in_off;
+                    a.nin = p.nin; a.tot = p.nin; a.stride = p.nin; a.off = 0; a.ext = 0; a.ch = p.ch;
+                    hip_check(launch_stage(a, s), "staging kernel");
+                    hip_check(hipStreamSynchronize(s), "staging sync");
+                }
+                return;
             }
-            if (p.nin > 0) {
-                StageArgs a;
-                a.in = src; a.head = src; a.tail = src; a.xx = d_in + J.in_off;
-                a.nin = p.nin; a.tot = p.nin; a.stride = p.nin; a.off = 0; a.ext = 0; a.ch = p.ch;
-                hip_check(launch_stage(a, s), "staging kernel");
-                hip_check(hipStreamSynchronize(s), "staging sync");
-            }
-            return;
-        }
-        if (on_device) stage_device(p, b->opt.lpc_edges, interleaved, d_in + J.in_off, s);
-        else stage(p, b->opt.lpc_edges, interleaved, d_in + J.in_off, s);
+            if (on_device) stage_device(p, b->opt.lpc_edges, interleaved, d_in + J.in_off, s);
+            else stage(p, b->opt.lpc_edges, interleaved, d_in + J.in_off, s);
+        });
     });
 }
+
+}  // extern "C"
 
 namespace {
 
@@ -906,6 +918,8 @@ kbsrc_graph* capture(const std::function<void(hipStream_t)>& run) {
 
 }  // namespace
 
+extern "C" {
+
 int kbsrc_batch_execute(kbsrc_batch* b, const double* d_in, double* d_out, void* stream) {
     return guarded([&] {
         if (!b || !d_in || !d_out) throw std::invalid_argument("bad input buffer");
@@ -969,8 +983,10 @@ int kbsrc_tuned_shapes(void) { return tuned_shapes(); }
 int kbsrc_plain_dft16(const double* d_x, long long nvec, double* d_y, void* stream) {
     return guarded([&] {
         if (nvec < 0 || (nvec > 0 && (!d_x || !d_y))) throw std::invalid_argument("bad input buffer");
-        NearestRounding rn(true);   // the stage tables are made in host dd arithmetic on first use
-        plain_dft16(d_x, nvec, d_y, static_cast<hipStream_t>(stream));
+        // the stage tables are made in host dd arithmetic on first use
+        in_nearest_mode(true, [&] {
+            plain_dft16(d_x, nvec, d_y, static_cast<hipStream_t>(stream));
+        });
     });
 }
 
@@ -979,19 +995,22 @@ int kbsrc_split_dft16(const double* d_x, long long nvec, int slices, double* d_y
         if (nvec < 0 || (nvec > 0 && (!d_x || !d_y))) throw std::invalid_argument("bad input buffer");
         const int S = slices;
         if (!split_slices_ok(S)) throw std::invalid_argument("option out of range");
-        NearestRounding rn(true);
-        hipStream_t s = static_cast<hipStream_t>(stream);
-        hip_check(launch_split_dft16(d_x, nvec, S, split_dft16_digits(S, s), d_y, s), "split radix-16");
-        hip_check(hipStreamSynchronize(s), "split radix-16 sync");
+        in_nearest_mode(true, [&] {
+            hipStream_t s = static_cast<hipStream_t>(stream);
+            hip_check(launch_split_dft16(d_x, nvec, S, split_dft16_digits(S, s), d_y, s), "split radix-16");
+            hip_check(hipStreamSynchronize(s), "split radix-16 sync");
+        });
     });
 }
 
 int kbsrc_kernel_taps(int L, long long a, long long b, double beta, double* out) {
     return guarded([&] {
         if (L < 1 || a < 1 || b < 1 || !(beta > 0) || !out) throw std::invalid_argument("option out of range");
-        NearestRounding rn(true);
-        const DevBuf h = kernel_taps_gpu(L, a, b, beta, nullptr);
-        hip_check(hipMemcpy(out, h.d(), sizeof(double) * 2 * ((size_t)L + 1), hipMemcpyDeviceToHost), "taps download");
+        in_nearest_mode(true, [&] {
+            const DevBuf h = kernel_taps_gpu(L, a, b, beta, nullptr);
+            hip_check(hipMemcpy(out, h.d(), sizeof(double) * 2 * ((size_t)L + 1), hipMemcpyDeviceToHost),
+                      "taps download");
+        });
     });
 }
 
@@ -999,14 +1018,15 @@ int kbsrc_kernel_spectrum(const double* taps, int Lk, int n, double scale, doubl
     return guarded([&] {
         if (!taps || !out || Lk < 0 || n < 2 || (n & 1) || Lk >= n || !(scale != 0))
             throw std::invalid_argument("option out of range");
-        NearestRounding rn(true);
-        DevBuf h;
-        h.upload(taps, sizeof(double) * 2 * ((size_t)Lk + 1), nullptr);
-        DeviceConsts k;
-        k.init(nullptr);
-        const DevBuf H = kernel_spectrum_gpu(h.d(), Lk, n, scale, k, nullptr);
-        hip_check(hipMemcpy(out, H.d(), sizeof(double) * 2 * ((size_t)n / 2 + 1), hipMemcpyDeviceToHost),
-                  "spectrum download");
+        in_nearest_mode(true, [&] {
+            DevBuf h;
+            h.upload(taps, sizeof(double) * 2 * ((size_t)Lk + 1), nullptr);
+            DeviceConsts k;
+            k.init(nullptr);
+            const DevBuf H = kernel_spectrum_gpu(h.d(), Lk, n, scale, k, nullptr);
+            hip_check(hipMemcpy(out, H.d(), sizeof(double) * 2 * ((size_t)n / 2 + 1), hipMemcpyDeviceToHost),
+                      "spectrum download");
+        });
     });
 }
 

@@ -1,8 +1,11 @@
-// kbsrc command-line tool over the GPU engine. Flags, messages and exit codes follow the reference CLI
-// (app/main.cpp:84-165): 0 on success, 1 for a rejected conversion (no output file), 2 for usage errors.
+// kbsrc command-line tool over the GPU engine. The flags, the messages and the exit codes are the reference CLI's
+// (app/main.cpp:84-165): 0 on success, 1 for a rejected conversion (no output file is left), 2 for a usage error.
+#include <cerrno>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <exception>
 #include <new>
 #include <stdexcept>
@@ -19,32 +22,41 @@
 
 namespace {
 
-struct UsageError : std::exception {
-    std::string msg;
-    explicit UsageError(std::string m) : msg(std::move(m)) {}
-    const char* what() const noexcept override { return msg.c_str(); }
+// A command-line mistake: reported with a pointer to --help and exit status 2.
+class BadUsage : public std::runtime_error {
+public:
+    using std::runtime_error::runtime_error;
 };
 
-double now() { return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count(); }
+class Stopwatch {
+public:
+    double seconds() const { return std::chrono::duration<double>(Clock::now() - t0_).count(); }
 
-int parse_int(const std::string& v, const char* name) {
-    try {
-        size_t used = 0;
-        const int r = std::stoi(v, &used);
-        if (used == v.size()) return r;
-    } catch (...) {
-    }
-    throw UsageError(std::string("invalid value for ") + name + ": " + v);
+private:
+    using Clock = std::chrono::steady_clock;
+    Clock::time_point t0_ = Clock::now();
+};
+
+// Option values are taken whole: leading blanks are allowed, anything after the number, an int out of range, a real
+// that overflows or underflows, or a non-finite real makes the value invalid (std::stoi / std::stod acceptance).
+[[noreturn]] void bad_value(const char* flag, const std::string& v) {
+    throw BadUsage(std::string("invalid value for ") + flag + ": " + v);
 }
-
-double parse_double(const std::string& v, const char* name) {
-    try {
-        size_t used = 0;
-        const double r = std::stod(v, &used);
-        if (used == v.size() && std::isfinite(r)) return r;
-    } catch (...) {
-    }
-    throw UsageError(std::string("invalid value for ") + name + ": " + v);
+int whole_int(const std::string& v, const char* flag) {
+    const char* s = v.c_str();
+    char* end = nullptr;
+    errno = 0;
+    const long x = std::strtol(s, &end, 10);
+    if (end == s || *end != '\0' || errno == ERANGE || x < INT_MIN || x > INT_MAX) bad_value(flag, v);
+    return (int)x;
+}
+double whole_real(const std::string& v, const char* flag) {
+    const char* s = v.c_str();
+    char* end = nullptr;
+    errno = 0;
+    const double x = std::strtod(s, &end);
+    if (end == s || *end != '\0' || errno == ERANGE || !std::isfinite(x)) bad_value(flag, v);
+    return x;
 }
 
 // ---- streamed conversion: the file is decoded and uploaded while the GPU designs the kernel, and the output is
@@ -120,7 +132,7 @@ struct Times {
 void convert_streamed(kbsrc::io::WavReader& r, const char* out_path, int rate, const std::string& format,
                       bool dither, const kbsrc_options& o, kbsrc_stats& st, long long& nout, long long& clipped,
                       Times& t) {
-    const double t0 = now();
+    const Stopwatch clock;
     const int ch = r.channels();
     const long long frames = r.frames();
     bool early_ok = r.rate() <= 10000000;
@@ -167,7 +179,7 @@ void convert_streamed(kbsrc::io::WavReader& r, const char* out_path, int rate, c
         }
         hip_ok(hipStreamSynchronize(cs.s), "input upload");
     }
-    t.read = now() - t0;
+    t.read = clock.seconds();
     if (r.rate() > 10000000) throw std::runtime_error("input sample rate above 10000000 Hz");
     kbsrc::io::check_wav_output(ch, rate, format);
     design.join();
@@ -182,7 +194,7 @@ void convert_streamed(kbsrc::io::WavReader& r, const char* out_path, int rate, c
     kb_ok(kbsrc_plan_query(plan.p, v));
     nout = v[2];
     const long long xstride = v[22];
-    const double t1 = now();
+    const double t1 = clock.seconds();
     Stream s;
     DevMem d_out((size_t)ch * nout * sizeof(double));
     {
@@ -192,15 +204,13 @@ void convert_streamed(kbsrc::io::WavReader& r, const char* out_path, int rate, c
         kb_ok(kbsrc_plan_execute(plan.p, static_cast<const double*>(d_pad.p), static_cast<double*>(d_out.p), s.s));
         hip_ok(hipStreamSynchronize(s.s), "engine");
     }
-    const double t2 = now();
+    const double t2 = clock.seconds();
     t.resample = t2 - t1;
     clipped = 0;
     if (dither) {   // TPDF dither keeps the reference's mt19937_64 sequence: quantised on the host
         std::vector<double> flat((size_t)ch * nout);
         hip_ok(hipMemcpy(flat.data(), d_out.p, sizeof(double) * flat.size(), hipMemcpyDeviceToHost), "download");
-        std::vector<std::vector<double>> y(ch);
-        for (int c = 0; c < ch; ++c) y[c].assign(flat.begin() + (size_t)c * nout, flat.begin() + (size_t)(c + 1) * nout);
-        kbsrc::io::write_wav(out_path, y, ch, nout, rate, format, true, clipped);
+        kbsrc::io::write_wav(out_path, flat.data(), ch, nout, rate, format, true, clipped);
     } else {
         kbsrc::io::WavWriter w(out_path, ch, nout, rate, format);
         const size_t fb = w.frame_bytes();
@@ -235,70 +245,119 @@ void convert_streamed(kbsrc::io::WavReader& r, const char* out_path, int rate, c
         clipped = (long long)c;
         w.finish();
     }
-    t.write = now() - t2;
+    t.write = clock.seconds() - t2;
 }
 
-void help() {
-    std::printf(
-        "kbsrc %s - Kaiser-Bessel sample-rate converter (MI355X engine)\n"
-        "\n"
-        "usage: kbsrc [options] input.wav output.wav\n"
-        "\n"
-        "  -r, --rate N        output sample rate in Hz (default 44100)\n"
-        "  -m, --mode M        exact (default) or fast\n"
-        "  -f, --format F      f64 (default), f32, s32, s24 or s16\n"
-        "      --dither        TPDF dither for integer formats\n"
-        "  -t, --threads N     accepted for compatibility (the GPU engine has no host workers)\n"
-        "      --len S         one-sided kernel length in seconds (default 0.2)\n"
-        "      --beta B        Kaiser beta (default 40)\n"
-        "      --block N       block size factor (default: automatic)\n"
-        "      --no-lpc        zero-pad edges instead of LPC extrapolation\n"
-        "  -v, --verbose       print plan and timing\n"
-        "  -h, --help          show this help\n"
-        "      --version       show version\n",
-        kbsrc::version());
-}
-
-int run(int argc, char** argv) {
+// ---- the command line
+struct Request {
     kbsrc::Options opt;
     int rate = 44100;
     std::string format = "f64", in_path, out_path;
     bool dither = false, verbose = false;
-    for (int i = 1; i < argc; ++i) {
-        const std::string a = argv[i];
-        auto value = [&]() -> std::string {
-            if (i + 1 >= argc) throw UsageError("missing value for " + a);
-            return argv[++i];
-        };
-        if (a == "-r" || a == "--rate") rate = parse_int(value(), "--rate");
-        else if (a == "-m" || a == "--mode") {
-            const std::string m = value();
-            if (m == "exact") opt.precision = kbsrc::Precision::Exact;
-            else if (m == "fast") opt.precision = kbsrc::Precision::Fast;
-            else if (m == "ld") opt.precision = kbsrc::Precision::LongDouble;
-            else throw UsageError("mode must be exact, fast or ld");
-        } else if (a == "-f" || a == "--format") format = value();
-        else if (a == "-t" || a == "--threads") opt.threads = parse_int(value(), "--threads");
-        else if (a == "--block") opt.block = parse_int(value(), "--block");
-        else if (a == "--len") opt.length = parse_double(value(), "--len");
-        else if (a == "--beta") opt.beta = parse_double(value(), "--beta");
-        else if (a == "--dither") dither = true;
-        else if (a == "--no-lpc") opt.lpc_edges = false;
-        else if (a == "-v" || a == "--verbose") verbose = true;
-        else if (a == "-h" || a == "--help") { help(); return 0; }
-        else if (a == "--version") { std::printf("kbsrc %s\n", kbsrc::version()); return 0; }
-        else if (a.size() > 1 && a[0] == '-') throw UsageError("unknown option " + a);
-        else if (in_path.empty()) in_path = a;
-        else if (out_path.empty()) out_path = a;
-        else throw UsageError("too many arguments");
-    }
-    if (in_path.empty() || out_path.empty()) throw UsageError("expected input and output paths");
-    if (rate < 1 || rate > 10000000) throw UsageError("--rate must be between 1 and 10000000");
-    if (format != "f64" && format != "f32" && format != "s32" && format != "s24" && format != "s16")
-        throw UsageError("format must be f64, f32, s32, s24 or s16");
+    enum { Convert, ShowHelp, ShowVersion } action = Convert;
+};
 
-    const double t0 = now();
-    kbsrc::io::WavReader r(in_path.c_str());
+// One flag: its short and long spellings, whether it takes a value, and what it does with it.
+struct Flag {
+    const char* brief;   // "" when there is none
+    const char* full;
+    bool takes_value;
+    void (*apply)(Request&, const std::string& value);
+};
+
+const Flag kFlags[] = {
+    {"-r", "--rate", true, [](Request& q, const std::string& v) { q.rate = whole_int(v, "--rate"); }},
+    {"-m", "--mode", true,
+     [](Request& q, const std::string& v) {
+         if (v == "exact") q.opt.precision = kbsrc::Precision::Exact;
+         else if (v == "fast") q.opt.precision = kbsrc::Precision::Fast;
+         else if (v == "ld") q.opt.precision = kbsrc::Precision::LongDouble;   // refused later, as by the library
+         else throw BadUsage("mode must be exact, fast or ld");
+     }},
+    {"-f", "--format", true, [](Request& q, const std::string& v) { q.format = v; }},
+    {"-t", "--threads", true, [](Request& q, const std::string& v) { q.opt.threads = whole_int(v, "--threads"); }},
+    {"", "--block", true, [](Request& q, const std::string& v) { q.opt.block = whole_int(v, "--block"); }},
+    {"", "--len", true, [](Request& q, const std::string& v) { q.opt.length = whole_real(v, "--len"); }},
+    {"", "--beta", true, [](Request& q, const std::string& v) { q.opt.beta = whole_real(v, "--beta"); }},
+    {"", "--dither", false, [](Request& q, const std::string&) { q.dither = true; }},
+    {"", "--no-lpc", false, [](Request& q, const std::string&) { q.opt.lpc_edges = false; }},
+    {"-v", "--verbose", false, [](Request& q, const std::string&) { q.verbose = true; }},
+    {"-h", "--help", false, [](Request& q, const std::string&) { q.action = Request::ShowHelp; }},
+    {"", "--version", false, [](Request& q, const std::string&) { q.action = Request::ShowVersion; }},
+};
+
+const Flag* find_flag(const std::string& a) {
+    for (const Flag& f : kFlags)
+        if (a == f.full || (f.brief[0] && a == f.brief)) return &f;
+    return nullptr;
+}
+
+// Arguments are taken left to right; help and version act as soon as they are met.
+Request parse(int argc, char** argv) {
+    Request q;
+    std::vector<std::string> paths;
+    for (int i = 1; i < argc && q.action == Request::Convert; ++i) {
+        const std::string a = argv[i];
+        if (const Flag* f = find_flag(a)) {
+            if (f->takes_value && i + 1 >= argc) throw BadUsage("missing value for " + a);
+            f->apply(q, f->takes_value ? std::string(argv[++i]) : std::string());
+        } else if (a.size() > 1 && a[0] == '-') {
+            throw BadUsage("unknown option " + a);
+        } else if (a.empty() && paths.size() < 2) {
+            // an empty argument names no file and fills no place
+        } else if (paths.size() == 2) {
+            throw BadUsage("too many arguments");
+        } else {
+            paths.push_back(a);
+        }
+    }
+    if (q.action != Request::Convert) return q;
+    if (paths.size() < 2) throw BadUsage("expected input and output paths");
+    q.in_path = paths[0];
+    q.out_path = paths[1];
+    if (q.rate < 1 || q.rate > 10000000) throw BadUsage("--rate must be between 1 and 10000000");
+    if (!kbsrc::io::find_format(q.format)) throw BadUsage("format must be f64, f32, s32, s24 or s16");
+    return q;
+}
+
+void print_usage() {
+    std::printf(
+        "kbsrc %s (MI355X engine)\n"
+        "Converts the sample rate of a WAV file with a Kaiser-windowed sinc kernel.\n"
+        "\n"
+        "usage: kbsrc [options] input.wav output.wav\n"
+        "\n"
+        "options:\n"
+        "  -r, --rate N      rate of the output file, Hz; default 44100\n"
+        "  -m, --mode M      exact (double-double, the default) or fast (float64)\n"
+        "  -f, --format F    output samples: f64 (default), f32, s32, s24 or s16\n"
+        "      --dither      add TPDF dither before rounding to an integer format\n"
+        "  -t, --threads N   accepted; the GPU engine uses no host worker threads\n"
+        "      --len S       kernel half-length in seconds; default 0.2\n"
+        "      --beta B      Kaiser window beta; default 40\n"
+        "      --block N     block size factor; default chosen from the rates\n"
+        "      --no-lpc      pad the signal edges with zeros instead of Burg LPC extension\n"
+        "  -v, --verbose     print the plan and the time of each step\n"
+        "  -h, --help        print this text\n"
+        "      --version     print the version\n",
+        kbsrc::version());
+}
+
+int run(int argc, char** argv) {
+    const Request q = parse(argc, argv);
+    if (q.action == Request::ShowHelp) {
+        print_usage();
+        return 0;
+    }
+    if (q.action == Request::ShowVersion) {
+        std::printf("kbsrc %s\n", kbsrc::version());
+        return 0;
+    }
+    const kbsrc::Options& opt = q.opt;
+    const int rate = q.rate;
+    const std::string& format = q.format;
+    const Stopwatch clock;
+    kbsrc::io::WavReader r(q.in_path.c_str());
     long long nout = 0, clipped = 0;
     Times t;
     kbsrc_stats st{};
@@ -311,28 +370,30 @@ int run(int argc, char** argv) {
         o.threads = opt.threads;
         o.block = opt.block;
         o.lpc_edges = opt.lpc_edges ? 1 : 0;
-        convert_streamed(r, out_path.c_str(), rate, format, dither, o, st, nout, clipped, t);
-    } else {   // equal rates (a copy) and the rejected long-double mode: the library call on the whole file
+        convert_streamed(r, q.out_path.c_str(), rate, format, q.dither, o, st, nout, clipped, t);
+    } else {   // equal rates (a copy) and the refused long-double mode: the library call on the whole file
         std::vector<double> d((size_t)r.frames() * r.channels());
         r.read(d.data(), r.frames());
-        t.read = now() - t0;
+        t.read = clock.seconds();
         if (r.rate() > 10000000) throw std::runtime_error("input sample rate above 10000000 Hz");
         kbsrc::io::check_wav_output(r.channels(), rate, format);
-        const double t1 = now();
+        const double t1 = clock.seconds();
         kbsrc::Stats s;
-        auto y = kbsrc::resample(d.data(), r.frames(), r.channels(), r.rate(), rate, opt, &s);
-        const double t2 = now();
+        const auto y = kbsrc::resample(d.data(), r.frames(), r.channels(), r.rate(), rate, opt, &s);
+        const double t2 = clock.seconds();
         nout = y.empty() ? 0 : (long long)y[0].size();
-        kbsrc::io::write_wav(out_path.c_str(), y, r.channels(), nout, rate, format, dither, clipped);
+        std::vector<double> planar;
+        planar.reserve((size_t)r.channels() * (size_t)nout);
+        for (const auto& c : y) planar.insert(planar.end(), c.begin(), c.end());
+        kbsrc::io::write_wav(q.out_path.c_str(), planar.data(), r.channels(), nout, rate, format, q.dither, clipped);
         t.design = s.design_seconds;
         t.resample = t2 - t1 - s.design_seconds;
-        t.write = now() - t2;
+        t.write = clock.seconds() - t2;
         st.p = s.p; st.q = s.q; st.taps = s.taps; st.block = s.block; st.threads = s.threads;
         st.general_fold = s.general_fold; st.passband = s.passband; st.stopband = s.stopband; st.isa = s.isa;
     }
-    const double t3 = now();
     if (clipped) std::fprintf(stderr, "kbsrc: warning: %lld samples clipped\n", clipped);
-    if (verbose) {
+    if (q.verbose) {
         std::fprintf(stderr, "%d Hz -> %d Hz, %d ch, %lld -> %lld frames\n", r.rate(), rate, r.channels(),
                      r.frames(), nout);
         std::fprintf(stderr, "P/Q=%lld/%lld taps=%lld block=%d threads=%d engine=%s%s\n", st.p, st.q, st.taps,
@@ -341,9 +402,24 @@ int run(int argc, char** argv) {
             std::fprintf(stderr, "passband 0-%.1f Hz, stopband from %.1f Hz\n", st.passband, st.stopband);
         // the design runs on the GPU while the file is read: read and design overlap
         std::fprintf(stderr, "read %.3fs  design %.3fs  resample %.3fs  write %.3fs  total %.3fs\n", t.read,
-                     t.design, t.resample, t.write, t3 - t0);
+                     t.design, t.resample, t.write, clock.seconds());
     }
     return 0;
+}
+
+// Exit status and message of whatever ended the run: usage mistakes 2, every other failure 1.
+int report(std::exception_ptr failure) {
+    try {
+        std::rethrow_exception(failure);
+    } catch (const BadUsage& e) {
+        std::fprintf(stderr, "kbsrc: %s (see --help)\n", e.what());
+        return 2;
+    } catch (const std::bad_alloc&) {
+        std::fprintf(stderr, "kbsrc: out of memory\n");
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "kbsrc: %s\n", e.what());
+    }
+    return 1;
 }
 
 }  // namespace
@@ -351,14 +427,7 @@ int run(int argc, char** argv) {
 int main(int argc, char** argv) {
     try {
         return run(argc, argv);
-    } catch (const UsageError& e) {
-        std::fprintf(stderr, "kbsrc: %s (see --help)\n", e.what());
-        return 2;
-    } catch (const std::bad_alloc&) {
-        std::fprintf(stderr, "kbsrc: out of memory\n");
-        return 1;
-    } catch (const std::exception& e) {
-        std::fprintf(stderr, "kbsrc: %s\n", e.what());
-        return 1;
+    } catch (...) {
+        return report(std::current_exception());
     }
 }

@@ -1,12 +1,11 @@
-// WAV I/O (host). Header layout, decoding and quantisation follow the reference byte for byte
-// (src/io/wav.cpp:134-278): the undithered s16/s24/s32 output must be bit-identical.
+// WAV reading and writing for the command-line tool. The bytes written and the inputs refused (with their messages,
+// checked in the same order) are the reference's (src/io/wav.cpp:109-253); undithered s16/s24/s32 and f32/f64 files
+// are byte-identical to its output.
 #include "wav.hpp"
 
 #include <algorithm>
 #include <climits>
 #include <cmath>
-#include <cstdint>
-#include <cstdio>
 #include <cstring>
 #include <random>
 #include <stdexcept>
@@ -18,233 +17,299 @@ namespace {
 
 [[noreturn]] void fail(const char* m) { throw std::runtime_error(m); }
 
-struct FileCloser {
-    FILE* f = nullptr;
-    ~FileCloser() {
-        if (f) fclose(f);
+constexpr SampleFormat kFormats[] = {
+    {"f64", 64, true}, {"f32", 32, true}, {"s32", 32, false}, {"s24", 24, false}, {"s16", 16, false},
+};
+
+// little-endian field of n bytes at p
+uint64_t le(const uint8_t* p, int n) {
+    uint64_t v = 0;
+    while (n-- > 0) v = v << 8 | p[n];
+    return v;
+}
+
+long long file_size(std::FILE* f) {
+    fseeko(f, 0, SEEK_END);
+    const long long n = ftello(f);
+    fseeko(f, 0, SEEK_SET);
+    return n;
+}
+
+// ---- decoders: n samples of one encoding to float64. Integer PCM is scaled by 2^-(8B-1) (8-bit PCM is unsigned
+// around 128), which is exact.
+template <int B> void decode_pcm(const uint8_t* src, size_t n, double* dst) {
+    constexpr double unit = 1.0 / (double)(1ull << (8 * B - 1));
+    for (size_t k = 0; k < n; ++k, src += B) {
+        const uint32_t u = (uint32_t)le(src, B);
+        const int32_t v = B == 1 ? (int32_t)u - 128 : (int32_t)(u << (32 - 8 * B)) >> (32 - 8 * B);
+        dst[k] = v * unit;
+    }
+}
+void decode_f32(const uint8_t* src, size_t n, double* dst) {
+    for (size_t k = 0; k < n; ++k) {
+        float v;
+        std::memcpy(&v, src + 4 * k, 4);
+        dst[k] = v;
+    }
+}
+void decode_f64(const uint8_t* src, size_t n, double* dst) { std::memcpy(dst, src, 8 * n); }
+
+// What the fmt chunk says (the last one before the data chunk counts).
+struct FmtFields {
+    bool seen = false;
+    unsigned tag = 0, channels = 0, block_align = 0, bits = 0;
+    uint32_t rate = 0;
+};
+
+// Walks the RIFF chunks after the 12-byte RIFF/WAVE header: fmt fields, and the data chunk's offset and length (a
+// declared length of 0 or 0xFFFFFFFF means "to the end of the file", as a streamed writer leaves it).
+struct Layout {
+    FmtFields fmt;
+    long long data_at = -1, data_len = 0;
+};
+Layout walk_chunks(std::FILE* f, long long size) {
+    Layout L;
+    for (long long at = 12; at + 8 <= size;) {
+        uint8_t head[8];
+        fseeko(f, at, SEEK_SET);
+        if (std::fread(head, 1, 8, f) != 8) break;
+        const uint32_t len = (uint32_t)le(head + 4, 4);
+        const long long body = at + 8;
+        if (std::memcmp(head, "fmt ", 4) == 0) {
+            uint8_t b[40] = {};
+            const size_t got = std::fread(b, 1, std::min<uint32_t>(len, sizeof b), f);
+            if (got < 16) fail("bad fmt chunk");
+            FmtFields& m = L.fmt;
+            m.seen = true;
+            m.tag = (unsigned)le(b, 2);
+            m.channels = (unsigned)le(b + 2, 2);
+            m.rate = (uint32_t)le(b + 4, 4);
+            m.block_align = (unsigned)le(b + 12, 2);
+            m.bits = (unsigned)le(b + 14, 2);
+            if (m.tag == 0xFFFE && got >= 26) m.tag = (unsigned)le(b + 24, 2);   // the sub-format's tag
+        } else if (std::memcmp(head, "data", 4) == 0) {
+            L.data_at = body;
+            L.data_len = (len == 0 || len == 0xFFFFFFFFu) ? size - body : std::min<long long>(len, size - body);
+            break;
+        }
+        at = body + len + (len & 1);
+    }
+    return L;
+}
+
+// ---- header of an output file, assembled chunk by chunk
+class Bytes {
+public:
+    Bytes& u16(unsigned v) { return put(v, 2); }
+    Bytes& u32(uint32_t v) { return put(v, 4); }
+    Bytes& id(const char* four) {
+        b_.insert(b_.end(), four, four + 4);
+        return *this;
+    }
+    Bytes& raw(const uint8_t* p, size_t n) {
+        b_.insert(b_.end(), p, p + n);
+        return *this;
+    }
+    Bytes& chunk(const char* four, const Bytes& payload) {
+        id(four).u32((uint32_t)payload.size());
+        return raw(payload.b_.data(), payload.size());
+    }
+    size_t size() const { return b_.size(); }
+    const std::vector<uint8_t>& bytes() const { return b_; }
+
+private:
+    Bytes& put(uint64_t v, int n) {
+        for (int i = 0; i < n; ++i) b_.push_back((uint8_t)(v >> (8 * i)));
+        return *this;
+    }
+    std::vector<uint8_t> b_;
+};
+
+// KSDATAFORMAT_SUBTYPE_{PCM,IEEE_FLOAT}: the format tag in the first two bytes, then this fixed GUID tail
+constexpr uint8_t kSubtypeTail[14] = {0x00, 0x00, 0x00, 0x00, 0x10, 0x00, 0x80,
+                                      0x00, 0x00, 0xAA, 0x00, 0x38, 0x9B, 0x71};
+
+std::vector<uint8_t> header(const SampleFormat& f, int ch, int rate, long long frames) {
+    const unsigned align = (unsigned)ch * f.bytes();
+    const uint32_t data = (uint32_t)((unsigned long long)frames * align);
+    // WAVE_FORMAT_EXTENSIBLE for more than two channels and for integer samples wider than 16 bits; otherwise the
+    // 16-byte PCM fmt, or the 18-byte float fmt with its fact chunk
+    const bool ext = ch > 2 || (!f.floating && f.bits > 16);
+    Bytes fmt;
+    fmt.u16(ext ? 0xFFFE : f.floating ? 3 : 1).u16(ch).u32(rate).u32((uint32_t)rate * align).u16(align).u16(f.bits);
+    if (ext) {
+        const uint32_t mask = ch == 1 ? 0x4 : ch == 2 ? 0x3 : 0;   // front centre; front left and right
+        fmt.u16(22).u16(f.bits).u32(mask).u16(f.floating ? 3 : 1).raw(kSubtypeTail, sizeof kSubtypeTail);
+    } else if (f.floating) {
+        fmt.u16(0);
+    }
+    Bytes body;
+    body.id("WAVE").chunk("fmt ", fmt);
+    if (f.floating) {
+        Bytes fact;
+        fact.u32((uint32_t)frames);
+        body.chunk("fact", fact);
+    }
+    body.id("data").u32(data);
+    Bytes file;
+    file.id("RIFF").u32((uint32_t)(body.size() + data + (data & 1))).raw(body.bytes().data(), body.size());
+    return file.bytes();
+}
+
+// One integer sample: round in the caller's mode, clip to [-2^(bits-1), 2^(bits-1) - 1], little-endian bytes.
+struct IntegerSample {
+    int bytes;
+    double full;   // 2^(bits-1)
+    void store(uint8_t* p, double v, long long& clipped) const {
+        v = std::nearbyint(v);
+        if (v > full - 1) {
+            v = full - 1;
+            ++clipped;
+        } else if (v < -full) {
+            v = -full;
+            ++clipped;
+        }
+        const uint64_t q = v == v ? (uint64_t)(int64_t)v : 0;   // NaN is written as zero
+        for (int i = 0; i < bytes; ++i) p[i] = (uint8_t)(q >> (8 * i));
     }
 };
 
-unsigned rd16(const uint8_t* p) { return (unsigned)p[0] | (unsigned)p[1] << 8; }
-uint32_t rd32(const uint8_t* p) { return (uint32_t)rd16(p) | (uint32_t)rd16(p + 2) << 16; }
-
-int bytes_of(const std::string& fmt) { return fmt == "f64" ? 8 : fmt == "s24" ? 3 : fmt == "s16" ? 2 : 4; }
-
-// One sample to a B-byte little-endian integer: scale, optional TPDF dither, round in the current mode, clip.
-template <int B> void quantise(uint8_t* p, double y, bool dither, std::mt19937_64& rng,
-                               std::uniform_real_distribution<double>& U, long long& clipped) {
-    constexpr double sc = B == 2 ? 32768.0 : B == 3 ? 8388608.0 : 2147483648.0;
-    double v = y * sc;
-    if (dither) {
-        const double d1 = U(rng);
-        const double d2 = U(rng);
-        v += d1 + d2;
-    }
-    v = std::nearbyint(v);
-    if (v > sc - 1) {
-        v = sc - 1;
-        ++clipped;
-    } else if (v < -sc) {
-        v = -sc;
-        ++clipped;
-    }
-    const int64_t iv = std::isnan(v) ? 0 : (int64_t)v;   // NaN is written as zero
-    for (int t = 0; t < B; ++t) p[t] = (uint8_t)((iv >> (8 * t)) & 255);
-}
-
 }  // namespace
 
+const SampleFormat* find_format(const std::string& name) {
+    for (const SampleFormat& f : kFormats)
+        if (name == f.name) return &f;
+    return nullptr;
+}
+
 WavReader::WavReader(const char* path) {
-    FileCloser fc{fopen(path, "rb")};
-    FILE* f = fc.f;
+    std::FILE* f = std::fopen(path, "rb");
     if (!f) fail("cannot open input");
-    fseeko(f, 0, SEEK_END);
-    const long long size = ftello(f);
-    fseeko(f, 0, SEEK_SET);
-    uint8_t riff[12];
-    if (size < 12 || fread(riff, 1, 12, f) != 12 || memcmp(riff, "RIFF", 4) || memcmp(riff + 8, "WAVE", 4))
-        fail("not a RIFF/WAVE file");
-    long long pos = 12, data_pos = -1, data_len = 0;
-    int tag = 0, ch = 0, bits = 0, align = 0;
-    uint32_t rate = 0;
-    bool have_fmt = false;
-    while (pos + 8 <= size) {
-        uint8_t hdr[8];
-        fseeko(f, pos, SEEK_SET);
-        if (fread(hdr, 1, 8, f) != 8) break;
-        const uint32_t len = rd32(hdr + 4);
-        const long long body = pos + 8;
-        if (!memcmp(hdr, "fmt ", 4)) {
-            uint8_t fm[40] = {};
-            const size_t got = fread(fm, 1, std::min<uint32_t>(len, 40), f);
-            if (got < 16) fail("bad fmt chunk");
-            have_fmt = true;
-            tag = (int)rd16(fm);
-            ch = (int)rd16(fm + 2);
-            rate = rd32(fm + 4);
-            align = (int)rd16(fm + 12);
-            bits = (int)rd16(fm + 14);
-            if (tag == 0xFFFE && got >= 26) tag = (int)rd16(fm + 24);   // EXTENSIBLE: sub-format GUID's tag
-        } else if (!memcmp(hdr, "data", 4)) {
-            data_pos = body;
-            data_len = std::min<long long>(len, size - body);
-            if (len == 0xFFFFFFFFu || len == 0) data_len = size - body;   // streamed files
-            break;
+    file_ = f;   // closed by the destructor, also when a check below throws
+    struct CloseOnThrow {
+        WavReader* r;
+        bool armed = true;
+        ~CloseOnThrow() {
+            if (armed && r->file_) {
+                std::fclose(r->file_);
+                r->file_ = nullptr;
+            }
         }
-        pos = body + len + (len & 1);
-    }
-    if (!have_fmt || data_pos < 0) fail("missing fmt/data chunk");
-    if (!ch) fail("invalid WAV channel count");
-    if (!rate || rate > INT_MAX) fail("invalid WAV sample rate");
-    if (align <= 0 || align % ch) fail("invalid WAV block alignment");
-    if (!(tag == 1 || (tag == 3 && (bits == 32 || bits == 64)))) fail("unsupported WAV format (need PCM or IEEE float)");
-    const int bps = align / ch;
-    if (tag == 1 && (bps < 1 || bps > 4 || (bits + 7) / 8 != bps)) fail("unsupported PCM width");
-    if (tag == 3 && bits != 8 * bps) fail("IEEE float sample size does not match the block alignment");
-    rate_ = (int)rate;
-    ch_ = ch;
-    tag_ = tag;
-    bits_ = bits;
-    bps_ = bps;
-    frames_ = data_len / align;
-    fseeko(f, data_pos, SEEK_SET);
-    buf_.resize(((size_t)1 << 20) * bps);
-    f_ = f;
-    fc.f = nullptr;
+    } guard{this};
+    const long long size = file_size(f);
+    uint8_t riff[12];
+    if (size < 12 || std::fread(riff, 1, 12, f) != 12 || std::memcmp(riff, "RIFF", 4) ||
+        std::memcmp(riff + 8, "WAVE", 4))
+        fail("not a RIFF/WAVE file");
+    const Layout L = walk_chunks(f, size);
+    const FmtFields& m = L.fmt;
+    if (!m.seen || L.data_at < 0) fail("missing fmt/data chunk");
+    if (m.channels == 0) fail("invalid WAV channel count");
+    if (m.rate == 0 || m.rate > (uint32_t)INT_MAX) fail("invalid WAV sample rate");
+    if (m.block_align == 0 || m.block_align % m.channels) fail("invalid WAV block alignment");
+    const bool pcm = m.tag == 1, ieee = m.tag == 3 && (m.bits == 32 || m.bits == 64);
+    if (!pcm && !ieee) fail("unsupported WAV format (need PCM or IEEE float)");
+    // a PCM sample takes the fewest whole bytes that hold its bits (12-bit audio in 2 bytes decodes as 16-bit); a
+    // float sample exactly 4 or 8
+    const int width = (int)(m.block_align / m.channels);
+    if (pcm && (width < 1 || width > 4 || (int)(m.bits + 7) / 8 != width)) fail("unsupported PCM width");
+    if (ieee && (int)m.bits != 8 * width) fail("IEEE float sample size does not match the block alignment");
+    static constexpr Decoder kPcm[4] = {decode_pcm<1>, decode_pcm<2>, decode_pcm<3>, decode_pcm<4>};
+    decode_ = ieee ? (width == 8 ? decode_f64 : decode_f32) : kPcm[width - 1];
+    rate_ = (int)m.rate;
+    ch_ = (int)m.channels;
+    sample_bytes_ = width;
+    frames_ = L.data_len / m.block_align;
+    fseeko(f, L.data_at, SEEK_SET);
+    guard.armed = false;
 }
 
 WavReader::~WavReader() {
-    if (f_) fclose(static_cast<FILE*>(f_));
+    if (file_) std::fclose(file_);
 }
 
 void WavReader::read(double* dst, long long frames) {
-    FILE* f = static_cast<FILE*>(f_);
-    if (frames < 0 || frames > frames_ - done_) fail("read error");
-    const size_t n = (size_t)frames * ch_, chunk = (size_t)1 << 20, bps = bps_;
-    for (size_t k0 = 0; k0 < n; k0 += chunk) {
-        const size_t m = std::min(chunk, n - k0);
-        if (fread(buf_.data(), bps, m, f) != m) fail("read error");
-        const uint8_t* s = buf_.data();
-        double* o = dst + k0;
-        if (tag_ == 3 && bits_ == 64) {
-            memcpy(o, s, 8 * m);
-        } else if (tag_ == 3) {
-            for (size_t k = 0; k < m; ++k) {
-                float v;
-                memcpy(&v, s + 4 * k, 4);
-                o[k] = v;
-            }
-        } else if (bps == 1) {
-            for (size_t k = 0; k < m; ++k) o[k] = ((int)s[k] - 128) / 128.0;
-        } else if (bps == 2) {
-            for (size_t k = 0; k < m; ++k) o[k] = (int16_t)(s[2 * k] | s[2 * k + 1] << 8) / 32768.0;
-        } else if (bps == 3) {
-            for (size_t k = 0; k < m; ++k) {
-                const uint8_t* p = s + 3 * k;
-                int32_t v = p[0] | p[1] << 8 | p[2] << 16;
-                if (v & 0x800000) v -= 1 << 24;
-                o[k] = v / 8388608.0;
-            }
-        } else {
-            for (size_t k = 0; k < m; ++k) {
-                const uint8_t* p = s + 4 * k;
-                o[k] = (int32_t)(p[0] | p[1] << 8 | p[2] << 16 | (uint32_t)p[3] << 24) / 2147483648.0;
-            }
-        }
+    if (frames < 0 || frames > frames_ - consumed_) fail("read error");
+    constexpr size_t kBlock = (size_t)1 << 18;   // samples decoded per file read
+    if (raw_.empty()) raw_.resize(kBlock * (size_t)sample_bytes_);
+    const size_t total = (size_t)frames * (size_t)ch_;
+    for (size_t done = 0; done < total;) {
+        const size_t n = std::min(kBlock, total - done);
+        if (std::fread(raw_.data(), (size_t)sample_bytes_, n, file_) != n) fail("read error");
+        decode_(raw_.data(), n, dst + done);
+        done += n;
     }
-    done_ += frames;
+    consumed_ += frames;
 }
 
-Audio read_wav(const char* path) {
-    WavReader r(path);
-    Audio a;
-    a.rate = r.rate();
-    a.ch = r.channels();
-    a.frames = r.frames();
-    a.d.resize((size_t)a.frames * a.ch);
-    r.read(a.d.data(), a.frames);
-    return a;
-}
-
-void check_wav_output(int ch, int rate, const std::string& fmt) {
-    const long long frame = (long long)ch * bytes_of(fmt);
+void check_wav_output(int ch, int rate, const std::string& format) {
+    const SampleFormat* f = find_format(format);
+    const long long frame = (long long)ch * (f ? f->bytes() : 4);
     if (ch < 1 || frame > 0xFFFF) fail("too many channels for a WAV file in this format");
     if (rate < 1 || rate * frame > 0xFFFFFFFFLL) fail("sample rate too high for a WAV file with this many channels");
 }
 
-WavWriter::WavWriter(const char* path, int ch, long long frames, int rate, const std::string& fmt) {
-    check_wav_output(ch, rate, fmt);
-    const bool is_float = fmt == "f64" || fmt == "f32";
-    const int bits = fmt == "f64" ? 64 : fmt == "f32" ? 32 : fmt == "s32" ? 32 : fmt == "s24" ? 24 : 16;
-    const int bps = bits / 8, align = ch * bps;
+WavWriter::WavWriter(const char* path, int ch, long long frames, int rate, const std::string& format) {
+    check_wav_output(ch, rate, format);
+    const SampleFormat* f = find_format(format);
+    if (!f) fail("unsupported output format");
+    const long long align = (long long)ch * f->bytes();
+    // the RIFF size field holds the whole file less 8 bytes, header included (at most 80 bytes)
     if (frames < 0 || frames > (0xFFFFFFFFLL - 80) / align) fail("output too large for WAV (4 GB limit)");
-    const size_t data_bytes = (size_t)frames * align;
-    std::vector<uint8_t> h;
-    auto put16 = [&](unsigned v) { h.push_back(v & 255); h.push_back(v >> 8 & 255); };
-    auto put32 = [&](uint32_t v) { put16(v & 0xFFFF); put16(v >> 16); };
-    auto tag4 = [&](const char* t) { h.insert(h.end(), t, t + 4); };
-    const bool extensible = ch > 2 || (!is_float && bits > 16);
-    const unsigned fmt_len = extensible ? 40 : (is_float ? 18 : 16);
-    const uint32_t riff_len = 4 + 8 + fmt_len + (is_float ? 12 : 0) + 8 + data_bytes + (data_bytes & 1);
-    tag4("RIFF"); put32(riff_len); tag4("WAVE"); tag4("fmt "); put32(fmt_len);
-    put16(extensible ? 0xFFFE : is_float ? 3 : 1); put16(ch); put32(rate); put32((uint32_t)rate * align);
-    put16(align); put16(bits);
-    if (fmt_len >= 18) put16(extensible ? 22 : 0);
-    if (extensible) {
-        put16(bits);
-        put32(ch == 1 ? 4 : ch == 2 ? 3 : 0);   // speaker mask: mono = FC, stereo = FL|FR
-        static const uint8_t kGuidTail[14] = {0x00, 0x00, 0x00, 0x00, 0x10, 0x00, 0x80,
-                                              0x00, 0x00, 0xAA, 0x00, 0x38, 0x9B, 0x71};
-        put16(is_float ? 3 : 1);
-        h.insert(h.end(), kGuidTail, kGuidTail + 14);
-    }
-    if (is_float) { tag4("fact"); put32(4); put32((uint32_t)frames); }
-    tag4("data"); put32((uint32_t)data_bytes);
-    FileCloser fc{fopen(path, "wb")};
-    if (!fc.f) fail("cannot open output");
-    if (fwrite(h.data(), 1, h.size(), fc.f) != h.size()) fail("write error");
-    align_ = (size_t)align;
-    data_bytes_ = data_bytes;
-    f_ = fc.f;
-    fc.f = nullptr;
+    const std::vector<uint8_t> h = header(*f, ch, rate, frames);
+    std::FILE* out = std::fopen(path, "wb");
+    if (!out) fail("cannot open output");
+    file_ = out;
+    frame_bytes_ = (size_t)align;
+    data_bytes_ = (unsigned long long)frames * (unsigned long long)align;
+    write(h.data(), h.size());
 }
 
 WavWriter::~WavWriter() {
-    if (f_) fclose(static_cast<FILE*>(f_));
+    if (file_) std::fclose(file_);
 }
 
 void WavWriter::write(const void* bytes, size_t n) {
-    if (n && fwrite(bytes, 1, n, static_cast<FILE*>(f_)) != n) fail("write error");
+    if (n && std::fwrite(bytes, 1, n, file_) != n) fail("write error");
 }
 
 void WavWriter::finish() {
-    FILE* f = static_cast<FILE*>(f_);
-    if (data_bytes_ & 1) fputc(0, f);
-    f_ = nullptr;
-    if (fclose(f)) fail("write error");
+    if (data_bytes_ & 1) std::fputc(0, file_);
+    std::FILE* f = file_;
+    file_ = nullptr;
+    if (std::fclose(f) != 0) fail("write error");
 }
 
-void write_wav(const char* path, const std::vector<std::vector<double>>& y, int ch, long long frames, int rate,
-               const std::string& fmt, bool dither, long long& clipped) {
-    WavWriter w(path, ch, frames, rate, fmt);
-    const size_t align = w.frame_bytes();
-    const int bps = (int)(align / ch);
+void write_wav(const char* path, const double* y, int ch, long long frames, int rate, const std::string& format,
+               bool dither, long long& clipped) {
+    WavWriter w(path, ch, frames, rate, format);
+    const SampleFormat& f = *find_format(format);
+    const size_t fb = w.frame_bytes();
+    const long long per_write = std::max<long long>(1, (long long)((1u << 20) / fb));   // about 1 MiB per write
+    std::vector<uint8_t> buf((size_t)per_write * fb);
+    const IntegerSample q{f.bytes(), std::ldexp(1.0, f.bits - 1)};
     std::mt19937_64 rng(12345);
-    std::uniform_real_distribution<double> U(-0.5, 0.5);
+    std::uniform_real_distribution<double> tpdf(-0.5, 0.5);
     clipped = 0;
-    const long long chunk = std::max(1, std::min(1 << 16, (int)((4 << 20) / align)));
-    std::vector<uint8_t> buf((size_t)chunk * align);
-    for (long long f0 = 0; f0 < frames; f0 += chunk) {
-        const long long nf = std::min(chunk, frames - f0);
+    for (long long f0 = 0; f0 < frames; f0 += per_write) {
+        const long long nf = std::min(per_write, frames - f0);
         uint8_t* p = buf.data();
         for (long long i = f0; i < f0 + nf; ++i)
-            for (int c = 0; c < ch; ++c, p += bps) {
-                const double x = y[c][i];
-                if (fmt == "f64") memcpy(p, &x, 8);
-                else if (fmt == "f32") { const float v = (float)x; memcpy(p, &v, 4); }
-                else if (bps == 2) quantise<2>(p, x, dither, rng, U, clipped);
-                else if (bps == 3) quantise<3>(p, x, dither, rng, U, clipped);
-                else quantise<4>(p, x, dither, rng, U, clipped);
+            for (int c = 0; c < ch; ++c, p += f.bytes()) {
+                const double x = y[(size_t)c * (size_t)frames + (size_t)i];
+                if (f.floating && f.bits == 64) {
+                    std::memcpy(p, &x, 8);
+                } else if (f.floating) {
+                    const float v = (float)x;
+                    std::memcpy(p, &v, 4);
+                } else {
+                    double v = x * q.full;
+                    if (dither) {
+                        const double d1 = tpdf(rng), d2 = tpdf(rng);
+                        v = v + (d1 + d2);
+                    }
+                    q.store(p, v, clipped);
+                }
             }
         w.write(buf.data(), (size_t)(p - buf.data()));
     }

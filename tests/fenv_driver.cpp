@@ -1,14 +1,21 @@
-// Test driver (not part of the library): calls kbsrc::resample with the caller in a given floating-point rounding
-// mode and reports whether the mode survived, as the reference's tests/fenv_driver.cpp does for its engines.
+// Rounding-mode probe for the tests (not part of the library): sets the calling thread's floating-point rounding
+// mode, calls kbsrc::resample, and reports whether the caller's mode came back unchanged.
 //
-// usage: kbsrc_fenv_driver <nearest|down|up|zero> <exact|fast> <rate_in> <rate_out> <channels> <length> <beta>
-//                          <in.f64> <out.f64>
-//   in.f64: interleaved little-endian doubles; out.f64: planar output, channel after channel.
-// Prints "mode_kept=<0|1> threw=<0|1> frames=<n>". Exit 0, 3 when resample threw std::invalid_argument, 2 on usage
-// or I/O errors.
+//   kbsrc_fenv_driver key=value ...
+//     round=nearest|down|up|zero  precision=exact|fast  from=<Hz> to=<Hz>  channels=<n>  len=<s>  beta=<b>
+//     src=<file of interleaved little-endian float64>  dst=<file for the planar float64 output>
+//     [tamper=1: after the call, change the SSE rounding field alone, as a library that failed to restore it would]
+//
+// Prints "restored=<0|1> rejected=<0|1> nout=<frames>". The mode counts as restored when fegetround(), the x87
+// and SSE rounding-control fields of the environment (fegetenv) and the direction in which SSE double division
+// actually rounds all read the same after the call as before it. Exit 0, 3 when the call was rejected
+// (std::invalid_argument), 2 for a bad request.
 #include <cfenv>
+#include <cmath>
 #include <cstdio>
-#include <cstdlib>
+#include <fstream>
+#include <iterator>
+#include <map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -16,64 +23,103 @@
 
 #include "kbsrc/kbsrc.hpp"
 
+// This file changes and inspects the rounding mode: no floating-point operation may be moved across fesetround or
+// evaluated in a mode other than the one in force where it is written.
+#pragma STDC FENV_ACCESS ON
+
 namespace {
 
-struct Mode {   // fegetround, and the SSE and x87 rounding-control bits the hardware holds
-    int fe = -1;
-    unsigned sse = 0, x87 = 0;
-    static Mode now() {
-        Mode m;
-        m.fe = std::fegetround();
-        unsigned short cw = 0;
-        __asm__ volatile("fnstcw %0" : "=m"(cw));
-        m.sse = _mm_getcsr() & 0x6000u;
-        m.x87 = cw & 0x0C00u;
-        return m;
+// The rounding state a thread can observe.
+struct RoundingState {
+    int fe;                // fegetround()
+    unsigned x87, sse;     // rounding-control fields of the x87 control word and of MXCSR (fenv_t on x86-64)
+    int direction[3];      // -1 / +1: whether q = a / b came out below or above the exact quotient, for three
+                           // quotients whose round-to-nearest, down, up and toward-zero results all differ in sign
+                           // pattern; the sign of fma(q, b, -a) is exact whatever the mode
+    bool operator==(const RoundingState& o) const {
+        return fe == o.fe && x87 == o.x87 && sse == o.sse && direction[0] == o.direction[0] &&
+               direction[1] == o.direction[1] && direction[2] == o.direction[2];
     }
-    bool same(const Mode& o) const { return fe == o.fe && sse == o.sse && x87 == o.x87; }
 };
+
+__attribute__((noinline)) RoundingState observe() {
+    RoundingState s{};
+    s.fe = std::fegetround();
+    std::fenv_t env;
+    std::fegetenv(&env);
+    s.x87 = (env.__control_word >> 10) & 3u;
+    s.sse = (env.__mxcsr >> 13) & 3u;
+    static const double num[3] = {1.0, -1.0, 5.0};
+    for (int k = 0; k < 3; ++k) {
+        volatile double a = num[k], b = 3.0;
+        volatile double q = a / b;
+        volatile double r = std::fma(q, (double)b, -(double)a);
+        s.direction[k] = r > 0 ? 1 : -1;
+    }
+    return s;
+}
+
+std::map<std::string, std::string> parse(int argc, char** argv) {
+    std::map<std::string, std::string> kv;
+    for (int i = 1; i < argc; ++i) {
+        const std::string a = argv[i];
+        const size_t eq = a.find('=');
+        if (eq == std::string::npos || eq == 0) throw std::runtime_error("expected key=value, got " + a);
+        kv[a.substr(0, eq)] = a.substr(eq + 1);
+    }
+    for (const char* need : {"round", "precision", "from", "to", "channels", "len", "beta", "src", "dst"})
+        if (!kv.count(need)) throw std::runtime_error(std::string("missing ") + need);
+    return kv;
+}
 
 }  // namespace
 
 int main(int argc, char** argv) {
-    if (argc != 10) {
-        std::fprintf(stderr, "usage: kbsrc_fenv_driver <nearest|down|up|zero> <exact|fast> <rate_in> <rate_out> "
-                             "<channels> <length> <beta> <in.f64> <out.f64>\n");
-        return 2;
-    }
-    const std::string mode = argv[1], prec = argv[2];
-    const int fe = mode == "nearest" ? FE_TONEAREST : mode == "down" ? FE_DOWNWARD : mode == "up" ? FE_UPWARD
-                   : mode == "zero" ? FE_TOWARDZERO : -1;
-    kbsrc::Options opt;
-    opt.precision = prec == "fast" ? kbsrc::Precision::Fast : kbsrc::Precision::Exact;
-    const int rate_in = std::atoi(argv[3]), rate_out = std::atoi(argv[4]), channels = std::atoi(argv[5]);
-    opt.length = std::strtod(argv[6], nullptr);   // parsed before the mode changes: strtod follows it
-    opt.beta = std::strtod(argv[7], nullptr);
-    if (fe < 0 || channels < 1 || (prec != "exact" && prec != "fast")) return 2;
+    std::map<std::string, std::string> kv;
     std::vector<double> in;
-    if (FILE* f = std::fopen(argv[8], "rb")) {
-        double buf[512];
-        for (size_t n; (n = std::fread(buf, sizeof(double), 512, f)) > 0;) in.insert(in.end(), buf, buf + n);
-        std::fclose(f);
-    } else {
+    int mode = 0;
+    kbsrc::Options opt;
+    int from = 0, to = 0, channels = 0;
+    try {
+        kv = parse(argc, argv);
+        const std::map<std::string, int> modes = {
+            {"nearest", FE_TONEAREST}, {"down", FE_DOWNWARD}, {"up", FE_UPWARD}, {"zero", FE_TOWARDZERO}};
+        mode = modes.at(kv["round"]);
+        if (kv["precision"] != "exact" && kv["precision"] != "fast") throw std::runtime_error("precision");
+        opt.precision = kv["precision"] == "fast" ? kbsrc::Precision::Fast : kbsrc::Precision::Exact;
+        from = std::stoi(kv["from"]);
+        to = std::stoi(kv["to"]);
+        channels = std::stoi(kv["channels"]);
+        opt.length = std::stod(kv["len"]);   // numbers are read before the mode changes
+        opt.beta = std::stod(kv["beta"]);
+        if (channels < 1) throw std::runtime_error("channels");
+        std::ifstream f(kv["src"], std::ios::binary);
+        if (!f) throw std::runtime_error("cannot read " + kv["src"]);
+        const std::vector<char> bytes((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+        in.resize(bytes.size() / sizeof(double));
+        if (!in.empty()) std::copy(bytes.begin(), bytes.begin() + in.size() * sizeof(double),
+                                   reinterpret_cast<char*>(in.data()));
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "kbsrc_fenv_driver: %s\n", e.what());
         return 2;
     }
-    if (std::fesetround(fe) != 0) return 2;
-    const Mode before = Mode::now();
-    std::vector<std::vector<double>> out;
-    bool threw = false;
+    if (std::fesetround(mode) != 0) return 2;
+    const RoundingState before = observe();
+    std::vector<std::vector<double>> y;
+    bool rejected = false;
     try {
-        out = kbsrc::resample(in.data(), (long long)(in.size() / channels), channels, rate_in, rate_out, opt);
+        y = kbsrc::resample(in.data(), (long long)(in.size() / channels), channels, from, to, opt);
     } catch (const std::invalid_argument&) {
-        threw = true;
+        rejected = true;
     }
-    const Mode after = Mode::now();
+    if (kv.count("tamper") && kv["tamper"] == "1")
+        _MM_SET_ROUNDING_MODE(_MM_GET_ROUNDING_MODE() == _MM_ROUND_NEAREST ? _MM_ROUND_DOWN : _MM_ROUND_NEAREST);
+    const RoundingState after = observe();
     std::fesetround(FE_TONEAREST);
-    std::printf("mode_kept=%d threw=%d frames=%zu\n", before.same(after) && before.fe == fe ? 1 : 0, threw ? 1 : 0,
-                out.empty() ? (size_t)0 : out[0].size());
-    FILE* f = std::fopen(argv[9], "wb");
-    if (!f) return 2;
-    for (const auto& c : out) std::fwrite(c.data(), sizeof(double), c.size(), f);
-    std::fclose(f);
-    return threw ? 3 : 0;
+    const bool restored = before == after && after.fe == mode;
+    std::printf("restored=%d rejected=%d nout=%zu\n", restored ? 1 : 0, rejected ? 1 : 0, y.empty() ? 0 : y[0].size());
+    std::ofstream out(kv["dst"], std::ios::binary);
+    for (const auto& c : y) out.write(reinterpret_cast<const char*>(c.data()), (std::streamsize)(c.size() * 8));
+    if (!out) return 2;
+    return rejected ? 3 : 0;
 }

@@ -3,6 +3,7 @@ with the reference's messages before touching a device. CPU only."""
 import ctypes
 import os
 import re
+import struct
 import subprocess
 
 import pytest
@@ -68,3 +69,20 @@ def test_cli_usage_errors():
         assert r.returncode == code and r.stderr.startswith("kbsrc: "), (argv, r.returncode, r.stderr)
     r = subprocess.run([H.GPU_CLI, "--version"], capture_output=True, text=True)
     assert r.returncode == 0 and r.stdout.startswith("kbsrc 1.3.0")
+
+
+@pytest.mark.parametrize("mode", ["nearest", "down", "up", "zero"])
+def test_rounding_driver_sees_a_changed_mode(mode, tmp_path):
+    """The rounding-mode driver of the GPU tests (tests/fenv_driver.cpp) on calls that need no GPU (equal rates, a
+    rejected rate): it reports the caller's mode as restored, and not restored when the SSE rounding field alone is
+    changed behind the library's back."""
+    driver = os.path.join(H.ROOT, "kbsrc_amd", "bin", "kbsrc_fenv_driver")
+    src, dst = tmp_path / "in.f64", tmp_path / "out.f64"
+    src.write_bytes(struct.pack("<4d", 0.1, 0.2, 0.3, 0.4))
+    for rate_in, code in ((8000, 0), (0, 3)):
+        for tamper, restored in (("0", "1"), ("1", "0")):
+            r = subprocess.run([driver, f"round={mode}", "precision=exact", f"from={rate_in}", "to=8000", "channels=2",
+                                "len=0.01", "beta=40", f"src={src}", f"dst={dst}", f"tamper={tamper}"],
+                               capture_output=True, text=True, timeout=60)
+            info = dict(kv.split("=", 1) for kv in r.stdout.split() if "=" in kv)
+            assert r.returncode == code and info.get("restored") == restored, (rate_in, tamper, r.stdout)

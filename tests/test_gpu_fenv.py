@@ -22,8 +22,9 @@ def drive(tmp_path, mode, x, rate_in, rate_out, prec="exact", length="0.1"):
     src.write_bytes(np.asarray(x, dtype="<f8").tobytes())
     if dst.exists():
         dst.unlink()
-    r = subprocess.run([DRIVER, mode, prec, str(rate_in), str(rate_out), "2", length, "40", str(src), str(dst)],
-                       capture_output=True, text=True, timeout=120)
+    r = subprocess.run([DRIVER, f"round={mode}", f"precision={prec}", f"from={rate_in}", f"to={rate_out}", "channels=2",
+                        f"len={length}", "beta=40", f"src={src}", f"dst={dst}"], capture_output=True, text=True,
+                       timeout=120)
     info = dict(kv.split("=", 1) for kv in r.stdout.split() if "=" in kv)
     data = dst.read_bytes() if dst.exists() else b""
     return r.returncode, info, data
@@ -33,21 +34,21 @@ def drive(tmp_path, mode, x, rate_in, rate_out, prec="exact", length="0.1"):
 def test_exact_is_independent_of_the_callers_mode(mode, tmp_path):
     # one stereo frame 8 kHz -> 16 kHz, sensitive to the rounding mode: the pinned samples in every mode
     rc, info, data = drive(tmp_path, mode, H.FRAME1, 8000, 16000, length="0.01")
-    assert rc == 0 and info.get("mode_kept") == "1"
+    assert rc == 0 and info.get("restored") == "1"
     got = [struct.pack(">d", v).hex() for v in struct.unpack("<%dd" % (len(data) // 8), data)]
     assert got == H.FRAME1_OUT
     # subnormal input 48k -> 96k: the pinned round-to-nearest output
     rc, info, data = drive(tmp_path, mode, H.subnormal97(), 48000, 96000)
-    assert rc == 0 and info.get("mode_kept") == "1" and H.sha(data) == H.SUBNORMAL_96K_PLANAR
+    assert rc == 0 and info.get("restored") == "1" and H.sha(data) == H.SUBNORMAL_96K_PLANAR
     # normal input, up- and downsampling: the round-to-nearest output
     for rate_out in (96000, 44100):
         rc0, _, ref = drive(tmp_path, "nearest", H.primary97(), 48000, rate_out)
         rc, info, data = drive(tmp_path, mode, H.primary97(), 48000, rate_out)
-        assert rc0 == 0 and rc == 0 and info.get("mode_kept") == "1" and data == ref
+        assert rc0 == 0 and rc == 0 and info.get("restored") == "1" and data == ref
 
 
 def test_callers_mode_survives_rejection_and_fast(tmp_path):
     rc, info, _ = drive(tmp_path, "down", H.FRAME1, 0, 16000)
-    assert rc == 3 and info.get("threw") == "1" and info.get("mode_kept") == "1"
+    assert rc == 3 and info.get("rejected") == "1" and info.get("restored") == "1"
     rc, info, _ = drive(tmp_path, "up", H.FRAME1, 8000, 16000, prec="fast", length="0.01")
-    assert rc == 0 and info.get("mode_kept") == "1"
+    assert rc == 0 and info.get("restored") == "1"
