@@ -16,7 +16,9 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <atomic>
 #include <condition_variable>
+#include <exception>
 #include <vector>
 
 #include "engine.hpp"
@@ -216,26 +218,78 @@ std::unique_ptr<kbsrc_plan> make(long long frames, int channels, int rate_in, in
     return pl;
 }
 
-// Staging (src/resampler.cpp:168-186) from the interleaved input in device memory (d_in) and its first and last
-// edge_frames() frames on the host: Burg extensions on the host, then deinterleave, pad and place on the GPU.
-void stage_dev(const Plan& p, bool lpc, const double* d_in, const double* first, const double* last, double* d_padded,
-               hipStream_t s) {
-    const long long ext = p.E + 32;
-    std::vector<double> head((size_t)p.ch * ext, 0.0), tail((size_t)p.ch * ext, 0.0);
-    if (lpc) {
-        for (int c = 0; c < p.ch; ++c) {
-            std::vector<double> hd, tl;
-            edge_extensions(p, first, last, c, hd, tl);
-            std::memcpy(head.data() + (size_t)c * ext, hd.data(), sizeof(double) * ext);
-            std::memcpy(tail.data() + (size_t)c * ext, tl.data(), sizeof(double) * ext);
+// The Burg head and tail extensions of every channel (src/lpc.cpp:13-41, src/resampler.cpp:176-185) from the first
+// and last edge_frames() interleaved frames on the host, ch x (E + 32) doubles each, zero without LPC. Every
+// (channel, end) is its own job; up to 16 worker threads take the jobs from construction on, each in the rounding
+// mode of the thread that made the work, so the caller can upload the input meanwhile. wait() joins them.
+class EdgeWork {
+public:
+    EdgeWork(const Plan& p, bool lpc, const double* first, const double* last)
+        : ext_(p.E + 32), head_((size_t)p.ch * ext_, 0.0), tail_((size_t)p.ch * ext_, 0.0) {
+        if (!lpc || p.nin == 0) return;
+        const int jobs = 2 * p.ch;
+        const unsigned mode = _MM_GET_ROUNDING_MODE();
+        auto worker = [this, &p, first, last, jobs, mode] {
+            _MM_SET_ROUNDING_MODE(mode);
+            try {
+                for (int j; (j = next_.fetch_add(1)) < jobs;) {
+                    const int c = j >> 1;
+                    const bool at_tail = j & 1;
+                    const std::vector<double> e = edge_extension(p, first, last, c, at_tail);
+                    std::memcpy((at_tail ? tail_ : head_).data() + (size_t)c * ext_, e.data(), sizeof(double) * ext_);
+                }
+            } catch (...) {
+                std::lock_guard<std::mutex> lock(err_mu_);
+                if (!err_) err_ = std::current_exception();
+            }
+        };
+        const int nt = std::min(jobs, 16);
+        for (int t = 0; t < nt; ++t) {
+            try {
+                threads_.emplace_back(worker);
+            } catch (...) {   // no thread: the jobs left run in wait()
+                break;
+            }
         }
+        finish_ = worker;
     }
+    ~EdgeWork() { join(); }
+    EdgeWork(const EdgeWork&) = delete;
+    EdgeWork& operator=(const EdgeWork&) = delete;
+    void wait() {
+        if (finish_) finish_();   // whatever no worker has taken
+        join();
+        if (err_) std::rethrow_exception(err_);
+    }
+    long long ext() const { return ext_; }
+    const std::vector<double>& head() const { return head_; }
+    const std::vector<double>& tail() const { return tail_; }
+
+private:
+    void join() {
+        for (auto& t : threads_)
+            if (t.joinable()) t.join();
+        threads_.clear();
+    }
+    long long ext_;
+    std::vector<double> head_, tail_;
+    std::atomic<int> next_{0};
+    std::vector<std::thread> threads_;
+    std::function<void()> finish_;
+    std::mutex err_mu_;
+    std::exception_ptr err_;
+};
+
+// Staging (src/resampler.cpp:168-186) from the interleaved input in device memory (d_in) and the finished edge
+// extensions: deinterleave, pad and place on the GPU.
+void stage_dev(const Plan& p, const double* d_in, EdgeWork& edges, double* d_padded, hipStream_t s) {
+    edges.wait();
     DevBuf dh, dt;
-    dh.upload(head.data(), sizeof(double) * head.size(), s);
-    dt.upload(tail.data(), sizeof(double) * tail.size(), s);
+    dh.upload(edges.head().data(), sizeof(double) * edges.head().size(), s);
+    dt.upload(edges.tail().data(), sizeof(double) * edges.tail().size(), s);
     StageArgs a;
     a.in = d_in; a.head = dh.d(); a.tail = dt.d(); a.xx = d_padded;
-    a.nin = p.nin; a.tot = p.tot; a.stride = p.xstride; a.off = p.base + p.pre; a.ext = ext; a.ch = p.ch;
+    a.nin = p.nin; a.tot = p.tot; a.stride = p.xstride; a.off = p.base + p.pre; a.ext = edges.ext(); a.ch = p.ch;
     hip_check(launch_stage(a, s), "staging kernel");
     hip_check(hipStreamSynchronize(s), "staging sync");
 }
@@ -410,8 +464,10 @@ void download_host(void* h_dst, const void* d_src, size_t bytes, hipStream_t s) 
     }
 }
 
-// Staging from host interleaved input: one upload, then as above.
+// Staging from host interleaved input: the edge extensions on worker threads while the input goes up, then as above.
 void stage(const Plan& p, bool lpc, const double* in, double* d_padded, hipStream_t s) {
+    const long long m = edge_frames(p);
+    EdgeWork edges(p, lpc, in, in + (p.nin - m) * p.ch);
     DevBuf din;
     if (p.nin > 0) {
         din.alloc(sizeof(double) * (size_t)(p.nin * p.ch));
@@ -419,8 +475,7 @@ void stage(const Plan& p, bool lpc, const double* in, double* d_padded, hipStrea
     } else {
         din.alloc(sizeof(double));
     }
-    const long long m = edge_frames(p);
-    stage_dev(p, lpc, din.d(), in, in + (p.nin - m) * p.ch, d_padded, s);
+    stage_dev(p, din.d(), edges, d_padded, s);
 }
 
 // Staging from device interleaved input: the edge frames come down to the host for the Burg recursion.
@@ -434,7 +489,8 @@ void stage_device(const Plan& p, bool lpc, const double* d_in, double* d_padded,
                   "edge download");
         hip_check(hipStreamSynchronize(s), "edge sync");
     }
-    stage_dev(p, lpc, d_in, first.data(), last.data(), d_padded, s);
+    EdgeWork edges(p, lpc, first.data(), last.data());
+    stage_dev(p, d_in, edges, d_padded, s);
 }
 
 void fill_stats(const kbsrc_plan& pl, double process_seconds, kbsrc_stats* st) {
@@ -474,19 +530,52 @@ void resample_host(const double* in, long long frames, int channels, int rate_in
             }
             return;
         }
-        hipStream_t s = nullptr;
-        hip_check(hipStreamCreate(&s), "stream");
+        // the plan's integers first (host, microseconds): the reference's rejections in its order, and the sizes
+        check_options(o);
+        const Plan P = make_plan(frames, channels, rate_in, rate_out, o.precision == KBSRC_FAST, o.length, o.beta,
+                                 o.block);
         struct StreamGuard {
-            hipStream_t s;
-            ~StreamGuard() { (void)hipStreamDestroy(s); }
-        } sg{s};
-        auto pl = make(frames, channels, rate_in, rate_out, o, s);
+            hipStream_t s = nullptr;
+            ~StreamGuard() {
+                if (s) (void)hipStreamDestroy(s);
+            }
+        } sg;
+        hip_check(hipStreamCreate(&sg.s), "stream");
+        hipStream_t s = sg.s;
+        // the kernel design runs on a thread and a stream of its own and the Burg edges on worker threads, all in
+        // this thread's rounding mode, while this thread uploads the input
+        std::unique_ptr<kbsrc_plan> pl;
+        std::exception_ptr design_err;
+        const unsigned mode = _MM_GET_ROUNDING_MODE();
+        std::thread design([&] {
+            _MM_SET_ROUNDING_MODE(mode);
+            try {
+                StreamGuard ds;
+                hip_check(hipStreamCreateWithFlags(&ds.s, hipStreamNonBlocking), "stream");
+                pl = make(frames, channels, rate_in, rate_out, o, ds.s);
+                hip_check(hipStreamSynchronize(ds.s), "design sync");
+            } catch (...) {
+                design_err = std::current_exception();
+            }
+        });
+        struct Joiner {
+            std::thread& t;
+            ~Joiner() {
+                if (t.joinable()) t.join();
+            }
+        } joiner{design};
+        const long long m = edge_frames(P);
+        EdgeWork edges(P, o.lpc_edges, in, in + (P.nin - m) * P.ch);
+        DevBuf din(sizeof(double) * (size_t)std::max(1LL, P.nin * P.ch));
+        if (P.nin > 0) upload_host(din.d(), in, sizeof(double) * (size_t)(P.nin * P.ch), s);
+        DevBuf xx((size_t)P.ch * P.xstride * sizeof(double));
+        DevBuf dout((size_t)std::max(1LL, P.ch * P.nout) * sizeof(double));
+        design.join();
+        if (design_err) std::rethrow_exception(design_err);
         const Plan& p = pl->p;
-        const double t1 = now();
-        DevBuf xx((size_t)p.ch * p.xstride * sizeof(double));
-        DevBuf dout((size_t)std::max(1LL, p.ch * p.nout) * sizeof(double));
-        stage(p, o.lpc_edges, in, xx.d(), s);
-        pl->design_seconds += now() - t1;
+        stage_dev(p, din.d(), edges, xx.d(), s);
+        din.release();
+        pl->design_seconds = now() - t0;   // planning, design and staging, the input's upload overlapping them
         const double t2 = now();
         pl->eng->execute(xx.d(), dout.d(), s);
         if (p.ch * p.nout > 0) download_host(out, dout.d(), sizeof(double) * p.ch * p.nout, s);

@@ -202,24 +202,24 @@ void edge_extensions(const Plan& p, const double* in, int channel, std::vector<d
     edge_extensions(p, in, in + (p.nin - m) * p.ch, channel, head, tail);
 }
 
-void edge_extensions(const Plan& p, const double* first, const double* last_frames, int channel,
-                     std::vector<double>& head, std::vector<double>& tail) {
+std::vector<double> edge_extension(const Plan& p, const double* first, const double* last_frames, int channel,
+                                   bool at_tail) {
     const long long ext = p.E + 32, ch = p.ch;
     const long long m = edge_frames(p);
     double fd[32];
     edge_fade(fd);
     // tail: the last m samples predict forward; head: the first m samples reversed predict backward
-    std::vector<double> last(m), first_rev(m);
-    for (long long i = 0; i < m; ++i) {
-        last[i] = last_frames[i * ch + channel];
-        first_rev[i] = first[(m - 1 - i) * ch + channel];
-    }
-    tail = burg_extrap(last.data(), m, ext);
-    head = burg_extrap(first_rev.data(), m, ext);
-    for (int i = 0; i < 32; ++i) {
-        tail[ext - 32 + i] *= fd[i];
-        head[ext - 32 + i] *= fd[i];
-    }
+    std::vector<double> train(m);
+    for (long long i = 0; i < m; ++i)
+        train[i] = at_tail ? last_frames[i * ch + channel] : first[(m - 1 - i) * ch + channel];
+    std::vector<double> e = burg_extrap(train.data(), m, ext);
+    for (int i = 0; i < 32; ++i) e[ext - 32 + i] *= fd[i];
+    return e;
+}
+void edge_extensions(const Plan& p, const double* first, const double* last_frames, int channel,
+                     std::vector<double>& head, std::vector<double>& tail) {
+    tail = edge_extension(p, first, last_frames, channel, true);
+    head = edge_extension(p, first, last_frames, channel, false);
 }
 
 }  // namespace kbsrc::gpu

@@ -59,5 +59,7 @@ void edge_extensions(const Plan& p, const double* first, const double* last, int
                      std::vector<double>& tail);
 // Frames each edge's prediction trains on: min(frames, 4096) (src/lpc.cpp:13-41, src/resampler.cpp:176-185).
 inline long long edge_frames(const Plan& p) { return p.nin < 4096 ? p.nin : 4096; }
+// One of them: the tail (at_tail) or the head extension of one channel.
+std::vector<double> edge_extension(const Plan& p, const double* first, const double* last, int channel, bool at_tail);
 
 }  // namespace kbsrc::gpu
