@@ -492,14 +492,22 @@ def bench_extreme(args):
         ms = timed(lambda: plan.execute(xx.data_ptr(), out.data_ptr(), stream))
         kt = plan.execute_timed(xx.data_ptr(), out.data_ptr(), stream)
         nout_total = ch * info["nout"]
+        # the same conversion in fast mode (FP64; its own automatic block)
+        fplan = kbsrc_amd.Plan(frames, ch, rate_in, rate_out, "fast", length=0.5)
+        fxx = torch.empty(ch * fplan.info["xstride"], dtype=torch.float64, device=dev)
+        fplan.stage(x, fxx.data_ptr(), stream)
+        fast_ms = timed(lambda: fplan.execute(fxx.data_ptr(), out.data_ptr(), stream))
+        fplan.close()
+        del fxx
+        plan.execute(xx.data_ptr(), out.data_ptr(), stream)   # the exact output again, for the split comparison
         fl = block_flops(info["Nb"], info["Mb"], True, bool(info["up"]))
         gflop = sum(fl.values()) * info["tasks"] / 1e9
         row = {"workload": name, "ms": round(ms, 3), "output_msamples_s": round(nout_total / (ms / 1e3) / 1e6, 1),
                "tflops": round(gflop / ms, 2), "nb": info["nb"], "Nb": info["Nb"], "Mb": info["Mb"],
                "tasks": info["tasks"], "design_s": round(t_design, 3),
-               "pass_ms": {k: round(v, 3) for k, v in kt.items()}}
+               "pass_ms": {k: round(v, 3) for k, v in kt.items()}, "fast_ms": round(fast_ms, 3)}
         print(f"{name}: {ms:.3f} ms -> {row['output_msamples_s']} output Msamples/s, {row['tflops']} TFLOP/s "
-              f"(design {t_design:.2f} s)", file=sys.stderr, flush=True)
+              f"(design {t_design:.2f} s); fast mode {fast_ms:.3f} ms", file=sys.stderr, flush=True)
         if split_slices:
             plain = out.clone()
             peak = float(plain.abs().max().item())
