@@ -194,6 +194,73 @@ def summarize_rocprof(d, mode, steps=20, warmup=5):
     return 0
 
 
+COUNTER_SETS = (
+    ("pmc1", "SQ_WAVES SQ_INSTS_VALU SQ_INSTS_VALU_ADD_F64 SQ_INSTS_VALU_MUL_F64 SQ_INSTS_VALU_FMA_F64 SQ_INSTS_LDS "
+             "SQ_INSTS_SALU SQ_ACTIVE_INST_VALU GRBM_GUI_ACTIVE"),
+    ("pmc2", "SQ_WAVE_CYCLES SQ_BUSY_CYCLES SQ_WAIT_INST_LDS SQ_WAIT_INST_ANY SQ_ACTIVE_INST_LDS SQ_LDS_BANK_CONFLICT "
+             "SQ_ACTIVE_INST_ANY SQ_THREAD_CYCLES_VALU GRBM_GUI_ACTIVE"),
+)
+
+
+def summarize_counters(d, mode):
+    """Reads <d>/pmc1 and <d>/pmc2 (rocprofv3 --pmc, csv) and prints the per-pass VALU and LDS summary kept under
+    profiles/: counters summed over XCDs and shader engines per dispatch, averaged over the dispatches of a pass."""
+    per = {}
+    for sub, _ in COUNTER_SETS:
+        rows = _csv_rows(os.path.join(d, sub, "run_counter_collection.csv"))
+        disp = {}
+        for r in rows:
+            key = (int(r["Dispatch_Id"]), r["Kernel_Name"])
+            disp.setdefault(key, {}).setdefault(r["Counter_Name"], 0.0)
+            disp[key][r["Counter_Name"]] += float(r["Counter_Value"])
+        keys = sorted(disp)
+        lab = _pass_of_dispatches([k[1] for k in keys])
+        for k, p in zip(keys, lab):
+            if p:
+                acc = per.setdefault(p, {})
+                for c, v in disp[k].items():
+                    acc.setdefault(c, []).append(v)
+    avg = {p: {c: sum(v) / len(v) for c, v in cs.items()} for p, cs in per.items()}
+    out = [f"# VALU and LDS counters per pass, stereo 96000 -> 44100 Hz, 180 s, {mode} (214 tasks per launch); MI355X, "
+           "rocprofv3.",
+           "# commands, from /tmp with TMPDIR=/tmp, R = the repository root, D = an output folder (two passes: at most "
+           "8 SQ counters each):"]
+    for sub, counters in COUNTER_SETS:
+        out.append(f"#   rocprofv3 --pmc {counters} -d D/{sub} -o run --output-format csv -- python3 R/bench.py --mode "
+                   f"{mode} --steps 2 --warmup 1 --no-cpu --no-split --cpu-runs 1")
+    out.append(f"#   python3 R/bench.py --summarize-counters D --mode {mode}")
+    out.append("# per dispatch, summed over all XCDs/SEs, averaged over dispatches. VALU busy = SQ_ACTIVE_INST_VALU x 4 "
+               "cycles /")
+    out.append("# (1024 SIMDs x GRBM_GUI_ACTIVE / 8 XCDs): SQ_ACTIVE_INST_VALU counts one quad-cycle per wave64 VALU "
+               "instruction (FP64 at full rate on 16 lanes per SIMD).")
+    out.append(f"# {'pass':8s} {'VALU insts':>10s} {'FP64 share':>10s} {'adds':>6s} {'muls':>6s} {'fmas':>6s} "
+               f"{'VALU busy':>9s} {'LDS insts':>10s} {'LDS bank-conflict/LDS active':>28s}")
+    for p in PASSES:
+        a = avg.get(p)
+        if not a:
+            continue
+        v = a["SQ_INSTS_VALU"]
+        f64 = a["SQ_INSTS_VALU_ADD_F64"] + a["SQ_INSTS_VALU_MUL_F64"] + a["SQ_INSTS_VALU_FMA_F64"]
+        busy = a["SQ_ACTIVE_INST_VALU"] * 4 / (1024 * a["GRBM_GUI_ACTIVE"] / 8)
+        conf = a["SQ_LDS_BANK_CONFLICT"] / a["SQ_ACTIVE_INST_LDS"] if a.get("SQ_ACTIVE_INST_LDS") else 0.0
+        out.append(f"  {p:8s} {v:10.4g} {f64 / v:10.2f} {a['SQ_INSTS_VALU_ADD_F64'] / v:6.2f} "
+                   f"{a['SQ_INSTS_VALU_MUL_F64'] / v:6.2f} {a['SQ_INSTS_VALU_FMA_F64'] / v:6.2f} {busy:9.2f} "
+                   f"{a['SQ_INSTS_LDS']:10.4g} {conf:28.2f}")
+    out.append("")
+    out.append(f"# {'pass':6s} {'wave_cycles':>11s} {'wait_any/wave':>13s} {'wait_lds/wave':>13s} {'active_any/wave':>15s} "
+               "thread_cycles_valu/(64*active_valu)")
+    for p in PASSES:
+        a = avg.get(p)
+        if not a:
+            continue
+        wc = a["SQ_WAVE_CYCLES"]
+        out.append(f"  {p:8s} {wc:11.4g} {a['SQ_WAIT_INST_ANY'] / wc:13.3f} {a['SQ_WAIT_INST_LDS'] / wc:13.3f} "
+                   f"{a['SQ_ACTIVE_INST_ANY'] / wc:15.3f} "
+                   f"{a['SQ_THREAD_CYCLES_VALU'] / (64 * a['SQ_ACTIVE_INST_VALU']) if 'SQ_ACTIVE_INST_VALU' in a else 0:10.3f}")
+    print("\n".join(out))
+    return 0
+
+
 def committed_traffic(mode, pass_name):
     """HBM bytes per launch of a pass from the committed rocprofv3 summary, or None."""
     path = os.path.join(ROOT, "profiles", f"passes_96k_to_44k1_180s_{mode}.txt")
@@ -714,11 +781,15 @@ def main():
                     help="single: the stereo 96 kHz -> 44.1 kHz 180 s job (default); batch: 64 mixed jobs in one "
                          "batch; split: int8-MFMA split-precision radix-16 DFTs against the plain dd pass; extreme: "
                          "10 MHz -> 48 kHz 30 s and 8-channel 384 kHz -> 44.1 kHz 600 s at --len 0.5")
+    ap.add_argument("--summarize-counters", metavar="DIR",
+                    help="print the per-pass VALU/LDS counter summary of rocprofv3 csv output under DIR/{pmc1,pmc2}")
     ap.add_argument("--summarize-rocprof", metavar="DIR",
                     help="print the per-pass summary of rocprofv3 csv output under DIR/{stats,fetch,write} and exit")
     args = ap.parse_args()
     if args.summarize_rocprof:
         return summarize_rocprof(args.summarize_rocprof, args.mode, args.steps, args.warmup)
+    if args.summarize_counters:
+        return summarize_counters(args.summarize_counters, args.mode)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}): the path would shard by block ranges with "
               "no exchange step, and nothing here drives a second GPU; run with --gpus 1", file=sys.stderr)
