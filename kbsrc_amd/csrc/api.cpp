@@ -9,6 +9,7 @@
 #include <cstdint>
 #include <cstring>
 #include <functional>
+#include <map>
 #include <tuple>
 #include <memory>
 #include <mutex>
@@ -21,6 +22,7 @@
 #include <exception>
 #include <vector>
 
+#include "copy_crew.hpp"
 #include "engine.hpp"
 #include "kbsrc/kbsrc.hpp"
 #include "kbsrc_gpu.h"
@@ -282,31 +284,53 @@ private:
 
 // Staging (src/resampler.cpp:168-186) from the interleaved input in device memory (d_in) and the finished edge
 // extensions: deinterleave, pad and place on the GPU.
-void stage_dev(const Plan& p, const double* d_in, EdgeWork& edges, double* d_padded, hipStream_t s) {
+// The edge extensions on the device: every channel's head, then every channel's tail (ext() doubles each). Waits for
+// the edge work; the upload is complete on return.
+DevBuf upload_edges(EdgeWork& edges, hipStream_t s) {
     edges.wait();
-    DevBuf dh, dt;
-    dh.upload(edges.head().data(), sizeof(double) * edges.head().size(), s);
-    dt.upload(edges.tail().data(), sizeof(double) * edges.tail().size(), s);
-    StageArgs a;
-    a.in = d_in; a.head = dh.d(); a.tail = dt.d(); a.xx = d_padded;
-    a.nin = p.nin; a.tot = p.tot; a.stride = p.xstride; a.off = p.base + p.pre; a.ext = edges.ext(); a.ch = p.ch;
-    hip_check(launch_stage(a, s), "staging kernel");
-    hip_check(hipStreamSynchronize(s), "staging sync");
+    const std::vector<double>& head = edges.head();
+    const std::vector<double>& tail = edges.tail();
+    std::vector<double> both(head.size() + tail.size());
+    std::copy(head.begin(), head.end(), both.begin());
+    std::copy(tail.begin(), tail.end(), both.begin() + (std::ptrdiff_t)head.size());
+    DevBuf d;
+    d.upload(both.data(), sizeof(double) * both.size(), s);
+    return d;
 }
 
-// Host <-> device copies of caller buffers through pairs of pinned 64 MB chunks: each chunk is filled or drained on
-// the host by up to 8 threads (page faults on a fresh caller buffer are taken in parallel) while the DMA of the
-// other chunk runs. A pair has its own stream, which first waits (on the device) for the work already queued on the
-// caller's stream, so the copies keep the caller's order without any thread blocking on the caller's stream. Pairs
-// come from a small pool made on first use and kept for the process; a lease returns its pair only once the pair's
-// stream has drained, whatever the exit path. Without pinned memory the copies go straight from and to the caller's
-// buffer on the caller's stream. Synchronous: returns when done.
+// Staging on the device from the interleaved input d_in and the uploaded edges (upload_edges). sync: return once the
+// padded input is complete (else only queued on s).
+void stage_edges(const Plan& p, const double* d_in, const EdgeWork& edges, const DevBuf& d_edges, double* d_padded,
+                 hipStream_t s, bool sync) {
+    StageArgs a;
+    a.in = d_in; a.head = d_edges.d(); a.tail = d_edges.d() + edges.head().size(); a.xx = d_padded;
+    a.nin = p.nin; a.tot = p.tot; a.stride = p.xstride; a.off = p.base + p.pre; a.ext = edges.ext(); a.ch = p.ch;
+    hip_check(launch_stage(a, s), "staging kernel");
+    if (sync) hip_check(hipStreamSynchronize(s), "staging sync");
+}
+
+void stage_dev(const Plan& p, const double* d_in, EdgeWork& edges, double* d_padded, hipStream_t s) {
+    const DevBuf d_edges = upload_edges(edges, s);
+    stage_edges(p, d_in, edges, d_edges, d_padded, s, true);
+}
+
+// Host <-> device copies of caller buffers through pinned staging: a pair of 64 MiB pinned buffers used as a ring of
+// 16 MiB slots. Each slot is filled or drained on the host by a crew of copy threads (page faults on a fresh caller
+// buffer are taken in parallel) while the DMA of the other slots runs; with small slots the first and the last host
+// copy, which nothing overlaps, are short. A pair has its own stream, which first waits (on the device) for the work
+// already queued on the caller's stream, so the copies keep the caller's order without any thread blocking on the
+// caller's stream. Pairs come from a small pool made on first use and kept for the process; a lease returns its pair
+// only once the pair's stream has drained, whatever the exit path. Without pinned memory the copies go straight from
+// and to the caller's buffer on the caller's stream. Synchronous: returns when done.
 struct ChunkPair {
-    static constexpr size_t kBytes = 64u << 20;
+    static constexpr size_t kBytes = 64u << 20;   // each pinned buffer
+    static constexpr size_t kSlot = 16u << 20;
+    static constexpr int kSlots = (int)(2 * kBytes / kSlot);
     void* buf[2] = {nullptr, nullptr};
-    hipEvent_t ev[2] = {nullptr, nullptr};   // the last DMA issued from / into buf[i]
+    hipEvent_t ev[kSlots] = {};              // the last DMA issued from / into slot i
     hipEvent_t order = nullptr;              // recorded on the caller's stream
     hipStream_t st = nullptr;
+    char* slot(int i) const { return static_cast<char*>(buf[i / (kSlots / 2)]) + (size_t)(i % (kSlots / 2)) * kSlot; }
 };
 
 class ChunkPool {
@@ -350,15 +374,15 @@ private:
         auto c = std::make_unique<ChunkPair>();
         bool ok = hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) == hipSuccess &&
                   hipEventCreateWithFlags(&c->order, hipEventDisableTiming) == hipSuccess;
-        for (int i = 0; i < 2 && ok; ++i)
-            ok = hipHostMalloc(&c->buf[i], ChunkPair::kBytes, hipHostMallocDefault) == hipSuccess &&
-                 hipEventCreateWithFlags(&c->ev[i], hipEventDisableTiming) == hipSuccess;
+        for (int i = 0; i < 2 && ok; ++i) ok = hipHostMalloc(&c->buf[i], ChunkPair::kBytes, hipHostMallocDefault) == hipSuccess;
+        for (int i = 0; i < ChunkPair::kSlots && ok; ++i)
+            ok = hipEventCreateWithFlags(&c->ev[i], hipEventDisableTiming) == hipSuccess;
         if (ok) return c.release();
         (void)hipGetLastError();
-        for (int i = 0; i < 2; ++i) {
+        for (int i = 0; i < 2; ++i)
             if (c->buf[i]) (void)hipHostFree(c->buf[i]);
+        for (int i = 0; i < ChunkPair::kSlots; ++i)
             if (c->ev[i]) (void)hipEventDestroy(c->ev[i]);
-        }
         if (c->order) (void)hipEventDestroy(c->order);
         if (c->st) (void)hipStreamDestroy(c->st);
         return nullptr;
@@ -392,29 +416,6 @@ private:
     ChunkPair* c_;
 };
 
-void parallel_copy(void* dst, const void* src, size_t bytes) {
-    const unsigned hw = std::thread::hardware_concurrency();
-    const unsigned nt = std::min(8u, std::max(1u, hw));
-    if (nt == 1 || bytes < (8u << 20)) {
-        std::memcpy(dst, src, bytes);
-        return;
-    }
-    const size_t per = ((bytes + nt - 1) / nt + 4095) & ~(size_t)4095;
-    auto part = [=](size_t o) {
-        std::memcpy(static_cast<char*>(dst) + o, static_cast<const char*>(src) + o, std::min(per, bytes - o));
-    };
-    std::vector<std::thread> th;
-    for (unsigned t = 1; t < nt && t * per < bytes; ++t) {
-        try {
-            th.emplace_back(part, t * per);
-        } catch (...) {   // no thread: this part here
-            part(t * per);
-        }
-    }
-    part(0);
-    for (auto& x : th) x.join();
-}
-
 void upload_host(void* d_dst, const void* h_src, size_t bytes, hipStream_t s) {
     ChunkLease lease;
     ChunkPair* c = lease.get();
@@ -425,44 +426,105 @@ void upload_host(void* d_dst, const void* h_src, size_t bytes, hipStream_t s) {
     }
     hip_check(hipEventRecord(c->order, s), "staging order");
     hip_check(hipStreamWaitEvent(c->st, c->order, 0), "staging order");
-    const size_t K = ChunkPair::kBytes;
+    const size_t K = ChunkPair::kSlot;
     for (size_t off = 0, k = 0; off < bytes; off += K, ++k) {
-        const int b = (int)(k & 1);
+        const int b = (int)(k % ChunkPair::kSlots);
         const size_t n = std::min(K, bytes - off);
-        if (k >= 2) hip_check(hipEventSynchronize(c->ev[b]), "staging wait");   // the DMA that last read buf[b]
-        parallel_copy(c->buf[b], static_cast<const char*>(h_src) + off, n);
-        hip_check(hipMemcpyAsync(static_cast<char*>(d_dst) + off, c->buf[b], n, hipMemcpyHostToDevice, c->st),
+        if (k >= (size_t)ChunkPair::kSlots) hip_check(hipEventSynchronize(c->ev[b]), "staging wait");   // slot free
+        CopyCrew::get().copy(c->slot(b), static_cast<const char*>(h_src) + off, n);
+        hip_check(hipMemcpyAsync(static_cast<char*>(d_dst) + off, c->slot(b), n, hipMemcpyHostToDevice, c->st),
                   "upload");
         hip_check(hipEventRecord(c->ev[b], c->st), "staging event");
     }
     hip_check(hipStreamSynchronize(c->st), "upload sync");
 }
 
-void download_host(void* h_dst, const void* d_src, size_t bytes, hipStream_t s) {
+// Pieces of a device buffer downloaded into host memory in order, each once the work before its event is done: the
+// first pieces go down while the work that writes the later ones still runs. Staged like upload_host, in reverse.
+struct DownPiece {
+    void* host;
+    const void* dev;
+    size_t bytes;
+    hipEvent_t ready;
+};
+void download_pieces(const std::vector<DownPiece>& pieces) {
     ChunkLease lease;
     ChunkPair* c = lease.get();
     if (!c) {   // no pinned memory: straight into the caller's buffer
-        hip_check(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, s), "download");
-        hip_check(hipStreamSynchronize(s), "download sync");
+        for (const DownPiece& q : pieces) {
+            hip_check(hipEventSynchronize(q.ready), "download wait");
+            hip_check(hipMemcpy(q.host, q.dev, q.bytes, hipMemcpyDeviceToHost), "download");
+        }
         return;
     }
-    hip_check(hipEventRecord(c->order, s), "staging order");   // after the work that writes d_src
-    hip_check(hipStreamWaitEvent(c->st, c->order, 0), "staging order");
-    const size_t K = ChunkPair::kBytes, chunks = (bytes + K - 1) / K;
-    auto issue = [&](size_t k) {
-        const size_t off = k * K, n = std::min(K, bytes - off);
-        hip_check(hipMemcpyAsync(c->buf[k & 1], static_cast<const char*>(d_src) + off, n, hipMemcpyDeviceToHost,
-                                 c->st), "download");
-        hip_check(hipEventRecord(c->ev[k & 1], c->st), "staging event");
+    struct Job {
+        size_t piece, off, n;
     };
-    if (chunks > 0) issue(0);
-    for (size_t k = 0; k < chunks; ++k) {
-        if (k + 1 < chunks) issue(k + 1);   // into the other chunk, whose copy-out finished last iteration
-        hip_check(hipEventSynchronize(c->ev[k & 1]), "staging wait");
-        const size_t off = k * K, n = std::min(K, bytes - off);
-        parallel_copy(static_cast<char*>(h_dst) + off, c->buf[k & 1], n);
+    std::vector<Job> jobs;
+    const size_t K = ChunkPair::kSlot;
+    for (size_t i = 0; i < pieces.size(); ++i)
+        for (size_t off = 0; off < pieces[i].bytes; off += K) jobs.push_back({i, off, std::min(K, pieces[i].bytes - off)});
+    auto issue = [&](size_t k) {
+        const Job& j = jobs[k];
+        const DownPiece& q = pieces[j.piece];
+        if (j.off == 0) hip_check(hipStreamWaitEvent(c->st, q.ready, 0), "download order");
+        const int b = (int)(k % ChunkPair::kSlots);
+        hip_check(hipMemcpyAsync(c->slot(b), static_cast<const char*>(q.dev) + j.off, j.n, hipMemcpyDeviceToHost, c->st),
+                  "download");
+        hip_check(hipEventRecord(c->ev[b], c->st), "staging event");
+    };
+    for (size_t k = 0; k < jobs.size() && k < (size_t)ChunkPair::kSlots; ++k) issue(k);
+    for (size_t k = 0; k < jobs.size(); ++k) {
+        const int b = (int)(k % ChunkPair::kSlots);
+        hip_check(hipEventSynchronize(c->ev[b]), "staging wait");
+        const Job& j = jobs[k];
+        CopyCrew::get().copy(static_cast<char*>(pieces[j.piece].host) + j.off, c->slot(b), j.n);
+        if (k + ChunkPair::kSlots < jobs.size()) issue(k + ChunkPair::kSlots);   // into the slot just emptied
     }
 }
+
+// Non-blocking streams on the current device kept for reuse by host-buffer conversions (making one costs ~0.1 ms,
+// destroying one ~0.7 ms, measured on MI355X). A lease takes one (or makes one) and gives it back drained.
+class StreamLease {
+public:
+    StreamLease() {
+        hip_check(hipGetDevice(&dev_), "device");
+        {
+            std::lock_guard<std::mutex> lock(mu());
+            auto& v = free_list()[dev_];
+            if (!v.empty()) {
+                s_ = v.back();
+                v.pop_back();
+            }
+        }
+        if (!s_) hip_check(hipStreamCreateWithFlags(&s_, hipStreamNonBlocking), "stream");
+    }
+    ~StreamLease() {
+        if (!s_) return;
+        if (hipStreamSynchronize(s_) != hipSuccess) {   // a failed stream is not reused
+            (void)hipGetLastError();
+            (void)hipStreamDestroy(s_);
+            return;
+        }
+        std::lock_guard<std::mutex> lock(mu());
+        free_list()[dev_].push_back(s_);
+    }
+    StreamLease(const StreamLease&) = delete;
+    StreamLease& operator=(const StreamLease&) = delete;
+    hipStream_t get() const { return s_; }
+
+private:
+    static std::mutex& mu() {
+        static std::mutex* m = new std::mutex;
+        return *m;
+    }
+    static std::map<int, std::vector<hipStream_t>>& free_list() {
+        static auto* f = new std::map<int, std::vector<hipStream_t>>;   // kept until the process ends
+        return *f;
+    }
+    int dev_ = 0;
+    hipStream_t s_ = nullptr;
+};
 
 // Staging from host interleaved input: the edge extensions on worker threads while the input goes up, then as above.
 void stage(const Plan& p, bool lpc, const double* in, double* d_padded, hipStream_t s) {
@@ -534,26 +596,28 @@ void resample_host(const double* in, long long frames, int channels, int rate_in
         check_options(o);
         const Plan P = make_plan(frames, channels, rate_in, rate_out, o.precision == KBSRC_FAST, o.length, o.beta,
                                  o.block);
-        struct StreamGuard {
-            hipStream_t s = nullptr;
-            ~StreamGuard() {
-                if (s) (void)hipStreamDestroy(s);
-            }
-        } sg;
-        hip_check(hipStreamCreate(&sg.s), "stream");
-        hipStream_t s = sg.s;
+        const StreamLease sl;
+        const hipStream_t s = sl.get();
+        // every device buffer of this call, the plan's included, is allocated and freed in stream order (on s, the
+        // design's on its own stream), so that no allocation or free waits for the copies and kernels in flight;
+        // the plan is released before the scope ends, after the last work on s
+        const PoolScope scope(s);
         // the kernel design runs on a thread and a stream of its own and the Burg edges on worker threads, all in
-        // this thread's rounding mode, while this thread uploads the input
+        // this thread's rounding mode, while this thread uploads the input; the design thread then uploads the edges
+        const long long m = edge_frames(P);
+        EdgeWork edges(P, o.lpc_edges, in, in + (P.nin - m) * P.ch);
         std::unique_ptr<kbsrc_plan> pl;
+        DevBuf dedge;
         std::exception_ptr design_err;
         const unsigned mode = _MM_GET_ROUNDING_MODE();
         std::thread design([&] {
             _MM_SET_ROUNDING_MODE(mode);
             try {
-                StreamGuard ds;
-                hip_check(hipStreamCreateWithFlags(&ds.s, hipStreamNonBlocking), "stream");
-                pl = make(frames, channels, rate_in, rate_out, o, ds.s);
-                hip_check(hipStreamSynchronize(ds.s), "design sync");
+                const StreamLease ds;
+                const PoolScope dscope(ds.get());
+                pl = make(frames, channels, rate_in, rate_out, o, ds.get());
+                hip_check(hipStreamSynchronize(ds.get()), "design sync");
+                dedge = upload_edges(edges, ds.get());
             } catch (...) {
                 design_err = std::current_exception();
             }
@@ -564,8 +628,6 @@ void resample_host(const double* in, long long frames, int channels, int rate_in
                 if (t.joinable()) t.join();
             }
         } joiner{design};
-        const long long m = edge_frames(P);
-        EdgeWork edges(P, o.lpc_edges, in, in + (P.nin - m) * P.ch);
         DevBuf din(sizeof(double) * (size_t)std::max(1LL, P.nin * P.ch));
         if (P.nin > 0) upload_host(din.d(), in, sizeof(double) * (size_t)(P.nin * P.ch), s);
         DevBuf xx((size_t)P.ch * P.xstride * sizeof(double));
@@ -573,12 +635,40 @@ void resample_host(const double* in, long long frames, int channels, int rate_in
         design.join();
         if (design_err) std::rethrow_exception(design_err);
         const Plan& p = pl->p;
-        stage_dev(p, din.d(), edges, xx.d(), s);
+        stage_edges(p, din.d(), edges, dedge, xx.d(), s, false);
         din.release();
         pl->design_seconds = now() - t0;   // planning, design and staging, the input's upload overlapping them
         const double t2 = now();
-        pl->eng->execute(xx.d(), dout.d(), s);
-        if (p.ch * p.nout > 0) download_host(out, dout.d(), sizeof(double) * p.ch * p.nout, s);
+        // the engine in up to four parts of the channel-major task order: a part's output samples (planar, the
+        // same offsets on both sides) go down while the next part computes
+        Engine& eng = *pl->eng;
+        const long long nt = eng.tasks(), hopP = (long long)p.hop * p.P;
+        const int parts = (int)std::min<long long>(nt, 4);
+        struct Events {
+            std::vector<hipEvent_t> e;
+            ~Events() {
+                for (hipEvent_t x : e) (void)hipEventDestroy(x);
+            }
+        } ev;
+        std::vector<DownPiece> pieces;
+        // first output sample of block b (blocks tile [0, nout) in order, src/resampler.cpp:80-84)
+        auto start = [&](long long b) { return b >= p.nblk ? p.nout : std::clamp<long long>(b * hopP - p.o, 0, p.nout); };
+        for (int k = 0; k < parts; ++k) {
+            const long long t0 = nt * k / parts, t1 = nt * (k + 1) / parts;
+            eng.execute_tasks(xx.d(), dout.d(), s, t0, t1);
+            hipEvent_t e = nullptr;
+            hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "event");
+            ev.e.push_back(e);
+            hip_check(hipEventRecord(e, s), "event");
+            for (long long t = t0; t < t1;) {   // per channel, the samples of blocks [b0, b1)
+                const long long c = t / p.nblk, b0 = t - c * p.nblk, b1 = std::min(p.nblk, b0 + (t1 - t));
+                const long long lo = start(b0), hi = b1 >= p.nblk ? p.nout : start(b1);
+                if (hi > lo)
+                    pieces.push_back({out + c * p.nout + lo, dout.d() + c * p.nout + lo, sizeof(double) * (hi - lo), e});
+                t += b1 - b0;
+            }
+        }
+        if (!pieces.empty()) download_pieces(pieces);
         hip_check(hipStreamSynchronize(s), "engine sync");
         fill_stats(*pl, now() - t2, st);
     });

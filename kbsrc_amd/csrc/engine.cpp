@@ -17,21 +17,136 @@ void hip_check(hipError_t e, const char* what) {
     if (e != hipSuccess) throw HipError(std::string(what) + ": " + hipGetErrorString(e));
 }
 
+namespace {
+thread_local const PoolScope* t_scope = nullptr;
+
+// Device blocks of released pooled DevBufs, kept for reuse by later scoped allocations (on this ROCm a hipFree or
+// hipFreeAsync of a large block costs ~1.5 ms on the calling thread and the stream-ordered pools return freed memory
+// at once, whatever their release threshold). A block carries an event recorded on the stream of its release; the
+// allocation that takes it makes its own stream wait for that event. Up to kCap bytes stay cached, the most recent
+// releases first; beyond that the oldest blocks are freed.
+class BlockCache {
+public:
+    // a cached block of at least `bytes` (at most twice that) on the current device, ordered on s, or nullptr
+    void* take(size_t bytes, hipStream_t s, size_t& n) {
+        int dev = 0;
+        hip_check(hipGetDevice(&dev), "device");
+        std::lock_guard<std::mutex> lock(m_);
+        size_t best = blocks_.size();
+        for (size_t i = 0; i < blocks_.size(); ++i) {
+            const Block& b = blocks_[i];
+            if (b.dev == dev && b.n >= bytes && b.n <= 2 * bytes && (best == blocks_.size() || b.n < blocks_[best].n))
+                best = i;
+        }
+        if (best == blocks_.size()) return nullptr;
+        Block b = blocks_[best];
+        blocks_.erase(blocks_.begin() + (std::ptrdiff_t)best);
+        cached_ -= b.n;
+        if (hipStreamWaitEvent(s, b.ready, 0) != hipSuccess) {   // cannot order it: free it once the device is idle
+            (void)hipGetLastError();
+            evict(b);
+            return nullptr;
+        }
+        spare_.push_back(b.ready);
+        n = b.n;
+        return b.p;
+    }
+    // keeps p (n bytes, on the current device) for reuse once the work queued on s so far is done
+    void give(void* p, size_t n, hipStream_t s) {
+        int dev = 0;
+        Block b{p, n, nullptr, 0};
+        std::vector<Block> out;
+        {
+            std::lock_guard<std::mutex> lock(m_);
+            if (!spare_.empty()) {
+                b.ready = spare_.back();
+                spare_.pop_back();
+            }
+        }
+        bool ok = hipGetDevice(&dev) == hipSuccess &&
+                  (b.ready || hipEventCreateWithFlags(&b.ready, hipEventDisableTiming) == hipSuccess) &&
+                  hipEventRecord(b.ready, s) == hipSuccess;
+        b.dev = dev;
+        if (!ok) {   // not reusable in order: freed after everything queued on the device
+            (void)hipGetLastError();
+            (void)hipDeviceSynchronize();
+            (void)hipFree(p);
+            if (b.ready) (void)hipEventDestroy(b.ready);
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lock(m_);
+            blocks_.push_back(b);
+            cached_ += n;
+            while (cached_ > kCap && !blocks_.empty()) {
+                out.push_back(blocks_.front());
+                cached_ -= blocks_.front().n;
+                blocks_.erase(blocks_.begin());
+            }
+        }
+        for (const Block& o : out) evict(o);
+    }
+    static BlockCache& get() {
+        static BlockCache* c = new BlockCache;   // kept until the process ends
+        return *c;
+    }
+
+private:
+    static constexpr size_t kCap = (size_t)4 << 30;
+    struct Block {
+        void* p;
+        size_t n;
+        hipEvent_t ready;
+        int dev;
+    };
+    static void evict(const Block& b) {
+        (void)hipEventSynchronize(b.ready);
+        (void)hipFree(b.p);
+        (void)hipEventDestroy(b.ready);
+    }
+    std::mutex m_;
+    std::vector<Block> blocks_;
+    std::vector<hipEvent_t> spare_;
+    size_t cached_ = 0;
+};
+}  // namespace
+
+PoolScope::PoolScope(hipStream_t s) : s_(s), prev_(t_scope) { t_scope = this; }
+PoolScope::~PoolScope() { t_scope = prev_; }
+const PoolScope* PoolScope::current() { return t_scope; }
+
 void DevBuf::alloc(size_t bytes) {
     release();
     if (bytes == 0) bytes = 16;
-    hipError_t e = hipMalloc(&p_, bytes);
-    if (e != hipSuccess) {
+    const PoolScope* sc = PoolScope::current();
+    size_t n = bytes;
+    if (sc && (p_ = BlockCache::get().take(bytes, sc->stream(), n))) {
+        n_ = n;
+        pooled_ = true;
+        return;
+    }
+    if (hipMalloc(&p_, bytes) != hipSuccess) {
         p_ = nullptr;
         (void)hipGetLastError();
         throw std::bad_alloc();
     }
     n_ = bytes;
+    pooled_ = sc != nullptr;
 }
 void DevBuf::release() {
-    if (p_) (void)hipFree(p_);
+    if (p_) {
+        if (!pooled_) {
+            (void)hipFree(p_);
+        } else if (const PoolScope* sc = PoolScope::current()) {
+            BlockCache::get().give(p_, n_, sc->stream());
+        } else {   // no order known: reusable after everything queued on the device
+            (void)hipDeviceSynchronize();
+            BlockCache::get().give(p_, n_, nullptr);
+        }
+    }
     p_ = nullptr;
     n_ = 0;
+    pooled_ = false;
 }
 void DevBuf::upload(const void* host, size_t bytes, hipStream_t s) {
     if (bytes > n_) alloc(bytes);
@@ -681,6 +796,10 @@ Engine::Engine(const Plan& p, const double* H, long long nH, int Jg, hipStream_t
         // their z reads; their bin blocks (256 bins each) run next to each other
         if (!p.up && !Jg_) kchain_ = (int)((std::llabs(h - (long long)p.Mb) + 128) / 256);
         if (const char* v = std::getenv("KBSRC_MID_ORDER"); v && std::atoi(v) == 0) kchain_ = 0;   // test switch
+        // task groups of 4 (mid_order): a chained block's repeated z reads come back 4 workgroups later (groups of
+        // 2, 8, 16 and tasks fastest measured slower on 96 kHz -> 44.1 kHz, exact)
+        tgroup_ = kchain_ > 0 ? 4 : 0;
+        if (const char* v = std::getenv("KBSRC_MID_TGROUP")) tgroup_ = std::max(0, std::atoi(v));   // test switch
     }
     // H in the element format of the engine: (hi, lo) pairs or FP64
     H_.alloc((size_t)nH * (dd_ ? 16 : 8));
@@ -730,6 +849,12 @@ void Engine::execute(const double* xx, double* out, hipStream_t s, PassTimes* ti
     run(xx, out, s, times, split, w);
 }
 
+void Engine::execute_tasks(const double* xx, double* out, hipStream_t s, long long t0, long long t1) {
+    if (t0 < 0 || t1 > ntask_ || t0 > t1) throw std::invalid_argument("task range");
+    const Work w{{w_[0].d(), w_[1].d(), w_[2].d(), w_[3].d()}, {g_[0].d(), g_[1].d()}, tb_};
+    run(xx, out, s, nullptr, 0, w, t0, t1);
+}
+
 namespace {
 constexpr size_t kCarveAlign = 256;
 size_t carve_bytes(size_t tasks, size_t per_task) { return (tasks * per_task + kCarveAlign - 1) / kCarveAlign * kCarveAlign; }
@@ -765,7 +890,9 @@ void Engine::execute_in(const double* xx, double* out, hipStream_t s, double* me
     run(xx, out, s, nullptr, 0, w);
 }
 
-void Engine::run(const double* xx, double* out, hipStream_t s, PassTimes* times, int split, const Work& w) {
+void Engine::run(const double* xx, double* out, hipStream_t s, PassTimes* times, int split, const Work& w,
+                 long long tbeg, long long tend) {
+    if (tend < 0) tend = ntask_;
     PassTimes local;
     const int pol_four = dd_ ? POL_DDS : POL_F64;
     const bool fused = !split && use_fixed() && env_flag("KBSRC_FUSED") && !Jg_ && fw_.c.four && iv_.c.four &&
@@ -777,8 +904,8 @@ void Engine::run(const double* xx, double* out, hipStream_t s, PassTimes* times,
     double* W1 = w.w[1];
     double* W2 = w.w[2];
     double* W3 = w.w[3];
-    for (long long t0 = 0; t0 < ntask_; t0 += w.tb) {
-        const int nt = (int)std::min<long long>(w.tb, ntask_ - t0);
+    for (long long t0 = tbeg; t0 < tend; t0 += w.tb) {
+        const int nt = (int)std::min<long long>(w.tb, tend - t0);
         a.ntask = nt;
         a.task0 = t0;
         if (fused) {
@@ -801,6 +928,7 @@ void Engine::run(const double* xx, double* out, hipStream_t s, PassTimes* times,
         m.h = fw_.h; m.hinv = iv_.h; m.Nb = p_.Nb; m.Mb = p_.Mb; m.K = p_.Mb / 2 + 1;
         m.jmax = std::min(p_.Nb / 2, p_.Mb - 1); m.up = p_.up; m.ntask = nt; m.Jg = Jg_;
         m.kchain = kchain_;
+        m.tgroup = tgroup_;
         timed(c, &local.mid, [&] { hip_check(launch_mid(dd_ ? POL_DDA : POL_F64, m, s), "mid kernel"); });
         // inverse: z' (W1) -> output samples
         double* const si[3] = {W0, W2, W3};
@@ -867,6 +995,7 @@ bool Engine::describe(const double* xx, double* out, EngineSteps& steps) {
     m.h = fw_.h; m.hinv = iv_.h; m.Nb = p_.Nb; m.Mb = p_.Mb; m.K = p_.Mb / 2 + 1;
     m.jmax = std::min(p_.Nb / 2, p_.Mb - 1); m.up = p_.up; m.ntask = (int)ntask_; m.Jg = Jg_;
     m.kchain = kchain_;
+    m.tgroup = tgroup_;
     steps.mid_policy = dd_ ? POL_DDA : POL_F64;
     double* const si[3] = {W0, W2, W3};
     c.collect = &steps.inv;

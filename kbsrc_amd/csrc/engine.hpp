@@ -20,6 +20,27 @@ struct HipError : std::runtime_error {
 };
 void hip_check(hipError_t e, const char* what);
 
+// Stream-ordered allocation scope of the calling thread. While one is active, a DevBuf released on this thread goes
+// to the library's block cache instead of hipFree, reusable once the work queued on the scope's stream so far is
+// done, and a DevBuf allocated takes a cached block when one fits (its stream waiting for that block's last work)
+// before it calls hipMalloc. A release then neither waits for the device nor pays hipFree: for a conversion that
+// allocates and frees while its own copies and kernels run, and for the next conversion, which finds its buffers
+// cached. The owner of a scope keeps every use of such memory ordered before the release (on the scope's stream,
+// or synchronised before it). Scopes nest; the innermost one is current.
+class PoolScope {
+public:
+    explicit PoolScope(hipStream_t s);
+    ~PoolScope();
+    PoolScope(const PoolScope&) = delete;
+    PoolScope& operator=(const PoolScope&) = delete;
+    static const PoolScope* current();
+    hipStream_t stream() const { return s_; }
+
+private:
+    hipStream_t s_;
+    const PoolScope* prev_;
+};
+
 // Device allocation owned by RAII.
 class DevBuf {
 public:
@@ -28,9 +49,13 @@ public:
     ~DevBuf() { release(); }
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
-    DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_) { o.p_ = nullptr; o.n_ = 0; }
+    DevBuf(DevBuf&& o) noexcept : p_(o.p_), n_(o.n_), pooled_(o.pooled_) { o.p_ = nullptr; o.n_ = 0; }
     DevBuf& operator=(DevBuf&& o) noexcept {
-        if (this != &o) { release(); p_ = o.p_; n_ = o.n_; o.p_ = nullptr; o.n_ = 0; }
+        if (this != &o) {
+            release();
+            p_ = o.p_; n_ = o.n_; pooled_ = o.pooled_;
+            o.p_ = nullptr; o.n_ = 0;
+        }
         return *this;
     }
     void alloc(size_t bytes);
@@ -42,6 +67,7 @@ public:
 private:
     void* p_ = nullptr;
     size_t n_ = 0;
+    bool pooled_ = false;   // allocated in a PoolScope: released to the block cache
 };
 
 // Element format of a policy's device tables: dd = {re.hi, im.hi, re.lo, im.lo}, FP64 = {re, im}.
@@ -174,6 +200,9 @@ public:
     // split > 0: the split-precision form (exact-mode plans only; not bit-exact): sub-FFTs whose first two stages
     // are radix 4 run them as one int8-MFMA radix-16 stage with `split` base-128 slices per operand.
     void execute(const double* xx, double* out, hipStream_t s, PassTimes* times = nullptr, int split = 0);
+    // The plain execute of tasks [t0, t1) only (task t: channel t / nblk, block t % nblk), which write the output
+    // samples of their blocks; the other tasks' output is left as it is.
+    void execute_tasks(const double* xx, double* out, hipStream_t s, long long t0, long long t1);
     // The plain execute with every work buffer carved from the caller's device memory [mem, mem + bytes) instead of
     // the engine's own: tasks run in as many launches as that memory needs, with the same bits. Allocates nothing.
     // Throws std::invalid_argument when one task's buffers do not fit.
@@ -193,7 +222,8 @@ private:
         double* g[2];   // global-pass scratch (sub-FFTs longer than an LDS tile), or null
         int tb;         // tasks per launch
     };
-    void run(const double* xx, double* out, hipStream_t s, PassTimes* times, int split, const Work& w);
+    void run(const double* xx, double* out, hipStream_t s, PassTimes* times, int split, const Work& w,
+             long long t0 = 0, long long t1 = -1);
     RunCtx context(const double* xx, double* out, hipStream_t s, int split, const Work& w);
     Plan p_;
     bool dd_;
@@ -201,6 +231,7 @@ private:
     long long ntask_ = 0, ts_ = 0;   // ts_: doubles per task in the work buffers
     long long zkeep_ = -1;           // forward spectrum bins above this (and their split partners) go unstored
     int kchain_ = 0;                 // MidArgs::kchain
+    int tgroup_ = 0;                 // MidArgs::tgroup
     const long long* tdesc_ = nullptr;
     int tb_ = 1;
     DeviceConsts k_;

@@ -40,6 +40,21 @@ template <class A> __device__ __forceinline__ void gstore(double* base, long lon
         reinterpret_cast<double2*>(base)[i] = make_double2(v.re, v.im);
     }
 }
+// The split/fold kernel's output, stored non-temporally: it does not take L2 space from the z bins that the next
+// bin blocks read again. (The same stores in the pass kernels, whose element halves are written by separate
+// instructions, made the forward row pass 1.45x slower; non-temporal loads of the passes' sources were slower too.)
+__device__ __forceinline__ void st_stream2(double* p, double a, double b) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    __builtin_nontemporal_store(d2{a, b}, reinterpret_cast<d2*>(p));
+}
+template <class A> __device__ __forceinline__ void gstore_stream(double* base, long long i, Cx<A> v) {
+    if constexpr (A::kDD) {
+        st_stream2(base + 4 * i, v.re.hi, v.im.hi);
+        st_stream2(base + 4 * i + 2, v.re.lo, v.im.lo);
+    } else {
+        st_stream2(base + 2 * i, v.re, v.im);
+    }
+}
 template <class A> __device__ __forceinline__ Val<A> gload_real(const double* base, long long i) {
     if constexpr (A::kDD) {
         const double2 v = reinterpret_cast<const double2*>(base)[i];
@@ -393,32 +408,58 @@ __device__ __forceinline__ int mid_block(int q, int nk, int c) {
     return j + i * c;
 }
 
+// Task and launch-order bin block of workgroup L. tgroup = 0: tasks fastest (every task of a bin block, then the next
+// block). tgroup > 0: tasks in groups of tgroup; a group runs all its bin blocks, its tasks fastest, before the next
+// group. With the chained block order, a block's z reads that the next block repeats then come back tgroup
+// workgroups later, and with the launch's workgroups dealt to the XCDs in contiguous ranges (k_mid), on the same
+// XCD: from its L2, not from HBM. H and the split twiddles are read once per group instead of once per launch.
+__device__ __forceinline__ void mid_order(const MidArgs& a, int nk, unsigned L, int& task, int& q) {
+    if (a.tgroup <= 0) {
+        task = (int)(L % (unsigned)a.ntask);
+        q = (int)(L / (unsigned)a.ntask);
+        return;
+    }
+    const unsigned per = (unsigned)a.tgroup * (unsigned)nk;   // workgroups of a full group
+    const int grp = (int)(L / per), t0 = grp * a.tgroup, gs = min(a.tgroup, a.ntask - t0);
+    const unsigned r = L - (unsigned)grp * per;
+    q = (int)(r / (unsigned)gs);
+    task = t0 + (int)(r - (unsigned)q * (unsigned)gs);
+}
+
 template <class A> __device__ __forceinline__ void mid_body(const MidArgs& a, unsigned bid) {
-    // task-fastest workgroup order: the tasks of one bin range run together, so the H and split-twiddle slices
-    // they share stay in L2
-    const int task = (int)(bid % (unsigned)a.ntask);
     const int nk = (a.hinv / 2 + 1 + (int)blockDim.x - 1) / (int)blockDim.x;
-    const int k = mid_block((int)(bid / (unsigned)a.ntask), nk, a.kchain) * blockDim.x + threadIdx.x;
+    int task, q;
+    mid_order(a, nk, bid, task, q);
+    const int k = mid_block(q, nk, a.kchain) * blockDim.x + threadIdx.x;
     if (k > a.hinv / 2) return;
     const double* z = a.z + task * a.z_tstride;
     double* zi = a.zi + task * a.zi_tstride;
     const int j = a.hinv - k;
     const Cx<A> Yk = mid_bin<A>(a, z, k), Yj = mid_bin<A>(a, z, j);
-    gstore<A>(zi, k, pre_v<A>(Yk, Yj, gload<A>(a.wi, k)));
+    gstore_stream<A>(zi, k, pre_v<A>(Yk, Yj, gload<A>(a.wi, k)));
     if (k && j != k) {
         if constexpr (!A::kDD) {
-            gstore<A>(zi, j, pre_v<A>(Yj, Yk, gload<A>(a.wi, j)));
+            gstore_stream<A>(zi, j, pre_v<A>(Yj, Yk, gload<A>(a.wi, j)));
         } else {   // dd: the partner from the same E and split (src/dsp/fft.hpp:452-456)
             const Cx<A> B = c_conj<A>(Yj);
             const Cx<A> E = c_add<A>(Yk, B), sp = c_sub<A>(Yk, B);
             const Cx<A> Ej = c_conj<A>(E);
             const Cx<A> Dj = c_mul<A>(Cx<A>{A::neg(sp.re), sp.im}, c_conj<A>(gload<A>(a.wi, j)));
-            gstore<A>(zi, j, c_conj<A>(Cx<A>{A::sub(Ej.re, Dj.im), A::add(Ej.im, Dj.re)}));
+            gstore_stream<A>(zi, j, c_conj<A>(Cx<A>{A::sub(Ej.re, Dj.im), A::add(Ej.im, Dj.re)}));
         }
     }
 }
 
-template <class A> __global__ __launch_bounds__(256) void k_mid(MidArgs a) { mid_body<A>(a, blockIdx.x); }
+// Workgroups are dealt to the 8 XCDs round-robin; with tgroup > 0 each XCD takes a contiguous range of the launch
+// order instead, so that a task group's blocks share one L2.
+template <class A> __global__ __launch_bounds__(256) void k_mid(MidArgs a) {
+    unsigned L = blockIdx.x;
+    if (a.tgroup > 0) {
+        const unsigned n = gridDim.x, full = n & ~7u;
+        if (L < full) L = (L & 7u) * (full >> 3) + (L >> 3);
+    }
+    mid_body<A>(a, L);
+}
 
 template <class A>
 __global__ __launch_bounds__(256) void k_mid_multi(const MidArgs* __restrict__ d, const int* __restrict__ starts, int n) {

@@ -108,6 +108,7 @@ struct MidArgs {
     int h = 0, hinv = 0, Nb = 0, Mb = 0, K = 0, jmax = 0, up = 0, ntask = 0;
     int Jg = 0;                     // > 0: general fold of H[0..Jg] on the finer kernel grid (near-unity ratios)
     int kchain = 0;                 // > 0: bin blocks kchain apart run one after another (mid_block)
+    int tgroup = 0;                 // > 0: tasks in groups of tgroup, each group's bin blocks in turn (mid_order)
 };
 
 // Bluestein (chirp-z) steps of a complex FFT whose length n has a prime factor above 7 (src/dsp/fft.hpp:228-278),

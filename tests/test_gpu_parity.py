@@ -222,15 +222,19 @@ def test_unstored_bins_change_nothing(rin, rout, mode, length, monkeypatch):
                                            (192000, 44100, "exact")])
 def test_split_fold_block_order_changes_nothing(rin, rout, mode, monkeypatch):
     """Down conversions run the split/fold kernel's bin blocks abs(h - Mb) apart back to back (the threads there
-    share their z reads); the launch order changes no value: same bits as the plain block order
-    (KBSRC_MID_ORDER=0)."""
+    share their z reads), in groups of 16 tasks; the launch order changes no value: same bits as the plain block
+    order (KBSRC_MID_ORDER=0), as tasks fastest (KBSRC_MID_TGROUP=0) and as groups of 3 tasks (several groups, the
+    last one partial)."""
     x = np.random.default_rng(rin + 7 * rout).uniform(-0.5, 0.5, (3 * rin, 2))
     say(f"split/fold block order {rin} -> {rout} {mode}")
-    monkeypatch.delenv("KBSRC_MID_ORDER", raising=False)
+    for v in ("KBSRC_MID_ORDER", "KBSRC_MID_TGROUP"):
+        monkeypatch.delenv(v, raising=False)
     y = gpu().resample(x, rin, rout, mode)
-    monkeypatch.setenv("KBSRC_MID_ORDER", "0")
-    z = gpu().resample(x, rin, rout, mode)
-    assert y.tobytes() == z.tobytes()
+    for var, val in (("KBSRC_MID_ORDER", "0"), ("KBSRC_MID_TGROUP", "0"), ("KBSRC_MID_TGROUP", "3")):
+        monkeypatch.setenv(var, val)
+        z = gpu().resample(x, rin, rout, mode)
+        monkeypatch.delenv(var)
+        assert y.tobytes() == z.tobytes(), (var, val)
 
 
 GLOBAL_CASES = [c for c in CASES if c[3] >= 1000 and c[2] <= 2][::2]
