@@ -653,7 +653,12 @@ def bench_split(args):
         return (time.perf_counter() - t0) * 1e3 / args.steps
 
     xs = x[: sample * 64].cpu().numpy().reshape(sample, 16, 4)
+    inwave_ms = timed(kbsrc_amd.plain_dft16_inwave)
+    ref.copy_(y)
     plain_ms = timed(kbsrc_amd.plain_dft16)
+    inwave_same = bool(torch.equal(y, ref))
+    print(f"plain dd radix-16: LDS exchange {plain_ms:.3f} ms, in-wave exchange {inwave_ms:.3f} ms "
+          f"(same bits: {inwave_same})", file=sys.stderr, flush=True)
     ref.copy_(y)
     e_plain = max(errors(xs, ref[: sample * 64].cpu().numpy().reshape(sample, 16, 4), sample))
     rows = {}
@@ -687,6 +692,8 @@ def bench_split(args):
                    "slices_at_2^-100": s100, "parallelism": "single GPU"},
         "plain_dd": {"ms": round(plain_ms, 4), "ps_per_point": round(plain_ms * 1e9 / (16 * nvec), 2),
                      "err_vs_256bit_log2": round(float(np.log2(e_plain)), 1)},
+        "plain_dd_inwave_exchange": {"ms": round(inwave_ms, 4), "speed_vs_lds_exchange": round(plain_ms / inwave_ms, 3),
+                                     "same_bits": inwave_same},
         "split": rows,
         "conversion": conv,
     }

@@ -165,6 +165,10 @@ int kbsrc_tuned_shapes(void);   /* pass shapes measured so far in this process *
  * per device (and slice count) and kept for the process. */
 int kbsrc_split_dft16(const double* d_x, long long nvec, int slices, double* d_y, void* stream);
 int kbsrc_plain_dft16(const double* d_x, long long nvec, double* d_y, void* stream);
+/* The plain dd stage with the exchange between its two radix-4 stages done inside the wave (v_permlane32_swap)
+ * instead of through LDS (no counterpart in the reference: a measured alternative of the plain pass's LDS round
+ * trip; same bits as kbsrc_plain_dft16). Synchronous. */
+int kbsrc_plain_dft16_inwave(const double* d_x, long long nvec, double* d_y, void* stream);
 
 /* ---- The kernel design on the GPU (host buffers in and out; the plan calls do the same device to device). */
 

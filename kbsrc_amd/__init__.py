@@ -21,7 +21,7 @@ except ImportError:
 __all__ = ["resample", "output_frames", "Plan", "Stats", "version", "library_path", "FAST", "EXACT",
            "KbsrcError", "PLAN_FIELDS", "kernel_taps", "kernel_spectrum", "pack", "PACK_FORMATS", "Batch",
            "resample_batch", "tuned_shapes",
-           "split_dft16", "plain_dft16", "Graph"]
+           "split_dft16", "plain_dft16", "plain_dft16_inwave", "Graph"]
 
 FAST, EXACT, LONG_DOUBLE = 0, 1, 2
 OK, EINVAL, ERUNTIME, ENOMEM = 0, 1, 2, 3
@@ -88,6 +88,7 @@ def _load():
         "kbsrc_tuned_shapes": (I, []),
         "kbsrc_split_dft16": (I, [P, LL, I, P, P]),
         "kbsrc_plain_dft16": (I, [P, LL, P, P]),
+        "kbsrc_plain_dft16_inwave": (I, [P, LL, P, P]),
         "kbsrc_kernel_taps": (I, [I, LL, LL, D, P]),
         "kbsrc_kernel_spectrum": (I, [P, I, I, D, P]),
     }
@@ -130,6 +131,11 @@ def split_dft16(d_x: int, nvec: int, slices: int, d_y: int, stream: int = 0):
 def plain_dft16(d_x: int, nvec: int, d_y: int, stream: int = 0):
     """The same DFTs by the engine's plain dd radix-4 x 4 pass."""
     _check(_lib.kbsrc_plain_dft16(ctypes.c_void_p(d_x), nvec, ctypes.c_void_p(d_y), ctypes.c_void_p(stream)))
+
+
+def plain_dft16_inwave(d_x: int, nvec: int, d_y: int, stream: int = 0):
+    """The same plain dd pass with its stage exchange inside the wave instead of LDS (same bits)."""
+    _check(_lib.kbsrc_plain_dft16_inwave(ctypes.c_void_p(d_x), nvec, ctypes.c_void_p(d_y), ctypes.c_void_p(stream)))
 
 
 def tuned_shapes() -> int:

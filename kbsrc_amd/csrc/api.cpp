@@ -1169,6 +1169,15 @@ int kbsrc_plain_dft16(const double* d_x, long long nvec, double* d_y, void* stre
     });
 }
 
+int kbsrc_plain_dft16_inwave(const double* d_x, long long nvec, double* d_y, void* stream) {
+    return guarded([&] {
+        if (nvec < 0 || (nvec > 0 && (!d_x || !d_y))) throw std::invalid_argument("bad input buffer");
+        in_nearest_mode(true, [&] {
+            plain_dft16_inwave(d_x, nvec, d_y, static_cast<hipStream_t>(stream));
+        });
+    });
+}
+
 int kbsrc_split_dft16(const double* d_x, long long nvec, int slices, double* d_y, void* stream) {
     return guarded([&] {
         if (nvec < 0 || (nvec > 0 && (!d_x || !d_y))) throw std::invalid_argument("bad input buffer");

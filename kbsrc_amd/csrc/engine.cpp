@@ -739,6 +739,14 @@ void plain_dft16(const double* x, long long nvec, double* y, hipStream_t s) {
     hip_check(hipStreamSynchronize(s), "radix-16 sync");
 }
 
+void plain_dft16_inwave(const double* x, long long nvec, double* y, hipStream_t s) {
+    if (nvec < 1) return;
+    if (nvec > 0x7fffffffLL / 16) throw std::invalid_argument("bad input buffer");
+    const Dft16Tables& t = dft16_tables(s);
+    hip_check(launch_dft16_inwave(x, nvec, y, t.sf.tw.d(), t.k.odd_dd.d(), s), "in-wave radix-16 kernel");
+    hip_check(hipStreamSynchronize(s), "radix-16 sync");
+}
+
 DevBuf kernel_taps_gpu(long long L, long long a, long long b, double beta, hipStream_t s) {
     TapArgs t;
     t.L = L; t.a = a; t.b = b; t.beta = beta;

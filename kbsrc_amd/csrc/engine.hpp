@@ -175,6 +175,8 @@ int tuned_shapes();   // pass shapes measured so far in this process
 // nvec 16-point complex DFTs of dd vectors (plan element format, device) by the engine's dd Stockham pass: the
 // plain baseline of the split-precision form. Synchronous.
 void plain_dft16(const double* x, long long nvec, double* y, hipStream_t s);
+// the same stage with its exchange inside the wave instead of LDS (same bits). Synchronous.
+void plain_dft16_inwave(const double* x, long long nvec, double* y, hipStream_t s);
 // the split matrix slices (split_matrix_digits) on the current device, made on first use and kept for the process
 const long* split_dft16_digits(int S, hipStream_t s);
 

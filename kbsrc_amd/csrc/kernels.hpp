@@ -222,6 +222,10 @@ hipError_t launch_pass_multi(int policy, int ld, int st, const PassArgs* d, cons
 hipError_t launch_mid_multi(int policy, const MidArgs* d, const int* starts, int n, int total, hipStream_t s);
 int mid_workgroups(const MidArgs& a);   // workgroups of launch_mid for a, -1 if too many
 hipError_t launch_stage(const StageArgs& a, hipStream_t s);
+// The plain dd radix-16 stage of nvec 16-point vectors with its stage exchange inside the wave (kernels_inwave.hip):
+// tw and oddc the 16-point sub-FFT's stage twiddles and the radix constants (dd format). Same bits as the LDS pass.
+hipError_t launch_dft16_inwave(const double* x, long long nvec, double* y, const double* tw, const double* oddc,
+                               hipStream_t s);
 // policy: POL_DDA or POL_F64; kind: the LD kind (pre) or ST kind (post)
 hipError_t launch_blue(int policy, int phase, int kind, const BlueArgs& a, hipStream_t s);
 hipError_t launch_spec_split(const SpecArgs& a, hipStream_t s);

@@ -63,3 +63,15 @@ def test_plain_and_split_errors_against_256_bit():
         prev = e
     print({s: round(float(np.log2(e)), 1) for s, e in worst.items()}, "plain", round(float(np.log2(e_plain)), 1))
     assert worst[16] < 2.0 ** -100
+
+
+@pytest.mark.parametrize("nvec", [1, 31, 32, 33, 128, 1000, 4099])
+def test_inwave_exchange_gives_the_lds_pass_bits(nvec):
+    """The plain radix-16 stage with its stage exchange inside the wave (v_permlane32_swap) computes the same
+    operations as the LDS pass: the same bits on every vector, for counts that fill waves partly (32 vectors per
+    wave, 128 per workgroup) and wholly, and the DFT within 2^-100 of a 256-bit one."""
+    x = dd_vectors(nvec, 11 + nvec)
+    want = run(gpu().plain_dft16, x)
+    got = run(gpu().plain_dft16_inwave, x)
+    assert got.tobytes() == want.tobytes()
+    assert max(errors(x, got, min(nvec, 4))) < 2.0 ** -100
