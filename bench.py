@@ -194,6 +194,49 @@ def summarize_rocprof(d, mode, steps=20, warmup=5):
     return 0
 
 
+def summarize_split(d):
+    """Reads <d> (rocprofv3 --kernel-trace --stats of the split workload, csv) and prints the radix-16 stage
+    summary kept under profiles/: per kernel, launches and durations, and the per-launch times of the conversion's
+    320-point forward row passes, plain and split."""
+    rows = _csv_rows(os.path.join(d, "run_kernel_trace.csv"))
+    dur = {}
+    for r in rows:
+        dur.setdefault(r["Kernel_Name"], []).append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    gb = (1 << 22) * 16 * 64 / 1e9   # 32 B in + 32 B out per point (one dd complex each way)
+    stage, conv = [], []
+    for name, v in dur.items():
+        m = re.search(r"k_split_dft16<(\d+)>", name)
+        if m:
+            stage.append((2, int(m.group(1)), f"split radix-16, S = {m.group(1)}", v))
+        elif "k_dft16_inwave" in name:
+            stage.append((1, 0, "plain dd radix-16, in-wave exchange", v))
+        elif "Radices<4, 4>," in name:
+            stage.append((0, 0, "plain dd radix-16 pass (2 x radix 4, LDS)", v))
+        elif "Radices<4, 4, 4, 5>" in name and ("k_fixed_split" in name or "k_fixed<" in name):
+            m = re.search(r"k_fixed_split<.*, (\d+)>\(", name)
+            conv.append((int(m.group(1)) if m else 0,
+                         f"split forward row pass, S = {m.group(1)}" if m else "plain forward row pass (320 points)", v))
+    out = ["# Split-precision radix-16 DFT (int8 MFMA slices), the plain dd radix-16 pass and the plain pass with its "
+           "exchange inside the wave:",
+           "# 4,194,304 vectors of 16 complex dd values (67,108,864 points, 32 B in + 32 B out per point = 4.295 GB "
+           "per launch); MI355X, rocprofv3 (csv output).",
+           "# commands, from /tmp with TMPDIR=/tmp, R = the repository root, D = an output folder:",
+           "#   rocprofv3 --kernel-trace --stats -d D -o run --output-format csv -- python3 R/bench.py --workload split "
+           "--steps 3 --warmup 1 --no-cpu",
+           "#   python3 R/bench.py --summarize-split D",
+           "# (the same run also times the 180 s 96 kHz -> 44.1 kHz conversion, plain and in the split form; its row "
+           "passes are listed last)",
+           f"# {'kernel':44s} {'launches':>8s} {'avg_us':>9s} {'min_us':>9s} {'max_us':>9s} {'GB/s(algorithmic)':>18s}"]
+    for _, _, label, v in sorted(stage):
+        avg = sum(v) / len(v)
+        out.append(f"  {label:44s} {len(v):8d} {avg:9.1f} {min(v):9.1f} {max(v):9.1f} {gb / (avg / 1e6):18.0f}")
+    out.append("# conversion row passes (per launch; 214 tasks, sub-FFTs of 320 points):")
+    for _, label, v in sorted(conv):
+        out.append(f"  {label:44s} {len(v):8d} {sum(v) / len(v):9.1f} {min(v):9.1f} {max(v):9.1f}")
+    print("\n".join(out))
+    return 0
+
+
 COUNTER_SETS = (
     ("pmc1", "SQ_WAVES SQ_INSTS_VALU SQ_INSTS_VALU_ADD_F64 SQ_INSTS_VALU_MUL_F64 SQ_INSTS_VALU_FMA_F64 SQ_INSTS_LDS "
              "SQ_INSTS_SALU SQ_ACTIVE_INST_VALU GRBM_GUI_ACTIVE"),
@@ -653,10 +696,11 @@ def bench_split(args):
         return (time.perf_counter() - t0) * 1e3 / args.steps
 
     xs = x[: sample * 64].cpu().numpy().reshape(sample, 16, 4)
-    inwave_ms = timed(kbsrc_amd.plain_dft16_inwave)
-    ref.copy_(y)
     plain_ms = timed(kbsrc_amd.plain_dft16)
+    ref.copy_(y)
+    inwave_ms = timed(kbsrc_amd.plain_dft16_inwave)
     inwave_same = bool(torch.equal(y, ref))
+    plain_ms = min(plain_ms, timed(kbsrc_amd.plain_dft16))   # again, after the other: clocks as warm for both
     print(f"plain dd radix-16: LDS exchange {plain_ms:.3f} ms, in-wave exchange {inwave_ms:.3f} ms "
           f"(same bits: {inwave_same})", file=sys.stderr, flush=True)
     ref.copy_(y)
@@ -788,6 +832,8 @@ def main():
                     help="single: the stereo 96 kHz -> 44.1 kHz 180 s job (default); batch: 64 mixed jobs in one "
                          "batch; split: int8-MFMA split-precision radix-16 DFTs against the plain dd pass; extreme: "
                          "10 MHz -> 48 kHz 30 s and 8-channel 384 kHz -> 44.1 kHz 600 s at --len 0.5")
+    ap.add_argument("--summarize-split", metavar="DIR",
+                    help="print the radix-16 stage summary of a rocprofv3 kernel trace of the split workload in DIR")
     ap.add_argument("--summarize-counters", metavar="DIR",
                     help="print the per-pass VALU/LDS counter summary of rocprofv3 csv output under DIR/{pmc1,pmc2}")
     ap.add_argument("--summarize-rocprof", metavar="DIR",
@@ -797,6 +843,8 @@ def main():
         return summarize_rocprof(args.summarize_rocprof, args.mode, args.steps, args.warmup)
     if args.summarize_counters:
         return summarize_counters(args.summarize_counters, args.mode)
+    if args.summarize_split:
+        return summarize_split(args.summarize_split)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}): the path would shard by block ranges with "
               "no exchange step, and nothing here drives a second GPU; run with --gpus 1", file=sys.stderr)

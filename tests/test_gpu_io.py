@@ -202,3 +202,29 @@ def test_host_buffer_conversions_from_two_threads():
         both = list(ex.map(lambda x: kbsrc_amd.resample(x, 96000, 44100, "exact"), xs))
     for a, b in zip(alone, both):
         assert a.tobytes() == b.tobytes()
+
+
+def test_cached_device_blocks_across_conversions():
+    """kbsrc_resample keeps its released device buffers for later calls (each reused only after the work that last
+    used it): conversions of growing, shrinking and other-rate sizes in turn, from one thread and then from three
+    at once, give the bits of the oracle (short ones) or of the same conversion run first (long ones)."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    import kbsrc_amd
+    cases = [(96000, 44100, 2, 96000 * 30), (44100, 48000, 1, 5000), (96000, 44100, 2, 96000 * 60),
+             (48000, 96000, 3, 3000), (96000, 44100, 2, 96000 * 10), (44100, 48000, 2, 44100 * 40)]
+    xs = [np.random.default_rng(70 + k).uniform(-0.5, 0.5, (f, ch)) for k, (_, _, ch, f) in enumerate(cases)]
+    print("cached device blocks: six conversions in turn, then three threads", flush=True)
+    first = []
+    for (rin, rout, ch, f), x in zip(cases, xs):
+        y = kbsrc_amd.resample(x, rin, rout, "exact")
+        if f <= 5000:
+            assert y.tobytes() == H.oracle_resample(x.reshape(-1), ch, rin, rout, "exact").tobytes()
+        first.append(y)
+    for (rin, rout, _, _), x, y in zip(cases, xs, first):
+        assert kbsrc_amd.resample(x, rin, rout, "exact").tobytes() == y.tobytes()
+    with ThreadPoolExecutor(3) as ex:
+        again = list(ex.map(lambda k: kbsrc_amd.resample(xs[k], cases[k][0], cases[k][1], "exact"),
+                            [2, 0, 5, 1, 4, 3]))
+    for k, y in zip([2, 0, 5, 1, 4, 3], again):
+        assert y.tobytes() == first[k].tobytes(), k
