@@ -396,6 +396,21 @@ def batch_jobs(n=64, seed=4):
     return jobs
 
 
+def batch_jobs_short(n=256, seed=9):
+    """256 short jobs: lengths uniform in [0.5, 2] s, 1 or 2 channels, rate pairs from BATCH_RATES both ways, signals
+    as in batch_jobs."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    jobs = []
+    for j in range(n):
+        secs = float(rng.uniform(0.5, 2.0))
+        ch = int(rng.choice([1, 2]))
+        ri, ro = (int(v) for v in rng.choice(BATCH_RATES, 2, replace=False))
+        jobs.append({"frames": int(round(secs * ri)), "channels": ch, "rate_in": ri, "rate_out": ro,
+                     "kind": "noise" if j % 2 == 0 else "sines", "seed": seed * 1000 + j})
+    return jobs
+
+
 def batch_signal(job, max_frames=None):
     """The job's interleaved (frames, channels) float64 signal, generated on the GPU (torch)."""
     import numpy as np
@@ -441,7 +456,7 @@ def bench_batch(args):
     import torch
 
     import kbsrc_amd
-    jobs = batch_jobs()
+    jobs = batch_jobs_short() if args.batch_set == "short" else batch_jobs()
     t0 = time.perf_counter()
     b = kbsrc_amd.Batch([(j["frames"], j["channels"], j["rate_in"], j["rate_out"]) for j in jobs], "exact")
     t_design = time.perf_counter() - t0
@@ -531,7 +546,8 @@ def bench_batch(args):
                    "sample": "the first 4 jobs cut to 20 s each (" + "; ".join(names) +
                              f"): reference engine resample time {eng:.3f} s, design and file I/O excluded"}
     line = {
-        "metric": "batched exact-mode output Msamples/s (64 mixed jobs)",
+        "metric": "batched exact-mode output Msamples/s (" + ("256 short jobs" if args.batch_set == "short"
+                                                               else "64 mixed jobs") + ")",
         "value": round(value, 2),
         "unit": "output Msamples/s",
         "n_gpus": 1,
@@ -543,8 +559,10 @@ def bench_batch(args):
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "64 jobs, 0.5-300 s, 1/2/6/8 channels, rate pairs among 44.1/48/88.2/96/176.4/192 kHz "
-                               "both ways, half noise half 8 sinusoids, exact mode, len 0.2, beta 40, LPC on",
+        "config": {"workload": ("256 jobs, 0.5-2 s, 1/2 channels" if args.batch_set == "short" else
+                                "64 jobs, 0.5-300 s, 1/2/6/8 channels") +
+                               ", rate pairs among 44.1/48/88.2/96/176.4/192 kHz both ways, half noise half 8 "
+                               "sinusoids, exact mode, len 0.2, beta 40, LPC on",
                    "jobs": len(jobs), "shapes": b.groups, "input_samples": in_samples, "output_samples": out_samples,
                    "parallelism": "single GPU"},
         "graph_ms_per_step": round(graph_ms, 3),
@@ -826,6 +844,9 @@ def main():
     ap.add_argument("--no-split", action="store_true", help="skip timing the split-precision form")
     ap.add_argument("--profile-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (calibrates HBM byte counters under rocprofv3 --pmc)")
+    ap.add_argument("--batch-set", default="long", choices=["long", "short"],
+                    help="batch workload: long = 64 jobs of 0.5-300 s, 1/2/6/8 channels (default); short = 256 jobs "
+                         "of 0.5-2 s, 1/2 channels")
     ap.add_argument("--tune", action="store_true",
                     help="measure kernel shapes once before timing (Plan.tune / Batch.tune; same output bits)")
     ap.add_argument("--workload", default="single", choices=["single", "batch", "split", "extreme"],

@@ -852,7 +852,11 @@ int kbsrc_batch_create(int njobs, const kbsrc_job* jobs, const kbsrc_options* op
                 b->groups[g].jobs.push_back(j);
                 b->groups[g].ntask += (long long)p.ch * p.nblk;
             }
-            const int nstreams = (int)std::min<size_t>(4, std::max<size_t>(1, b->groups.size()));
+            // shapes run on up to four streams (HIP's default of hardware queues per process is four)
+            int want = 4;
+            if (const char* v = std::getenv("KBSRC_BATCH_STREAMS"))   // tuning knob: streams per batch
+                want = std::clamp(std::atoi(v), 1, 32);
+            const int nstreams = (int)std::min<size_t>((size_t)want, std::max<size_t>(1, b->groups.size()));
             {   // groups to streams, largest first onto the least-loaded stream (work ~ tasks x n log n of both FFTs)
                 std::vector<std::pair<double, int>> w;
                 for (int g = 0; g < (int)b->groups.size(); ++g) {
