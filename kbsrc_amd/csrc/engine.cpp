@@ -86,6 +86,26 @@ public:
         }
         for (const Block& o : out) evict(o);
     }
+    // frees every cached block of the current device (each once its last work is done); true if any was freed
+    bool trim() {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return false;
+        std::vector<Block> out;
+        {
+            std::lock_guard<std::mutex> lock(m_);
+            for (size_t i = 0; i < blocks_.size();) {
+                if (blocks_[i].dev == dev) {
+                    out.push_back(blocks_[i]);
+                    cached_ -= blocks_[i].n;
+                    blocks_.erase(blocks_.begin() + (std::ptrdiff_t)i);
+                } else {
+                    ++i;
+                }
+            }
+        }
+        for (const Block& o : out) evict(o);
+        return !out.empty();
+    }
     static BlockCache& get() {
         static BlockCache* c = new BlockCache;   // kept until the process ends
         return *c;
@@ -125,7 +145,12 @@ void DevBuf::alloc(size_t bytes) {
         pooled_ = true;
         return;
     }
-    if (hipMalloc(&p_, bytes) != hipSuccess) {
+    hipError_t e = hipMalloc(&p_, bytes);
+    if (e != hipSuccess && BlockCache::get().trim()) {   // out of memory: give the cached blocks back, once
+        (void)hipGetLastError();
+        e = hipMalloc(&p_, bytes);
+    }
+    if (e != hipSuccess) {
         p_ = nullptr;
         (void)hipGetLastError();
         throw std::bad_alloc();
