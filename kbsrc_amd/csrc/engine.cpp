@@ -2,6 +2,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -487,20 +488,18 @@ void fill_sub(PassArgs& a, const SubFFT& sf, const DeviceConsts& k, bool as_dd, 
 
 namespace {
 
-// Long twiddle table of CisTable(N) on the GPU: the two short tables go up, every entry is composed there.
+// Long twiddle table of CisTable(N) on the GPU: its two short tables computed there (the host's dd cis, the same
+// bits: CisTable's constructor), then every entry composed from them. Queued on s.
 void device_cis_table(DevBuf& out, long long N, long long count, int mode, long long N2, bool as_dd, hipStream_t s) {
-    CisTable ct(N);
-    std::vector<double> a, b;
-    ct.export_tables(a, b);
-    DevBuf da, db;
-    da.upload(a.data(), a.size() * sizeof(double), s);
-    db.upload(b.data(), b.size() * sizeof(double), s);
+    const long long B = (long long)std::ceil(std::sqrt((double)N)), J = N / B + 1;   // as CisTable::CisTable
+    DevBuf ab((size_t)(J + B) * 32);
+    hip_check(launch_cis_short(N, B, J, ab.d(), ab.d() + 4 * J, s), "twiddle short tables");
     out.alloc((size_t)count * (as_dd ? 32 : 16));
     TableArgs t;
-    t.a = da.d(); t.b = db.d(); t.N = N; t.B = ct.block(); t.count = count; t.N2 = N2; t.mode = mode;
+    t.a = ab.d(); t.b = ab.d() + 4 * J; t.N = N; t.B = B; t.count = count; t.N2 = N2; t.mode = mode;
     t.as_dd = as_dd; t.out = out.d();
     hip_check(launch_cis_table(t, s), "twiddle table");
-    hip_check(hipStreamSynchronize(s), "twiddle table");   // da, db are freed on return
+    // ab's release: in a PoolScope on s it is ordered after the kernels; otherwise hipFree waits for the device
 }
 
 }  // namespace

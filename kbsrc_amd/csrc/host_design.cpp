@@ -122,17 +122,6 @@ void CisTable::get(long long k, dd& c, dd& s) const {
     cis_compose(ac_[j], as_[j], bc_[l], bs_[l], c, s);
 }
 
-void CisTable::export_tables(std::vector<double>& a, std::vector<double>& b) const {
-    a.resize(4 * ac_.size());
-    b.resize(4 * bc_.size());
-    for (size_t j = 0; j < ac_.size(); ++j) {
-        a[4 * j] = ac_[j].hi; a[4 * j + 1] = ac_[j].lo; a[4 * j + 2] = as_[j].hi; a[4 * j + 3] = as_[j].lo;
-    }
-    for (size_t l = 0; l < bc_.size(); ++l) {
-        b[4 * l] = bc_[l].hi; b[4 * l + 1] = bc_[l].lo; b[4 * l + 2] = bs_[l].hi; b[4 * l + 3] = bs_[l].lo;
-    }
-}
-
 // --- Burg edge extension ------------------------------------------------------------------------------------
 std::vector<double> burg_extrap(const double* x, long long len, long long n, int order, int train) {
     std::vector<double> out(n, 0.0);

@@ -36,7 +36,6 @@ public:
     void get(long long k, dd& c, dd& s) const;
     long long block() const { return B_; }
     // the two short tables as {c.hi, c.lo, s.hi, s.lo} per entry: a[j] = cis(jB), b[l] = cis(l)
-    void export_tables(std::vector<double>& a, std::vector<double>& b) const;
 
 private:
     long long N_ = 0, B_ = 1;

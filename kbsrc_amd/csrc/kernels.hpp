@@ -253,6 +253,8 @@ hipError_t launch_split_pass(int policy, int ld, int st, const PassArgs& a, hipS
 bool split_lds_ok(int policy, int ld, int st, const PassArgs& a);
 hipError_t launch_split_lds(int policy, int ld, int st, const PassArgs& a, int threads, hipStream_t s);
 hipError_t launch_cis_table(const TableArgs& a, hipStream_t s);
+// CisTable(N)'s short tables on the GPU (block B = ceil(sqrt(N)), J = N / B + 1 entries in a, B in b; 4 doubles each)
+hipError_t launch_cis_short(long long N, long long B, long long J, double* a, double* b, hipStream_t s);
 // Bluestein chirp b[t] = cis(t^2 mod 2n, 2n), t < n, into bc (plan format) and the chirp sequence v (dd format,
 // m elements, zeroed by the caller): v[t] = v[m - t] = b[t] (src/dsp/fft.hpp:231-236)
 hipError_t launch_chirp(long long n, long long m, int as_dd, double* bc, double* v, hipStream_t s);

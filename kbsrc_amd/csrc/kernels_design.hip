@@ -53,6 +53,23 @@ __device__ __forceinline__ void get_pair(const double* t, long long j, dd& c, dd
     s = dd{v.z, v.w};
 }
 
+// CisTable's two short tables (src/dsp/fft.hpp:29-35) for length N and block B: a[j] = cis(j B, N) for j < J,
+// b[l] = cis(l, N) for l < B, each {c.hi, c.lo, s.hi, s.lo}; the host's dd code (design_math.hpp), the same bits
+__global__ __launch_bounds__(256) void k_cis_short(long long N, long long B, long long J, double* a, double* b) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= J + B) return;
+    dd c, s;
+    double* o;
+    if (t < J) {
+        cis(t * B, N, c, s);
+        o = a + 4 * t;
+    } else {
+        cis(t - J, N, c, s);
+        o = b + 4 * (t - J);
+    }
+    reinterpret_cast<double4*>(o)[0] = double4{c.hi, c.lo, s.hi, s.lo};
+}
+
 // CisTable::get (src/dsp/fft.hpp:36-42) for every element of a table
 __global__ __launch_bounds__(256) void k_cis_table(TableArgs a) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -96,6 +113,12 @@ __global__ __launch_bounds__(256) void k_div_m(const double* V, long long m, int
 dim3 grid_for(long long items) { return dim3((unsigned)((items + 255) / 256)); }
 
 }  // namespace
+
+hipError_t launch_cis_short(long long N, long long B, long long J, double* a, double* b, hipStream_t s) {
+    if (N < 1 || B < 1 || J < 1 || (J + B + 255) / 256 > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_cis_short, grid_for(J + B), dim3(256), 0, s, N, B, J, a, b);
+    return hipGetLastError();
+}
 
 hipError_t launch_taps(const TapArgs& a, hipStream_t s) {
     if (a.L < 0 || (a.L + 256) / 256 > 0x7fffffffLL) return hipErrorInvalidValue;
