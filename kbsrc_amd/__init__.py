@@ -32,7 +32,9 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def library_path() -> str:
-    return os.path.join(_HERE, "libkbsrc_gpu.so")
+    """The engine library: KBSRC_LIB when set (an alternative build, e.g. for side-by-side timing), else the
+    in-tree build."""
+    return os.environ.get("KBSRC_LIB") or os.path.join(_HERE, "libkbsrc_gpu.so")
 
 
 class KbsrcError(RuntimeError):

@@ -33,10 +33,38 @@ template <class A> __device__ __forceinline__ void gst(double* base, long long i
     }
 }
 
-// Radix-R butterfly of the reference's Stockham step on x[0..R): out[u] goes to destination q + u*s.
+// The same with a 32-bit element index: the byte offset i * 32 (dd) or i * 16 (FP64) is formed in 32 bits and
+// added to the base, so that a base that is the same for every lane stays in scalar registers and each access is one
+// 32-bit vector offset from it. Callers keep the byte offset below 2^32.
+template <class A> __device__ __forceinline__ const double2* at32(const double* base, unsigned i) {
+    return reinterpret_cast<const double2*>(reinterpret_cast<const char*>(base) +
+                                            (size_t)(i * (unsigned)(kElt<A> * sizeof(double))));
+}
+template <class A> __device__ __forceinline__ Cx<A> gld32(const double* base, unsigned i) {
+    const double2* p = at32<A>(base, i);
+    if constexpr (A::kDD) {
+        const double2 h = p[0], l = p[1];
+        return {dd{h.x, l.x}, dd{h.y, l.y}};
+    } else {
+        const double2 v = p[0];
+        return {v.x, v.y};
+    }
+}
+template <class A> __device__ __forceinline__ void gst32(double* base, unsigned i, Cx<A> v) {
+    double2* p = const_cast<double2*>(at32<A>(base, i));
+    if constexpr (A::kDD) {
+        p[0] = make_double2(v.re.hi, v.im.hi);
+        p[1] = make_double2(v.re.lo, v.im.lo);
+    } else {
+        p[0] = make_double2(v.re, v.im);
+    }
+}
+
+// Radix-R butterfly of the reference's Stockham step on x[0..R): out[u] goes to destination q + u*s. Its twiddle
+// u - 1 is element twi + u - 1 of tw.
 template <class A, int R>
 __device__ __forceinline__ void butterfly(const Cx<A> (&x)[R], Cx<A> (&out)[R], bool twid, const double* tw,
-                                          const double* oddc) {
+                                          const double* oddc, unsigned twi = 0) {
     if constexpr (R == 4) {
         const Cx<A> t0 = c_add<A>(x[0], x[2]), t1 = c_sub<A>(x[0], x[2]), t2 = c_add<A>(x[1], x[3]);
         const Cx<A> t3 = c_mul_mi<A>(c_sub<A>(x[1], x[3]));
@@ -45,14 +73,14 @@ __device__ __forceinline__ void butterfly(const Cx<A> (&x)[R], Cx<A> (&out)[R], 
         out[2] = c_sub<A>(t0, t2);
         out[3] = c_sub<A>(t1, t3);
         if (twid) {
-            out[1] = c_mul<A>(out[1], gld<A>(tw, 0));
-            out[2] = c_mul<A>(out[2], gld<A>(tw, 1));
-            out[3] = c_mul<A>(out[3], gld<A>(tw, 2));
+            out[1] = c_mul<A>(out[1], gld32<A>(tw, twi));
+            out[2] = c_mul<A>(out[2], gld32<A>(tw, twi + 1));
+            out[3] = c_mul<A>(out[3], gld32<A>(tw, twi + 2));
         }
     } else if constexpr (R == 2) {
         out[0] = c_add<A>(x[0], x[1]);
         out[1] = c_sub<A>(x[0], x[1]);
-        if (twid) out[1] = c_mul<A>(out[1], gld<A>(tw, 0));
+        if (twid) out[1] = c_mul<A>(out[1], gld32<A>(tw, twi));
     } else {   // odd radix by symmetric sums (src/dsp/fft.hpp:62-84)
         constexpr int H = (R - 1) / 2;
         constexpr int ri = R == 3 ? 0 : R == 5 ? 1 : 2;
@@ -78,8 +106,8 @@ __device__ __forceinline__ void butterfly(const Cx<A> (&x)[R], Cx<A> (&out)[R], 
             Cx<A> xu = {A::add(ac.re, bc.im), A::sub(ac.im, bc.re)};
             Cx<A> xr = {A::sub(ac.re, bc.im), A::add(ac.im, bc.re)};
             if (twid) {
-                xu = c_mul<A>(xu, gld<A>(tw, u - 1));
-                xr = c_mul<A>(xr, gld<A>(tw, R - u - 1));
+                xu = c_mul<A>(xu, gld32<A>(tw, twi + u - 1));
+                xr = c_mul<A>(xr, gld32<A>(tw, twi + R - u - 1));
             }
             out[u] = xu;
             out[R - u] = xr;

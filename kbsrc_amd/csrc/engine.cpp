@@ -414,7 +414,7 @@ void run_pass(int policy, int ld, int st, PassArgs a, hipStream_t s, const char*
         // (KBSRC_SPLIT_GLOBAL=1: always the global-memory route; =2: never the specialised passes)
         const char* sg = std::getenv("KBSRC_SPLIT_GLOBAL");
         const bool global = sg && sg[0] == '1', no_fixed = sg && sg[0] == '2';
-        const SplitFixedEntry* fe = policy == POL_DDS && a.pm == 0 && !global && !no_fixed
+        const SplitFixedEntry* fe = policy == POL_DDS && a.pm == 0 && !global && !no_fixed && pass_offsets_fit32(a)
                                         ? find_split_fixed(a.sd.n, ld, st, a.split) : nullptr;
         if (fe) {
             a.G = fe->G;   // the kernel's compile-time group; a partial last group is masked

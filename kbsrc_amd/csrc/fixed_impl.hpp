@@ -62,10 +62,10 @@ template <int... R> struct Radices {
 };
 
 // Outputs of the last stage that no later reader needs (PassArgs::zkeep): sub-FFT j's output e goes to element
-// e * ste + j * stg, unread when keep < i < zh - keep. keep < 0: every output is needed.
+// e * ste + j * stg, unread when keep < i < zh - keep. keep < 0: every output is needed. Element indices of a task
+// fit 32 bits (launch_fixed checks).
 struct Prune {
-    long long keep = -1, zh = 0, ste = 0, stg = 0;
-    int j0 = 0;
+    int keep = -1, zh = 0, ste = 0, stg = 0, j0 = 0;
 };
 
 // One Stockham stage over G sub-FFTs in place: compute all of this thread's butterflies, sync, write back. In the
@@ -76,14 +76,14 @@ __device__ __forceinline__ void stage(const Lds<A, RL::N * G>& L, int tid, const
                                       const Prune& pr) {
     constexpr int N = RL::N, r = RL::r[I], s = RL::stride(I), nbf = N / r, W = G * nbf, IT = (W + T - 1) / T;
     constexpr bool kLast = I == RL::nst - 1;
-    const double* tws = tw_all + (long long)RL::twoff(I) * kElt<A>;
+    const double* tws = tw_all + RL::twoff(I) * kElt<A>;
     // last stage: s = nbf, so butterfly b writes outputs b + u s
     auto needed = [&](int g, int b) -> bool {
         if (!kLast || pr.keep < 0) return true;
-        const long long j = pr.j0 + g;
+        const int j = pr.j0 + g;
 #pragma unroll
         for (int u = 0; u < r; ++u) {
-            const long long i = (long long)(b + u * s) * pr.ste + j * pr.stg;
+            const int i = (b + u * s) * pr.ste + j * pr.stg;
             if (i <= pr.keep || i >= pr.zh - pr.keep) return true;
         }
         return false;
@@ -98,7 +98,7 @@ __device__ __forceinline__ void stage(const Lds<A, RL::N * G>& L, int tid, const
             Cx<A> x[r];
 #pragma unroll
             for (int t = 0; t < r; ++t) x[t] = L.get((b + t * nbf) * G + g);
-            butterfly<A, r>(x, held[it], p != 0, tws + (long long)p * (r - 1) * kElt<A>, oddc);
+            butterfly<A, r>(x, held[it], p != 0, tws, oddc, (unsigned)(p * (r - 1)));
         }
     }
     __syncthreads();
@@ -146,7 +146,7 @@ __device__ __forceinline__ void first_stage(const Lds<A, RL::N * G>& L, int tid,
             Cx<A> x[r], y[r];
 #pragma unroll
             for (int t = 0; t < r; ++t) x[t] = load(b + t * nbf, j);
-            butterfly<A, r>(x, y, b != 0, tw + (long long)b * (r - 1) * kElt<A>, oddc);
+            butterfly<A, r>(x, y, b != 0, tw, oddc, (unsigned)(b * (r - 1)));
 #pragma unroll
             for (int u = 0; u < r; ++u) L.put((r * b + u) * G + g, y[u]);
         }
@@ -173,7 +173,7 @@ __device__ __forceinline__ void last_stage(const Lds<A, RL::N * G>& L, int tid, 
                 bool need = false;
 #pragma unroll
                 for (int u = 0; u < r; ++u) {
-                    const long long i = (long long)(b + u * s) * pr.ste + (long long)j * pr.stg;
+                    const int i = (b + u * s) * pr.ste + j * pr.stg;
                     need = need || i <= pr.keep || i >= pr.zh - pr.keep;
                 }
                 if (!need) continue;
@@ -282,15 +282,44 @@ template <class A> __device__ __forceinline__ Cx<A> renorm_dd(Cx<A> v) {
     else return v;
 }
 
+// One task's addresses in a pass, formed once per workgroup (the same for every lane, so they stay in scalar
+// registers): the task's source and destination, the element strides, and for ST_OUT the channel's output samples.
+// Each lane then reaches element e of sub-FFT j by one 32-bit offset from a base: a task's element indices stay
+// below 2^27 (launch_fixed checks), so the byte offsets fit 32 bits.
+struct PassView {
+    const double* src;
+    double* dst;
+    double* out;          // ST_OUT: sample x of this block's kept range [xlo, xhi) is out[x + xoff]
+    int lde, ldg, ste, stg;
+    int margP, xlo, xhi, xoff;
+    int zkeep, zh;        // zkeep >= 0: elements zkeep < i < zh - zkeep are not stored (z_unused)
+};
+template <int LD>
+__device__ __forceinline__ PassView pass_view(const PassArgs& a, const TaskIO& io, int task) {
+    PassView v;
+    v.src = LD == LD_REAL_F64 ? a.src + io.in_off : a.src + task * a.src_tstride;
+    v.dst = a.dst + task * a.dst_tstride;
+    v.lde = (int)a.lde; v.ldg = (int)a.ldg; v.ste = (int)a.ste; v.stg = (int)a.stg;
+    v.zkeep = (int)a.zkeep; v.zh = (int)a.zh;
+    // unpacked sample x (x = 2i - margP and x + 1) is kept when 0 <= x < hopP and its output sample
+    // d = blk * hopP + x - o lies in [0, nout) (src/resampler.cpp:80-84): x in [xlo, xhi), written at
+    // out + out_off + base + (x - (base - shift)) with shift = blk * hopP - o and base = clamp(shift, 0, nout)
+    const long long shift = io.blk * a.hopP - a.o;
+    const long long lo = std::max(0LL, std::min(a.hopP, -shift)), hi = std::max(0LL, std::min(a.hopP, io.nout - shift));
+    const long long base = std::max(0LL, std::min(io.nout, shift));
+    v.out = a.out ? a.out + io.out_off + base : nullptr;
+    v.margP = (int)a.margP; v.xlo = (int)lo; v.xhi = (int)hi; v.xoff = (int)(shift - base);
+    return v;
+}
+
 // Element e of sub-FFT j from a pass's source (real pairs of the padded input, or complex elements).
-template <class A, int LD>
-__device__ __forceinline__ Cx<A> pass_load(const PassArgs& a, const TaskIO& io, int task, int e, int j) {
-    const long long i = e * a.lde + (long long)j * a.ldg;
+template <class A, int LD> __device__ __forceinline__ Cx<A> pass_load(const PassView& v, int e, int j) {
+    const unsigned i = (unsigned)(e * v.lde + j * v.ldg);
     if constexpr (LD == LD_REAL_F64) {
-        const double2 pr = reinterpret_cast<const double2*>(a.src + io.in_off)[i];
+        const double2 pr = *at32<F64>(v.src, i);   // a real pair: 16 bytes, the size of an FP64 complex element
         return {A::lift(pr.x), A::lift(pr.y)};
     } else {
-        return gld<A>(a.src + task * a.src_tstride, i);
+        return gld32<A>(v.src, i);
     }
 }
 
@@ -298,34 +327,29 @@ __device__ __forceinline__ Cx<A> pass_load(const PassArgs& a, const TaskIO& io, 
 // store (bins no reader needs are skipped) or the overlap discard (samples 2i and 2i + 1 of the block's output,
 // src/dsp/fft.hpp:460 and src/resampler.cpp:80-84).
 template <class A, int ST>
-__device__ __forceinline__ void pass_store(const PassArgs& a, const TaskIO& io, int task, int e, int j, Cx<A> v) {
-    const long long jg = j;
-    const long long i = e * a.ste + jg * a.stg;
-    if (ST == ST_CPLX && z_unused(a, i)) return;
-    if (a.twmode && e != 0 && (a.twmode == 1 || jg != 0)) v = c_mul<A>(v, gld<A>(a.tw4, i));
+__device__ __forceinline__ void pass_store(const PassArgs& a, const PassView& w, int e, int j, Cx<A> v) {
+    const int i = e * w.ste + j * w.stg;
+    if (ST == ST_CPLX && w.zkeep >= 0 && i > w.zkeep && i < w.zh - w.zkeep) return;
+    if (a.twmode && e != 0 && (a.twmode == 1 || j != 0)) v = c_mul<A>(v, gld32<A>(a.tw4, (unsigned)i));
     if constexpr (A::kDD) {
         if (a.renorm) v = renorm_dd<A>(v);
     }
     if constexpr (ST == ST_CPLX) {
-        gst<A>(a.dst + task * a.dst_tstride, i, v);
+        gst32<A>(w.dst, (unsigned)i, v);
     } else if constexpr (ST == ST_OUT) {
-        double* o = a.out + io.out_off;
-        const long long x0 = 2 * i - a.margP;
-        if (x0 >= 0 && x0 < a.hopP) {
-            const long long d = io.blk * a.hopP + x0 - a.o;
-            if (d >= 0 && d < io.nout) {
-                if constexpr (A::kDD) o[d] = v.re.hi + v.re.lo;
-                else o[d] = v.re;
-            }
+        auto at = [&](int x) -> double* {
+            return reinterpret_cast<double*>(reinterpret_cast<char*>(w.out) + (size_t)((unsigned)(x + w.xoff) * 8u));
+        };
+        const int x0 = 2 * i - w.margP;
+        if (x0 >= w.xlo && x0 < w.xhi) {
+            if constexpr (A::kDD) *at(x0) = v.re.hi + v.re.lo;
+            else *at(x0) = v.re;
         }
-        const long long x1 = x0 + 1;
-        if (x1 >= 0 && x1 < a.hopP) {
-            const long long d = io.blk * a.hopP + x1 - a.o;
-            if (d >= 0 && d < io.nout) {
-                const Val<A> ni = A::neg(v.im);
-                if constexpr (A::kDD) o[d] = ni.hi + ni.lo;
-                else o[d] = ni;
-            }
+        const int x1 = x0 + 1;
+        if (x1 >= w.xlo && x1 < w.xhi) {
+            const Val<A> ni = A::neg(v.im);
+            if constexpr (A::kDD) *at(x1) = ni.hi + ni.lo;
+            else *at(x1) = ni;
         }
     }
 }
@@ -353,8 +377,9 @@ __global__ __launch_bounds__(T, 4) void k_fixed(PassArgs a) {
         if constexpr (kPaired) return slot_index<G>(g, grp, a.f.M);
         else return j0 + g < a.nsub ? j0 + g : -1;
     };
-    auto load_at = [&](int e, int j) -> Cx<A> { return pass_load<A, LD>(a, io, task, e, j); };
-    auto store_at = [&](int e, int j, Cx<A> v) { pass_store<A, ST>(a, io, task, e, j, v); };
+    const PassView view = pass_view<LD>(a, io, task);
+    auto load_at = [&](int e, int j) -> Cx<A> { return pass_load<A, LD>(view, e, j); };
+    auto store_at = [&](int e, int j, Cx<A> v) { pass_store<A, ST>(a, view, e, j, v); };
 
     if constexpr (LD == LD_FOLD) {
         // Y = X H with the bin map for every element, then the pre-split of each with its mirror bin
@@ -414,7 +439,7 @@ __global__ __launch_bounds__(T, 4) void k_fixed(PassArgs a) {
     }
     Prune pr;
     if constexpr (ST == ST_CPLX && !kPaired) {
-        if (a.zkeep >= 0) pr = Prune{a.zkeep, a.zh, a.ste, a.stg, j0};
+        if (a.zkeep >= 0) pr = Prune{(int)a.zkeep, (int)a.zh, (int)a.ste, (int)a.stg, j0};
     }
     if constexpr (IO) {
         first_stage<A, RL, G, T>(L, tid, a.tw, a.oddc, jof, load_at);

@@ -207,6 +207,9 @@ struct SplitFixedEntry {
 const SplitFixedEntry* find_split_fixed(int n, int ld, int st, int slices);
 const SplitFixedEntry* split_table_extreme(int* count);   // kernels_fixed_split_extreme.hip
 
+// true when every element index of one task in pass a (loads, stores, four-step twiddles) is below 2^27: the
+// specialised passes (plain and split) address a task's elements with 32-bit offsets
+bool pass_offsets_fit32(const PassArgs& a);
 // variant: index among the registered shapes of (policy, n, ld, st); -1: the default (or KBSRC_TUNE's)
 bool launch_fixed(int policy, int ld, int st, PassArgs& a, hipStream_t s, hipError_t* err, int variant = -1);
 int fixed_variants(int policy, int n, int ld, int st);

@@ -136,7 +136,15 @@ int fixed_variants(int policy, int n, int ld, int st) {
     return c;
 }
 
+bool pass_offsets_fit32(const PassArgs& a) {
+    constexpr long long lim = 1LL << 27;   // elements of at most 32 bytes: byte offsets below 2^32
+    const long long n = a.sd.n, m = a.nsub;
+    const long long ld_max = (n - 1) * a.lde + (m - 1) * a.ldg, st_max = (n - 1) * a.ste + (m - 1) * a.stg;
+    return ld_max < lim && st_max < lim && a.zh < lim && a.hopP < lim;
+}
+
 bool launch_fixed(int policy, int ld, int st, PassArgs& a, hipStream_t s, hipError_t* err, int variant) {
+    if (!pass_offsets_fit32(a)) return false;   // the specialised passes index a task's elements in 32 bits
     int skip = variant >= 0 ? variant : tune_index(a.sd.n);
     for (const Table& t : all_tables())
       for (int i = 0; i < t.n; ++i) {

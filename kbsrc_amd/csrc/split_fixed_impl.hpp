@@ -31,6 +31,7 @@ __global__ __launch_bounds__(T, 2) void k_fixed_split(PassArgs a) {
     const int j0 = grp * G;
     const TaskIO io = task_io(a.tdesc, a.task0 + task, a.nblk, a.in_cstride, a.in_bstride, a.nout);
     auto jof = [&](int g) -> int { return j0 + g < a.nsub ? j0 + g : -1; };
+    const PassView view = pass_view<LD>(a, io, task);
 
     // the radix-16 stage, its inputs straight from the pass's source: vector (g, p) = elements p + (N / 16) m of
     // local sub-FFT g; y[16 p + k] = w_N^{p k} DFT16, written to the LDS tile
@@ -46,7 +47,7 @@ __global__ __launch_bounds__(T, 2) void k_fixed_split(PassArgs a) {
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
                 if (live) {
-                    const Cx<A> c = pass_load<A, LD>(a, io, task, p + nv * (4 * q + i), j);
+                    const Cx<A> c = pass_load<A, LD>(view, p + nv * (4 * q + i), j);
                     hi[2 * i] = c.re.hi;
                     hi[2 * i + 1] = c.im.hi;
                     lo[2 * i] = c.re.lo;
@@ -62,7 +63,7 @@ __global__ __launch_bounds__(T, 2) void k_fixed_split(PassArgs a) {
                 for (int r = 0; r < 4; ++r) {
                     const int k = 4 * q + r;
                     cx<dd> z{re[r], im[r]};
-                    if (p != 0 && k != 0) z = c_mul<DDA>(z, gld<DDA>(a.tw16, (long long)p * 15 + k - 1));
+                    if (p != 0 && k != 0) z = c_mul<DDA>(z, gld32<DDA>(a.tw16, (unsigned)(p * 15 + k - 1)));
                     L.put((16 * p + k) * G + g, z);
                 }
             }
@@ -71,13 +72,13 @@ __global__ __launch_bounds__(T, 2) void k_fixed_split(PassArgs a) {
     __syncthreads();
     Prune pr;
     if constexpr (ST == ST_CPLX) {
-        if (a.zkeep >= 0) pr = Prune{a.zkeep, a.zh, a.ste, a.stg, j0};
+        if (a.zkeep >= 0) pr = Prune{(int)a.zkeep, (int)a.zh, (int)a.ste, (int)a.stg, j0};
     }
     mid_stages<RL, G, T>(L, tid, a.tw, a.oddc, std::make_integer_sequence<int, RL::nst - 3>{});
     // the last stage, its outputs through the specialised pass's store (four-step twiddle, renormalisation, complex
     // buffer or output samples)
     last_stage<A, RL, G, T>(L, tid, a.tw, a.oddc, pr, jof,
-                            [&](int e, int jj, Cx<A> v) { pass_store<A, ST>(a, io, task, e, jj, v); });
+                            [&](int e, int jj, Cx<A> v) { pass_store<A, ST>(a, view, e, jj, v); });
 }
 
 template <class RL, int G, int T, int LD, int ST, int SPL> hipError_t launch_split_one(const PassArgs& a, hipStream_t s) {
