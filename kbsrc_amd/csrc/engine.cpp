@@ -419,6 +419,7 @@ void run_pass(int policy, int ld, int st, PassArgs a, hipStream_t s, const char*
         if (fe) {
             a.G = fe->G;   // the kernel's compile-time group; a partial last group is masked
             a.ngrp = (a.nsub + a.G - 1) / a.G;
+            set_prune_window(a, st);
             e = fe->launch(a, s);
         } else if (split_lds_ok(policy, ld, st, a) && !global) {
             // up to 80 KB of LDS per workgroup: the split kernel's registers allow two workgroups per CU, and more

@@ -61,40 +61,75 @@ template <int... R> struct Radices {
     }
 };
 
-// Outputs of the last stage that no later reader needs (PassArgs::zkeep): sub-FFT j's output e goes to element
-// e * ste + j * stg, unread when keep < i < zh - keep. keep < 0: every output is needed. Element indices of a task
-// fit 32 bits (launch_fixed checks).
+// The sub-FFT outputs some later reader needs (PassArgs::wstart, wlen, from zkeep): e in the cyclic window
+// [wstart, wstart + wlen) mod N; wlen < 0: all of them. A Stockham stage of stride s needs exactly its butterflies
+// (p, q) with q in that window reduced mod s: butterfly (p, q) writes elements s r p + q + u s, which the next stage
+// (stride s r) reads only where their residue q + u s mod s r lies in the window reduced mod s r, and the last
+// stage's outputs are the sub-FFT's. Stages whose stride exceeds the window run only those butterflies; the
+// skipped ones' results would never be read, so no stored value changes.
 struct Prune {
-    int keep = -1, zh = 0, ste = 0, stg = 0, j0 = 0;
+    int wstart = 0, wlen = -1;
 };
 
-// One Stockham stage over G sub-FFTs in place: compute all of this thread's butterflies, sync, write back. In the
-// last stage a butterfly whose r outputs are all unread is skipped (the store skips those elements too), which
-// changes no value that is stored.
+// A stage whose stride s exceeds the window: the butterflies (p, q), q = (qs + k) mod s for k < len, every p.
+template <class A, class RL, int G, int T, int I>
+__device__ __forceinline__ void stage_window(const Lds<A, RL::N * G>& L, int tid, const double* tws,
+                                             const double* oddc, int qs, int len) {
+    constexpr int N = RL::N, r = RL::r[I], s = RL::stride(I), nbf = N / r, W = G * nbf, IT = (W + T - 1) / T;
+    const int Wn = G * (nbf / s) * len;
+    auto at = [&](int w, int& g, int& p, int& q) {
+        g = w % G;
+        const int k = w / G;
+        p = k / len;
+        q = qs + (k - p * len);
+        if (q >= s) q -= s;
+    };
+    Cx<A> held[IT][r];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int w = tid + it * T;
+        if (w < Wn) {
+            int g, p, q;
+            at(w, g, p, q);
+            Cx<A> x[r];
+#pragma unroll
+            for (int t = 0; t < r; ++t) x[t] = L.get((p * s + q + t * nbf) * G + g);
+            butterfly<A, r>(x, held[it], p != 0, tws, oddc, (unsigned)(p * (r - 1)));
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int w = tid + it * T;
+        if (w < Wn) {
+            int g, p, q;
+            at(w, g, p, q);
+#pragma unroll
+            for (int u = 0; u < r; ++u) L.put((s * r * p + q + u * s) * G + g, held[it][u]);
+        }
+    }
+    __syncthreads();
+}
+
+// One Stockham stage over G sub-FFTs in place: compute all of this thread's butterflies, sync, write back; only
+// the butterflies some reader needs when the window is narrower than the stride (stage_window).
 template <class A, class RL, int G, int T, int I>
 __device__ __forceinline__ void stage(const Lds<A, RL::N * G>& L, int tid, const double* tw_all, const double* oddc,
                                       const Prune& pr) {
     constexpr int N = RL::N, r = RL::r[I], s = RL::stride(I), nbf = N / r, W = G * nbf, IT = (W + T - 1) / T;
-    constexpr bool kLast = I == RL::nst - 1;
     const double* tws = tw_all + RL::twoff(I) * kElt<A>;
-    // last stage: s = nbf, so butterfly b writes outputs b + u s
-    auto needed = [&](int g, int b) -> bool {
-        if (!kLast || pr.keep < 0) return true;
-        const int j = pr.j0 + g;
-#pragma unroll
-        for (int u = 0; u < r; ++u) {
-            const int i = (b + u * s) * pr.ste + j * pr.stg;
-            if (i <= pr.keep || i >= pr.zh - pr.keep) return true;
+    if constexpr (I > 0) {
+        if (pr.wlen >= 0 && pr.wlen < s) {
+            stage_window<A, RL, G, T, I>(L, tid, tws, oddc, pr.wstart % s, pr.wlen);
+            return;
         }
-        return false;
-    };
+    }
     Cx<A> held[IT][r];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const int w = tid + it * T;
         if (W % T == 0 || w < W) {
             const int g = w % G, b = w / G, p = b / s;
-            if (!needed(g, b)) continue;
             Cx<A> x[r];
 #pragma unroll
             for (int t = 0; t < r; ++t) x[t] = L.get((b + t * nbf) * G + g);
@@ -107,7 +142,6 @@ __device__ __forceinline__ void stage(const Lds<A, RL::N * G>& L, int tid, const
         const int w = tid + it * T;
         if (W % T == 0 || w < W) {
             const int g = w % G, b = w / G, p = b / s, q = b - p * s;
-            if (!needed(g, b)) continue;
             const int dbase = s * r * p + q;
 #pragma unroll
             for (int u = 0; u < r; ++u) L.put((dbase + u * s) * G + g, held[it][u]);
@@ -125,9 +159,8 @@ __device__ __forceinline__ void all_stages(const Lds<A, RL::N * G>& L, int tid, 
 // Stages 1 .. nst-2 (the first and last run with their global-memory side, below).
 template <class A, class RL, int G, int T, int... I>
 __device__ __forceinline__ void inner_stages(const Lds<A, RL::N * G>& L, int tid, const double* tw, const double* oddc,
-                                             std::integer_sequence<int, I...>) {
-    const Prune none;
-    (stage<A, RL, G, T, I + 1>(L, tid, tw, oddc, none), ...);
+                                             const Prune& pr, std::integer_sequence<int, I...>) {
+    (stage<A, RL, G, T, I + 1>(L, tid, tw, oddc, pr), ...);
 }
 
 // Stage 0 with its inputs read from global memory: load(e, j) gives element e of sub-FFT j. It has stride 1, so
@@ -155,29 +188,23 @@ __device__ __forceinline__ void first_stage(const Lds<A, RL::N * G>& L, int tid,
 }
 
 // The last stage with its outputs written to global memory: store(e, j, v) takes element e of sub-FFT j. Its
-// stride is N / r, so butterfly b (no twiddles) writes elements b + u (N / r). Butterflies whose outputs are all
-// unread are skipped as in stage().
+// stride is N / r, so butterfly b (no twiddles) writes elements b + u (N / r). With a window narrower than the
+// stride only the butterflies b in it run, as in stage().
 template <class A, class RL, int G, int T, class Jof, class Store>
 __device__ __forceinline__ void last_stage(const Lds<A, RL::N * G>& L, int tid, const double* tw, const double* oddc,
                                            const Prune& pr, Jof jof, Store store) {
     constexpr int I = RL::nst - 1, N = RL::N, r = RL::r[I], s = RL::stride(I), nbf = N / r, W = G * nbf,
                   IT = (W + T - 1) / T;
     static_assert(s == nbf, "the last stage has stride N / r");
+    const bool win = pr.wlen >= 0 && pr.wlen < s;
+    const int Wn = win ? G * pr.wlen : W, qs = win ? pr.wstart % s : 0;
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const int w = tid + it * T;
-        if (W % T == 0 || w < W) {
-            const int g = w % G, b = w / G, j = jof(g);
+        if (win ? w < Wn : (W % T == 0 || w < W)) {
+            const int g = w % G, k = w / G, j = jof(g);
             if (j < 0) continue;
-            if (pr.keep >= 0) {
-                bool need = false;
-#pragma unroll
-                for (int u = 0; u < r; ++u) {
-                    const int i = (b + u * s) * pr.ste + j * pr.stg;
-                    need = need || i <= pr.keep || i >= pr.zh - pr.keep;
-                }
-                if (!need) continue;
-            }
+            const int b = win ? (qs + k < s ? qs + k : qs + k - s) : k;
             Cx<A> x[r], y[r];
 #pragma unroll
             for (int t = 0; t < r; ++t) x[t] = L.get((b + t * nbf) * G + g);
@@ -438,12 +465,10 @@ __global__ __launch_bounds__(T, 4) void k_fixed(PassArgs a) {
         }
     }
     Prune pr;
-    if constexpr (ST == ST_CPLX && !kPaired) {
-        if (a.zkeep >= 0) pr = Prune{(int)a.zkeep, (int)a.zh, (int)a.ste, (int)a.stg, j0};
-    }
+    if constexpr (ST == ST_CPLX && !kPaired) pr = Prune{a.wstart, a.wlen};
     if constexpr (IO) {
         first_stage<A, RL, G, T>(L, tid, a.tw, a.oddc, jof, load_at);
-        inner_stages<A, RL, G, T>(L, tid, a.tw, a.oddc, std::make_integer_sequence<int, RL::nst - 2>{});
+        inner_stages<A, RL, G, T>(L, tid, a.tw, a.oddc, pr, std::make_integer_sequence<int, RL::nst - 2>{});
         last_stage<A, RL, G, T>(L, tid, a.tw, a.oddc, pr, jof, store_at);
         return;
     }

@@ -91,6 +91,10 @@ struct PassArgs {
     // never read (the bin map uses the spectrum only up to bin zkeep, and the real-FFT split pairs bin i with
     // zh - i), so they are not stored
     long long zkeep = -1, zh = 0;
+    // The sub-FFT outputs e that some reader needs under zkeep (set_prune_window): the cyclic window
+    // [wstart, wstart + wlen) mod sd.n, wlen < 0 for all. The specialised passes run only the butterflies whose
+    // outputs reach it.
+    int wstart = 0, wlen = -1;
 };
 // true when a pass with zkeep set may skip storing output element i
 __host__ __device__ inline bool z_unused(const PassArgs& a, long long i) { return a.zkeep >= 0 && i > a.zkeep && i < a.zh - a.zkeep; }
@@ -210,6 +214,9 @@ const SplitFixedEntry* split_table_extreme(int* count);   // kernels_fixed_split
 // true when every element index of one task in pass a (loads, stores, four-step twiddles) is below 2^27: the
 // specialised passes (plain and split) address a task's elements with 32-bit offsets
 bool pass_offsets_fit32(const PassArgs& a);
+// Sets a.wstart, a.wlen from a.zkeep for a pass with store kind st: element e of sub-FFT j goes to
+// i = e ste + j stg, needed when i <= zkeep or i >= zh - zkeep for some j < nsub.
+void set_prune_window(PassArgs& a, int st);
 // variant: index among the registered shapes of (policy, n, ld, st); -1: the default (or KBSRC_TUNE's)
 bool launch_fixed(int policy, int ld, int st, PassArgs& a, hipStream_t s, hipError_t* err, int variant = -1);
 int fixed_variants(int policy, int n, int ld, int st);

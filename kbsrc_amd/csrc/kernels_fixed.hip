@@ -143,8 +143,27 @@ bool pass_offsets_fit32(const PassArgs& a) {
     return ld_max < lim && st_max < lim && a.zh < lim && a.hopP < lim;
 }
 
+void set_prune_window(PassArgs& a, int st) {
+    a.wstart = 0;
+    a.wlen = -1;
+    const long long n = a.sd.n;
+    if (st != ST_CPLX || a.zkeep < 0 || a.ste <= 0 || n < 2) return;
+    // e <= elo has a needed element at j = 0; e >= ehi at j = nsub - 1 (the last sub-FFT's offset is the largest)
+    const long long elo = a.zkeep / a.ste;
+    const long long top = a.zh - a.zkeep - (long long)(a.nsub - 1) * a.stg;
+    const long long ehi = top <= 0 ? 0 : (top + a.ste - 1) / a.ste;
+    if (elo >= n - 1 || ehi <= elo + 1) return;   // every output needed
+    if (ehi >= n) {
+        a.wlen = (int)(elo + 1);
+    } else {
+        a.wstart = (int)ehi;
+        a.wlen = (int)(n - ehi + elo + 1);
+    }
+}
+
 bool launch_fixed(int policy, int ld, int st, PassArgs& a, hipStream_t s, hipError_t* err, int variant) {
     if (!pass_offsets_fit32(a)) return false;   // the specialised passes index a task's elements in 32 bits
+    set_prune_window(a, st);
     int skip = variant >= 0 ? variant : tune_index(a.sd.n);
     for (const Table& t : all_tables())
       for (int i = 0; i < t.n; ++i) {

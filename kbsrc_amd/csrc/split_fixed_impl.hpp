@@ -13,9 +13,8 @@ namespace {
 
 template <class RL, int G, int T, int... I>
 __device__ __forceinline__ void mid_stages(const Lds<DDS, RL::N * G>& L, int tid, const double* tw,
-                                           const double* oddc, std::integer_sequence<int, I...>) {
-    const Prune none;
-    (stage<DDS, RL, G, T, I + 2>(L, tid, tw, oddc, none), ...);
+                                           const double* oddc, const Prune& pr, std::integer_sequence<int, I...>) {
+    (stage<DDS, RL, G, T, I + 2>(L, tid, tw, oddc, pr), ...);
 }
 
 // two waves per SIMD: the matrix stage's slicing needs up to ~150 VGPRs at 15 slices
@@ -71,10 +70,8 @@ __global__ __launch_bounds__(T, 2) void k_fixed_split(PassArgs a) {
     }
     __syncthreads();
     Prune pr;
-    if constexpr (ST == ST_CPLX) {
-        if (a.zkeep >= 0) pr = Prune{(int)a.zkeep, (int)a.zh, (int)a.ste, (int)a.stg, j0};
-    }
-    mid_stages<RL, G, T>(L, tid, a.tw, a.oddc, std::make_integer_sequence<int, RL::nst - 3>{});
+    if constexpr (ST == ST_CPLX) pr = Prune{a.wstart, a.wlen};
+    mid_stages<RL, G, T>(L, tid, a.tw, a.oddc, pr, std::make_integer_sequence<int, RL::nst - 3>{});
     // the last stage, its outputs through the specialised pass's store (four-step twiddle, renormalisation, complex
     // buffer or output samples)
     last_stage<A, RL, G, T>(L, tid, a.tw, a.oddc, pr, jof,
