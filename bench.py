@@ -27,6 +27,7 @@ FP64_PEAK_TFLOPS = 78.6   # MI355X published FP64 peak
 SPLIT_SLICES_2M100 = 15   # fewest base-128 slices whose radix-16 stage error is within 2^-100 (test_gpu_split)
 WARM_MIN = 30   # fewest untimed executes before a timed region
 HBM_PEAK_GBS = 8000.0
+PEAK_CLOCK_MHZ = 2400.0   # MI355X peak engine clock, the clock the published peaks are quoted at
 
 
 # ---- algorithmic FP64 flop counts of the reference's plain-dd / FP64 algorithm --------------------------------
@@ -185,6 +186,15 @@ def summarize_rocprof(d, mode, steps=20, warmup=5):
         out.append(f"  {p:8s} {len(t):8d} {avg:9.1f} {min(t):9.1f} {max(t):9.1f} {fb / 1e6:9.1f} {wb / 1e6:9.1f} "
                    f"{(fb + wb) / avg / 1e3:8.0f} {sum(ta) / len(ta):10.1f}  {kname[p]}")
     out.append(f"timed_sum_us {sum(sum(timed[p]) / len(timed[p]) for p in PASSES if timed[p]):.1f}")
+    same = os.path.join(d, "bench.json")   # the unprofiled bench line of the same box, if the run kept one
+    if os.path.exists(same):
+        with open(same) as f:
+            b = json.loads(f.read().strip().splitlines()[-1])
+        ev = " ".join(f"{k} {v['ms'] * 1e3:.1f}" for k, v in b["passes"].items())
+        ul = b["roofline"].get("under_load")
+        out.append(f"# the same box without the profiler (python3 R/bench.py --mode {mode} --no-cpu > D/bench.json): "
+                   f"{b['ms_per_step']} ms/step; pass events (us): {ev}" +
+                   (f"; under load {ul['socket_power_w']:.0f} W, GFX clock {ul['gfx_clock_mhz']:.0f} MHz" if ul else ""))
     for p in PASSES:
         if cnt["FETCH_SIZE"][p]:
             fb = sum(cnt["FETCH_SIZE"][p]) / len(cnt["FETCH_SIZE"][p]) * 1024 * factor["FETCH_SIZE"]
@@ -338,6 +348,19 @@ def committed_valu_busy(mode, pass_name):
         if len(f) == 9 and f[0] == pass_name:
             return float(f[6])
     return None
+
+
+def committed_valu_insts(mode):
+    """VALU wave-instructions of one execute (all passes) from the committed counter summary, or None."""
+    path = os.path.join(ROOT, "profiles", f"valu_counters_96k_to_44k1_180s_{mode}.txt")
+    if not os.path.exists(path):
+        return None
+    total = 0.0
+    for line in open(path):
+        f = line.split()
+        if len(f) == 9 and f[0] in PASSES:
+            total += float(f[1])
+    return total or None
 
 
 def host_cpu():
@@ -1004,6 +1027,26 @@ def main():
     graph_ms, graph_same = graph_timing(args, plan, xx, out)
     print(f"captured graph replay: {graph_ms:.3f} ms/step; same output: {graph_same}", file=sys.stderr, flush=True)
 
+    # the clock the timed region ran at: power and GFX clocks while the same execute repeats back to back
+    power = power_sample(lambda: plan.execute(xx.data_ptr(), out.data_ptr(), stream), torch.cuda.synchronize)
+    if power:
+        print(f"under load: socket power {power['socket_power_w']:.0f} W, GFX clock {power['gfx_clock_mhz']:.0f} MHz "
+              f"(XCDs {min(power['gfx_clock_mhz_per_xcd']):.0f}-{max(power['gfx_clock_mhz_per_xcd']):.0f})",
+              file=sys.stderr, flush=True)
+        if dd:
+            # the FP64 peak scales with the clock: at the clock the power limit leaves, the attainable peak is lower
+            at = FP64_PEAK_TFLOPS * power["gfx_clock_mhz"] / PEAK_CLOCK_MHZ
+            roof["peak_at_measured_clock"] = round(at, 2)
+            roof["frac_at_measured_clock"] = round(roof["achieved"] / at, 4)
+            # the step's VALU issue floor at that clock: every VALU wave-instruction of an execute (committed
+            # counters) at one per 4 cycles on each of the 1024 SIMDs
+            insts = committed_valu_insts(args.mode)
+            if insts:
+                floor_ms = insts * 4 / 1024 / (power["gfx_clock_mhz"] * 1e6) * 1e3
+                roof["step_valu_issue_floor_ms_at_measured_clock"] = round(floor_ms, 4)
+                roof["step_frac_of_issue_floor"] = round(floor_ms / ms, 4)
+        roof["under_load"] = {k: power[k] for k in ("socket_power_w", "gfx_clock_mhz")}
+
     # CPU baseline protocol: the reference's own engine (oracle/_ref/kbsrc, built from its sources) on the same input
     # on this host, at its default thread count for the mode (8 exact, 4 fast: src/resampler.cpp:139) and at the
     # CPU share one GPU gets on this host (16 threads; the reference's useful cap here is ch * ceil(nblk / 4) = 54,
@@ -1123,6 +1166,44 @@ def main():
     report_children()
     print(json.dumps(line), flush=True)
     return 0
+
+
+def power_sample(run, sync, seconds=3.0):
+    """Socket power and GFX clocks while `run` (one execute) repeats back to back: one `amd-smi metric` reading
+    taken 1.5 s into the loop. The published FP64 peak assumes the 2.4 GHz peak clock; under the board's power limit
+    the clock settles lower, and the attainable peak with it. None when amd-smi is absent or unreadable."""
+    import threading
+    smi = "/opt/rocm/bin/amd-smi"
+    if not os.path.exists(smi):
+        return None
+    got = {}
+
+    def sample():
+        time.sleep(1.5)
+        try:
+            r = subprocess.run([smi, "metric", "-c", "-p"], capture_output=True, text=True, timeout=30)
+            got["text"] = r.stdout
+        except (OSError, subprocess.SubprocessError) as e:
+            got["error"] = str(e)
+
+    th = threading.Thread(target=sample)
+    th.start()
+    t_end = time.time() + seconds
+    n = 0
+    while th.is_alive() or time.time() < t_end:
+        for _ in range(20):
+            run()
+        sync()
+        n += 20
+    th.join()
+    text = got.get("text", "")
+    pw = re.search(r"SOCKET_POWER:\s*([0-9.]+)\s*W", text)
+    # the CLOCK section lists GFX_0 .. GFX_7 (one per XCD), each with its current CLK first
+    clks = [float(m.group(1)) for m in re.finditer(r"GFX_\d+:\s*\n\s*CLK:\s*([0-9.]+)\s*MHz", text)]
+    if not pw or not clks:
+        return None
+    return {"socket_power_w": float(pw.group(1)), "gfx_clock_mhz": round(sum(clks) / len(clks), 1),
+            "gfx_clock_mhz_per_xcd": clks, "executes_in_loop": n}
 
 
 def report_children():
