@@ -305,9 +305,14 @@ int pick_group(int policy, int n, int nsub) {
 // Read when a plan is made: KBSRC_GLOBAL_PASSES=1 runs the passes through global scratch, as sub-FFTs too long for
 // an LDS tile always do (two LDS passes where the stage list splits, =2: one launch per stage). Read at every split
 // execute: KBSRC_SPLIT_GLOBAL=1 runs every split-form pass through the global-memory route (same bits).
+// KBSRC_LONG_PARTS=0 keeps long column sub-FFTs that have a two-part route in their one-pass kernel (same bits).
 bool env_flag(const char* name) {
     const char* v = std::getenv(name);
     return v && v[0] == '1';
+}
+bool env_off(const char* name) {
+    const char* v = std::getenv(name);
+    return v && v[0] == '0';
 }
 bool use_fixed() { return !env_flag("KBSRC_GENERIC_PASSES"); }
 bool global_forced() {   // KBSRC_GLOBAL_PASSES=1: split into two LDS passes where possible; =2: one launch per stage
@@ -693,7 +698,17 @@ void run_cfft(const CFFTPlan& f, const RunCtx& c, int ld, const double* src, int
         a.tw4 = f.tw4.d(); a.twmode = lane ? 1 : 2; a.renorm = lane && c.dd;
         a.f.M = f.N2;
         a.zkeep = -1;   // the column pass's outputs all feed the rows
-        do_pass(c, t_col, pol_four, ld, ST_CPLX, a, "column pass");
+        // a long column sub-FFT as two specialised passes through the row pass's destination, free until the rows
+        // (KBSRC_LONG_PARTS=0: always the one-pass kernel)
+        const bool two = !c.collect && a.split == 0 && dst && long_two_part(pol_four, a.sd.n, ld, ST_CPLX) &&
+                         pass_offsets_fit32(a) && (long long)a.sd.n * a.nsub * (c.dd ? 4 : 2) <= ts &&
+                         !env_off("KBSRC_LONG_PARTS");
+        if (two)
+            timed(c, t_col, [&] {
+                hip_check(launch_long_two_part(pol_four, ld, ST_CPLX, a, dst, ts, c.s), "column pass (two parts)");
+            });
+        else
+            do_pass(c, t_col, pol_four, ld, ST_CPLX, a, "column pass");
         PassArgs b = c.proto;
         fill_sub(b, f.s2, *c.k, c.dd, pol_four, f.N1);
         b.src = scr[0]; b.src_tstride = ts; b.lde = 1; b.ldg = f.N2;

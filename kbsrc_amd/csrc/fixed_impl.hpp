@@ -71,9 +71,31 @@ struct Prune {
     int wstart = 0, wlen = -1;
 };
 
+// Where a stage's twiddles come from: the stage's table and the index there of butterfly p of local sub-FFT g
+// (index 0: no twiddle).
+struct OwnTw {   // the sub-FFT's own tables: stage I's at RL::twoff(I), butterfly p
+    const double* tw;
+    template <class A, class RL, int I> __device__ __forceinline__ const double* table() const {
+        return tw + RL::twoff(I) * kElt<A>;
+    }
+    __device__ __forceinline__ int index(int, int p) const { return p; }
+};
+// Stages of a longer transform (the two parts of a long sub-FFT): stage I's table at st[I].tw, the long
+// transform's; butterfly p of local sub-FFT g is the long transform's butterfly c0 + cg g + pm p (a leading part
+// on column c0 + cg g of the long transform takes pm = its column count; a trailing part c0 = cg = 0, pm = 1).
+struct LongTw {
+    const double* tw;
+    const StageDesc* st;
+    int c0, cg, pm;
+    template <class A, class RL, int I> __device__ __forceinline__ const double* table() const {
+        return tw + st[I].tw * kElt<A>;
+    }
+    __device__ __forceinline__ int index(int g, int p) const { return c0 + cg * g + pm * p; }
+};
+
 // A stage whose stride s exceeds the window: the butterflies (p, q), q = (qs + k) mod s for k < len, every p.
-template <class A, class RL, int G, int T, int I>
-__device__ __forceinline__ void stage_window(const Lds<A, RL::N * G>& L, int tid, const double* tws,
+template <class A, class RL, int G, int T, int I, class TW>
+__device__ __forceinline__ void stage_window(const Lds<A, RL::N * G>& L, int tid, const TW& tw, const double* tws,
                                              const double* oddc, int qs, int len) {
     constexpr int N = RL::N, r = RL::r[I], s = RL::stride(I), nbf = N / r, W = G * nbf, IT = (W + T - 1) / T;
     const int Wn = G * (nbf / s) * len;
@@ -94,7 +116,8 @@ __device__ __forceinline__ void stage_window(const Lds<A, RL::N * G>& L, int tid
             Cx<A> x[r];
 #pragma unroll
             for (int t = 0; t < r; ++t) x[t] = L.get((p * s + q + t * nbf) * G + g);
-            butterfly<A, r>(x, held[it], p != 0, tws, oddc, (unsigned)(p * (r - 1)));
+            const int pg = tw.index(g, p);
+            butterfly<A, r>(x, held[it], pg != 0, tws, oddc, (unsigned)(pg * (r - 1)));
         }
     }
     __syncthreads();
@@ -113,14 +136,14 @@ __device__ __forceinline__ void stage_window(const Lds<A, RL::N * G>& L, int tid
 
 // One Stockham stage over G sub-FFTs in place: compute all of this thread's butterflies, sync, write back; only
 // the butterflies some reader needs when the window is narrower than the stride (stage_window).
-template <class A, class RL, int G, int T, int I>
-__device__ __forceinline__ void stage(const Lds<A, RL::N * G>& L, int tid, const double* tw_all, const double* oddc,
+template <class A, class RL, int G, int T, int I, class TW>
+__device__ __forceinline__ void stage(const Lds<A, RL::N * G>& L, int tid, const TW& tw, const double* oddc,
                                       const Prune& pr) {
     constexpr int N = RL::N, r = RL::r[I], s = RL::stride(I), nbf = N / r, W = G * nbf, IT = (W + T - 1) / T;
-    const double* tws = tw_all + RL::twoff(I) * kElt<A>;
+    const double* tws = tw.template table<A, RL, I>();
     if constexpr (I > 0) {
         if (pr.wlen >= 0 && pr.wlen < s) {
-            stage_window<A, RL, G, T, I>(L, tid, tws, oddc, pr.wstart % s, pr.wlen);
+            stage_window<A, RL, G, T, I>(L, tid, tw, tws, oddc, pr.wstart % s, pr.wlen);
             return;
         }
     }
@@ -133,7 +156,8 @@ __device__ __forceinline__ void stage(const Lds<A, RL::N * G>& L, int tid, const
             Cx<A> x[r];
 #pragma unroll
             for (int t = 0; t < r; ++t) x[t] = L.get((b + t * nbf) * G + g);
-            butterfly<A, r>(x, held[it], p != 0, tws, oddc, (unsigned)(p * (r - 1)));
+            const int pg = tw.index(g, p);
+            butterfly<A, r>(x, held[it], pg != 0, tws, oddc, (unsigned)(pg * (r - 1)));
         }
     }
     __syncthreads();
@@ -150,15 +174,15 @@ __device__ __forceinline__ void stage(const Lds<A, RL::N * G>& L, int tid, const
     __syncthreads();
 }
 
-template <class A, class RL, int G, int T, int... I>
-__device__ __forceinline__ void all_stages(const Lds<A, RL::N * G>& L, int tid, const double* tw, const double* oddc,
+template <class A, class RL, int G, int T, class TW, int... I>
+__device__ __forceinline__ void all_stages(const Lds<A, RL::N * G>& L, int tid, const TW& tw, const double* oddc,
                                            const Prune& pr, std::integer_sequence<int, I...>) {
     (stage<A, RL, G, T, I>(L, tid, tw, oddc, pr), ...);
 }
 
 // Stages 1 .. nst-2 (the first and last run with their global-memory side, below).
-template <class A, class RL, int G, int T, int... I>
-__device__ __forceinline__ void inner_stages(const Lds<A, RL::N * G>& L, int tid, const double* tw, const double* oddc,
+template <class A, class RL, int G, int T, class TW, int... I>
+__device__ __forceinline__ void inner_stages(const Lds<A, RL::N * G>& L, int tid, const TW& tw, const double* oddc,
                                              const Prune& pr, std::integer_sequence<int, I...>) {
     (stage<A, RL, G, T, I + 1>(L, tid, tw, oddc, pr), ...);
 }
@@ -166,10 +190,11 @@ __device__ __forceinline__ void inner_stages(const Lds<A, RL::N * G>& L, int tid
 // Stage 0 with its inputs read from global memory: load(e, j) gives element e of sub-FFT j. It has stride 1, so
 // butterfly b reads elements b + t (N / r) and writes r b + u; nothing in LDS is read, so it writes its outputs
 // straight away, and one sync follows.
-template <class A, class RL, int G, int T, class Jof, class Load>
-__device__ __forceinline__ void first_stage(const Lds<A, RL::N * G>& L, int tid, const double* tw, const double* oddc,
+template <class A, class RL, int G, int T, class TW, class Jof, class Load>
+__device__ __forceinline__ void first_stage(const Lds<A, RL::N * G>& L, int tid, const TW& tw, const double* oddc,
                                             Jof jof, Load load) {
     constexpr int N = RL::N, r = RL::r[0], nbf = N / r, W = G * nbf, IT = (W + T - 1) / T;
+    const double* tws = tw.template table<A, RL, 0>();
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const int w = tid + it * T;
@@ -179,7 +204,8 @@ __device__ __forceinline__ void first_stage(const Lds<A, RL::N * G>& L, int tid,
             Cx<A> x[r], y[r];
 #pragma unroll
             for (int t = 0; t < r; ++t) x[t] = load(b + t * nbf, j);
-            butterfly<A, r>(x, y, b != 0, tw, oddc, (unsigned)(b * (r - 1)));
+            const int pg = tw.index(g, b);
+            butterfly<A, r>(x, y, pg != 0, tws, oddc, (unsigned)(pg * (r - 1)));
 #pragma unroll
             for (int u = 0; u < r; ++u) L.put((r * b + u) * G + g, y[u]);
         }
@@ -188,14 +214,16 @@ __device__ __forceinline__ void first_stage(const Lds<A, RL::N * G>& L, int tid,
 }
 
 // The last stage with its outputs written to global memory: store(e, j, v) takes element e of sub-FFT j. Its
-// stride is N / r, so butterfly b (no twiddles) writes elements b + u (N / r). With a window narrower than the
-// stride only the butterflies b in it run, as in stage().
-template <class A, class RL, int G, int T, class Jof, class Store>
-__device__ __forceinline__ void last_stage(const Lds<A, RL::N * G>& L, int tid, const double* tw, const double* oddc,
+// stride is N / r, so butterfly b writes elements b + u (N / r); its p is 0, so it has no twiddles unless it is the
+// leading part of a longer transform (LongTw). With a window narrower than the stride only the butterflies b in it
+// run, as in stage().
+template <class A, class RL, int G, int T, class TW, class Jof, class Store>
+__device__ __forceinline__ void last_stage(const Lds<A, RL::N * G>& L, int tid, const TW& tw, const double* oddc,
                                            const Prune& pr, Jof jof, Store store) {
     constexpr int I = RL::nst - 1, N = RL::N, r = RL::r[I], s = RL::stride(I), nbf = N / r, W = G * nbf,
                   IT = (W + T - 1) / T;
     static_assert(s == nbf, "the last stage has stride N / r");
+    const double* tws = tw.template table<A, RL, I>();
     const bool win = pr.wlen >= 0 && pr.wlen < s;
     const int Wn = win ? G * pr.wlen : W, qs = win ? pr.wstart % s : 0;
 #pragma unroll
@@ -208,7 +236,8 @@ __device__ __forceinline__ void last_stage(const Lds<A, RL::N * G>& L, int tid, 
             Cx<A> x[r], y[r];
 #pragma unroll
             for (int t = 0; t < r; ++t) x[t] = L.get((b + t * nbf) * G + g);
-            butterfly<A, r>(x, y, false, tw, oddc);
+            const int pg = tw.index(g, 0);
+            butterfly<A, r>(x, y, pg != 0, tws, oddc, (unsigned)(pg * (r - 1)));
 #pragma unroll
             for (int u = 0; u < r; ++u) store(b + u * s, j, y[u]);
         }
@@ -466,14 +495,15 @@ __global__ __launch_bounds__(T, 4) void k_fixed(PassArgs a) {
     }
     Prune pr;
     if constexpr (ST == ST_CPLX && !kPaired) pr = Prune{a.wstart, a.wlen};
+    const OwnTw tw{a.tw};
     if constexpr (IO) {
-        first_stage<A, RL, G, T>(L, tid, a.tw, a.oddc, jof, load_at);
-        inner_stages<A, RL, G, T>(L, tid, a.tw, a.oddc, pr, std::make_integer_sequence<int, RL::nst - 2>{});
-        last_stage<A, RL, G, T>(L, tid, a.tw, a.oddc, pr, jof, store_at);
+        first_stage<A, RL, G, T>(L, tid, tw, a.oddc, jof, load_at);
+        inner_stages<A, RL, G, T>(L, tid, tw, a.oddc, pr, std::make_integer_sequence<int, RL::nst - 2>{});
+        last_stage<A, RL, G, T>(L, tid, tw, a.oddc, pr, jof, store_at);
         return;
     }
     __syncthreads();
-    all_stages<A, RL, G, T>(L, tid, a.tw, a.oddc, pr, std::make_integer_sequence<int, RL::nst>{});
+    all_stages<A, RL, G, T>(L, tid, tw, a.oddc, pr, std::make_integer_sequence<int, RL::nst>{});
 
     if constexpr (ST == ST_SPLIT) {
         // the sub-FFT outputs are z[M*e + j] (natural order); X[i] needs z[i] and z[h - i]

@@ -221,6 +221,11 @@ void set_prune_window(PassArgs& a, int st);
 bool launch_fixed(int policy, int ld, int st, PassArgs& a, hipStream_t s, hipError_t* err, int variant = -1);
 int fixed_variants(int policy, int n, int ld, int st);
 bool has_fixed(int policy, int n, int ld, int st);
+// A long sub-FFT pass as two specialised passes (kernels_fixed_long.hip): the leading stages on its columns, the
+// rest on its classes, through scr (scr_ts doubles per task, at least n * nsub elements); same bits as one pass.
+bool long_two_part(int policy, int n, int ld, int st);
+hipError_t launch_long_two_part(int policy, int ld, int st, const PassArgs& a, double* scr, long long scr_ts,
+                                hipStream_t s);
 // Sub-FFT pass through global memory, one launch per stage (lengths whose tile does not fit LDS).
 hipError_t launch_global(int policy, int ld, int st, const PassArgs& a, hipStream_t s);
 hipError_t launch_mid(int policy, const MidArgs& a, hipStream_t s);

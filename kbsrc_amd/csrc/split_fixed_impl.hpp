@@ -12,7 +12,7 @@ namespace kbsrc::gpu {
 namespace {
 
 template <class RL, int G, int T, int... I>
-__device__ __forceinline__ void mid_stages(const Lds<DDS, RL::N * G>& L, int tid, const double* tw,
+__device__ __forceinline__ void mid_stages(const Lds<DDS, RL::N * G>& L, int tid, const OwnTw& tw,
                                            const double* oddc, const Prune& pr, std::integer_sequence<int, I...>) {
     (stage<DDS, RL, G, T, I + 2>(L, tid, tw, oddc, pr), ...);
 }
@@ -71,10 +71,11 @@ __global__ __launch_bounds__(T, 2) void k_fixed_split(PassArgs a) {
     __syncthreads();
     Prune pr;
     if constexpr (ST == ST_CPLX) pr = Prune{a.wstart, a.wlen};
-    mid_stages<RL, G, T>(L, tid, a.tw, a.oddc, pr, std::make_integer_sequence<int, RL::nst - 3>{});
+    const OwnTw tw{a.tw};
+    mid_stages<RL, G, T>(L, tid, tw, a.oddc, pr, std::make_integer_sequence<int, RL::nst - 3>{});
     // the last stage, its outputs through the specialised pass's store (four-step twiddle, renormalisation, complex
     // buffer or output samples)
-    last_stage<A, RL, G, T>(L, tid, a.tw, a.oddc, pr, jof,
+    last_stage<A, RL, G, T>(L, tid, tw, a.oddc, pr, jof,
                             [&](int e, int jj, Cx<A> v) { pass_store<A, ST>(a, view, e, jj, v); });
 }
 

@@ -169,6 +169,29 @@ def test_long_subfft_extreme_ratio_pin(mode):
     assert hashlib.sha256(y.tobytes()).hexdigest() == H.PINS_10M_SHORT[mode]
 
 
+@pytest.mark.parametrize("mode,var,val", [(m, v, x) for m in ("exact", "fast")
+                                          for v, x in (("KBSRC_LONG_PARTS", "0"), ("KBSRC_LONG_VARIANT", "0"),
+                                                       ("KBSRC_LONG_VARIANT", "1"), ("KBSRC_LONG_ORDER", "0"))])
+def test_long_column_parts_give_the_one_pass_bits(mode, var, val, monkeypatch):
+    """The forward columns of the 10 MHz -> 48 kHz plans run as two passes through the row pass's buffer (exact:
+    3150 points as an 18-point head on 175 columns and a 175-point tail on 18 classes; fast: 3125 points as 25 x 125
+    or 125 x 25): each shape and workgroup order of that route, and the one-pass kernel (KBSRC_LONG_PARTS=0), give
+    the reference's bits; on two channels of several blocks each, the same bits as the one-pass kernel."""
+    import hashlib
+    for v in ("KBSRC_LONG_PARTS", "KBSRC_LONG_VARIANT", "KBSRC_LONG_ORDER"):
+        monkeypatch.delenv(v, raising=False)
+    x2 = np.random.default_rng(11).uniform(-0.5, 0.5, (700000, 2))
+    monkeypatch.setenv("KBSRC_LONG_PARTS", "0")
+    ref2 = gpu().resample(x2, 10000000, 48000, mode, length=0.5)
+    monkeypatch.delenv("KBSRC_LONG_PARTS")
+    monkeypatch.setenv(var, val)
+    say(f"10M->48k two-part columns {mode} {var}={val}")
+    y = gpu().resample(H.input_10m_short().reshape(-1, 1), 10000000, 48000, mode, length=0.5)
+    assert hashlib.sha256(y.tobytes()).hexdigest() == H.PINS_10M_SHORT[mode]
+    y2 = gpu().resample(x2, 10000000, 48000, mode, length=0.5)
+    assert y2.tobytes() == ref2.tobytes()
+
+
 def test_extreme_ratio_full_length_pins():
     """10 MHz -> 48 kHz, 30 s mono, --len 0.5 (the 5,000,001-tap design on the GPU), both modes."""
     import hashlib
