@@ -700,7 +700,8 @@ void run_cfft(const CFFTPlan& f, const RunCtx& c, int ld, const double* src, int
         a.zkeep = -1;   // the column pass's outputs all feed the rows
         // a long column sub-FFT as two specialised passes through the row pass's destination, free until the rows
         // (KBSRC_LONG_PARTS=0: always the one-pass kernel)
-        const bool two = !c.collect && a.split == 0 && dst && long_two_part(pol_four, a.sd.n, ld, ST_CPLX) &&
+        const bool split_stage = a.split > 0 && a.sd.nst >= 2 && a.sd.st[0].r == 4 && a.sd.st[1].r == 4;
+        const bool two = !c.collect && !split_stage && dst && long_two_part(pol_four, a.sd.n, ld, ST_CPLX) &&
                          pass_offsets_fit32(a) && (long long)a.sd.n * a.nsub * (c.dd ? 4 : 2) <= ts &&
                          !env_off("KBSRC_LONG_PARTS");
         if (two)
