@@ -1196,14 +1196,21 @@ def power_sample(run, sync, seconds=3.0):
         sync()
         n += 20
     th.join()
-    text = got.get("text", "")
+    r = parse_amd_smi_metric(got.get("text", ""))
+    if r:
+        r["executes_in_loop"] = n
+    return r
+
+
+def parse_amd_smi_metric(text):
+    """Socket power (W) and the GFX clocks (MHz, one per XCD) from `amd-smi metric -c -p` text, or None."""
     pw = re.search(r"SOCKET_POWER:\s*([0-9.]+)\s*W", text)
     # the CLOCK section lists GFX_0 .. GFX_7 (one per XCD), each with its current CLK first
     clks = [float(m.group(1)) for m in re.finditer(r"GFX_\d+:\s*\n\s*CLK:\s*([0-9.]+)\s*MHz", text)]
     if not pw or not clks:
         return None
     return {"socket_power_w": float(pw.group(1)), "gfx_clock_mhz": round(sum(clks) / len(clks), 1),
-            "gfx_clock_mhz_per_xcd": clks, "executes_in_loop": n}
+            "gfx_clock_mhz_per_xcd": clks}
 
 
 def report_children():
