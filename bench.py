@@ -255,6 +255,21 @@ COUNTER_SETS = (
 )
 
 
+def summarize_kernels(d, top=30):
+    """Reads <d>/run_kernel_stats.csv (rocprofv3 --kernel-trace --stats, csv) and prints the kernels by total time:
+    calls, average, minimum and maximum duration, total (the per-kernel summary kept under profiles/ for workloads
+    without a per-pass summary)."""
+    rows = _csv_rows(os.path.join(d, "run_kernel_stats.csv"))
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    out = [f"# {'kernel':70s} {'calls':>6s} {'avg_us':>10s} {'min_us':>10s} {'max_us':>10s} {'total_ms':>10s}"]
+    for r in rows[:top]:
+        out.append(f"  {_short_kernel(r['Name'])[:70]:70s} {int(r['Calls']):6d} {float(r['AverageNs']) / 1e3:10.1f} "
+                   f"{float(r['MinNs']) / 1e3:10.1f} {float(r['MaxNs']) / 1e3:10.1f} "
+                   f"{float(r['TotalDurationNs']) / 1e6:10.2f}")
+    print("\n".join(out))
+    return 0
+
+
 def summarize_counters(d, mode):
     """Reads <d>/pmc1 and <d>/pmc2 (rocprofv3 --pmc, csv) and prints the per-pass VALU and LDS summary kept under
     profiles/: counters summed over XCDs and shader engines per dispatch, averaged over the dispatches of a pass."""
@@ -880,6 +895,8 @@ def main():
                     help="print the radix-16 stage summary of a rocprofv3 kernel trace of the split workload in DIR")
     ap.add_argument("--summarize-counters", metavar="DIR",
                     help="print the per-pass VALU/LDS counter summary of rocprofv3 csv output under DIR/{pmc1,pmc2}")
+    ap.add_argument("--summarize-kernels", metavar="DIR",
+                    help="print the per-kernel summary (calls, durations) of rocprofv3 --stats csv output in DIR")
     ap.add_argument("--summarize-rocprof", metavar="DIR",
                     help="print the per-pass summary of rocprofv3 csv output under DIR/{stats,fetch,write} and exit")
     args = ap.parse_args()
@@ -889,6 +906,8 @@ def main():
         return summarize_counters(args.summarize_counters, args.mode)
     if args.summarize_split:
         return summarize_split(args.summarize_split)
+    if args.summarize_kernels:
+        return summarize_kernels(args.summarize_kernels)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}): the path would shard by block ranges with "
               "no exchange step, and nothing here drives a second GPU; run with --gpus 1", file=sys.stderr)

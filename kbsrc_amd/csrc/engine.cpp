@@ -864,8 +864,10 @@ Engine::Engine(const Plan& p, const double* H, long long nH, int Jg, hipStream_t
     // caller's budget, for engines that share the device with others)
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) free_b = 0;
-    const size_t budget = buffer_budget ? buffer_budget
-                                        : std::clamp<size_t>(free_b / 12, (size_t)1 << 30, (size_t)4 << 30);
+    size_t cap = (size_t)4 << 30;
+    if (const char* v = std::getenv("KBSRC_BUFFER_GIB"))   // tuning knob: the per-buffer cap, GiB
+        if (std::atoll(v) > 0) cap = (size_t)std::atoll(v) << 30;
+    const size_t budget = buffer_budget ? buffer_budget : std::clamp<size_t>(free_b / 12, (size_t)1 << 30, cap);
     tb_ = (int)std::max<long long>(1, std::min<long long>(ntask_, (long long)(budget / per_task)));
     if (const char* v = std::getenv("KBSRC_BATCH_TASKS")) {   // tuning knob: tasks per launch
         const long long want = std::atoll(v);
