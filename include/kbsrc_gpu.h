@@ -107,6 +107,13 @@ int kbsrc_plan_execute_timed(kbsrc_plan* plan, const double* d_padded, double* d
  * stream; the first call per slice count makes the stage tables (synchronous). */
 int kbsrc_plan_execute_split(kbsrc_plan* plan, const double* d_padded, double* d_out, int slices, void* stream);
 
+/* The same conversion with every output sample written as its double-double value before the final rounding to
+ * double: d_out_pairs holds channels x nout (hi, lo) pairs, 2 nout doubles per channel, hi + lo rounding to the
+ * sample kbsrc_plan_execute (slices 0) or kbsrc_plan_execute_split (slices > 0) writes. Exact-mode plans only. No
+ * counterpart in the reference: it exposes the internal dd result (src/resampler.cpp:80-84 rounds it away) so that
+ * its error can be stated against a high-precision restatement of the same blocks. Asynchronous on stream. */
+int kbsrc_plan_execute_dd(kbsrc_plan* plan, const double* d_padded, double* d_out_pairs, int slices, void* stream);
+
 void kbsrc_plan_destroy(kbsrc_plan* plan);
 
 /* ---- Batches: many jobs (files, channel counts, rate pairs, lengths) in one device input buffer and one device

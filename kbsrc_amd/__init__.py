@@ -70,6 +70,7 @@ def _load():
         "kbsrc_plan_stage": (I, [P, P, P, P]),
         "kbsrc_plan_execute": (I, [P, P, P, P]),
         "kbsrc_plan_execute_split": (I, [P, P, P, I, P]),
+        "kbsrc_plan_execute_dd": (I, [P, P, P, I, P]),
         "kbsrc_plan_execute_noalloc": (I, [P, P, P, P]),
         "kbsrc_plan_execute_timed": (I, [P, P, P, P, ctypes.POINTER(D)]),
         "kbsrc_plan_destroy": (None, [P]),
@@ -359,6 +360,12 @@ class Plan:
         int8-MFMA radix-16 stage with ``slices`` base-128 slices per operand. Not bit-exact; asynchronous."""
         _check(_lib.kbsrc_plan_execute_split(self._h, ctypes.c_void_p(d_padded), ctypes.c_void_p(d_out), int(slices),
                                              ctypes.c_void_p(stream)))
+
+    def execute_dd(self, d_padded: int, d_out_pairs: int, slices: int = 0, stream: int = 0):
+        """Every output sample as its dd value before the final rounding: d_out_pairs holds channels x nout (hi, lo)
+        pairs (exact-mode plans; slices 0 the plain form, > 0 the split form). Asynchronous."""
+        _check(_lib.kbsrc_plan_execute_dd(self._h, ctypes.c_void_p(d_padded), ctypes.c_void_p(d_out_pairs),
+                                          int(slices), ctypes.c_void_p(stream)))
 
     def tune(self, d_padded: int, d_out: int, stream: int = 0):
         """Execute once while measuring kernel shapes for the passes not measured before (same outputs)."""

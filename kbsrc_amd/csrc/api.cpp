@@ -802,6 +802,16 @@ int kbsrc_plan_execute_split(kbsrc_plan* pl, const double* d_padded, double* d_o
     });
 }
 
+int kbsrc_plan_execute_dd(kbsrc_plan* pl, const double* d_padded, double* d_out_pairs, int slices, void* stream) {
+    return guarded([&] {
+        if (!pl || !pl->eng || !d_padded || !d_out_pairs) throw std::invalid_argument("bad input buffer");
+        if (slices < 0) throw std::invalid_argument("option out of range");
+        in_nearest_mode(true, [&] {
+            pl->eng->execute(d_padded, d_out_pairs, static_cast<hipStream_t>(stream), nullptr, slices, true);
+        });
+    });
+}
+
 int kbsrc_plan_execute_timed(kbsrc_plan* pl, const double* d_padded, double* d_out, void* stream, double* ms) {
     return guarded([&] {
         if (!pl || !pl->eng || !d_padded || !d_out || !ms) throw std::invalid_argument("bad input buffer");

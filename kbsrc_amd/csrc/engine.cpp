@@ -677,6 +677,7 @@ void run_cfft(const CFFTPlan& f, const RunCtx& c, int ld, const double* src, int
         b.in_cstride = c.proto.in_cstride; b.in_bstride = c.proto.in_bstride; b.tdesc = c.proto.tdesc;
         b.bc = f.bc.d(); b.vf = f.vf.d();
         b.out = c.proto.out; b.nout = c.proto.nout; b.o = c.proto.o; b.margP = c.proto.margP; b.hopP = c.proto.hopP;
+        b.out_dd = c.proto.out_dd;
         const int pol = pol_one;
         b.src = src; b.src_tstride = ld == LD_REAL_DD ? 0 : ts; b.dst = scr[0]; b.dst_tstride = ts;
         timed(c, t_col, [&] { hip_check(launch_blue(pol, BLUE_PRE, ld, b, c.s), "Bluestein pre-multiply"); });
@@ -881,7 +882,8 @@ Engine::Engine(const Plan& p, const double* H, long long nH, int Jg, hipStream_t
     hip_check(hipStreamSynchronize(s), "engine set-up");   // H and the tables are in place; the caller may free H
 }
 
-void Engine::execute(const double* xx, double* out, hipStream_t s, PassTimes* times, int split) {
+void Engine::execute(const double* xx, double* out, hipStream_t s, PassTimes* times, int split, bool out_dd) {
+    if (out_dd && !dd_) throw std::invalid_argument("option out of range");   // pairs: exact-mode plans only
     if (split) {   // tables of the split form, made on first use
         if (!dd_ || !split_slices_ok(split)) throw std::invalid_argument("option out of range");
         fw_.c.init_split(s);
@@ -897,7 +899,7 @@ void Engine::execute(const double* xx, double* out, hipStream_t s, PassTimes* ti
         }
     }
     const Work w{{w_[0].d(), w_[1].d(), w_[2].d(), w_[3].d()}, {g_[0].d(), g_[1].d()}, tb_};
-    run(xx, out, s, times, split, w);
+    run(xx, out, s, times, split, w, 0, -1, out_dd);
 }
 
 void Engine::execute_tasks(const double* xx, double* out, hipStream_t s, long long t0, long long t1) {
@@ -942,13 +944,14 @@ void Engine::execute_in(const double* xx, double* out, hipStream_t s, double* me
 }
 
 void Engine::run(const double* xx, double* out, hipStream_t s, PassTimes* times, int split, const Work& w,
-                 long long tbeg, long long tend) {
+                 long long tbeg, long long tend, bool out_dd) {
     if (tend < 0) tend = ntask_;
     PassTimes local;
     const int pol_four = dd_ ? POL_DDS : POL_F64;
     const bool fused = !split && use_fixed() && env_flag("KBSRC_FUSED") && !Jg_ && fw_.c.four && iv_.c.four &&
                        has_fixed(pol_four, fw_.c.N2, LD_CPLX, ST_SPLIT) && has_fixed(pol_four, iv_.c.N1, LD_FOLD, ST_CPLX);
     RunCtx c = context(xx, out, s, split, w);
+    c.proto.out_dd = out_dd ? 1 : 0;
     c.times = times ? &local : nullptr;
     PassArgs& a = c.proto;
     double* W0 = w.w[0];

@@ -201,7 +201,10 @@ public:
     // xx: device, ch x plan.xstride padded planar input. out: device, ch x plan.nout. times: optional per-pass timing.
     // split > 0: the split-precision form (exact-mode plans only; not bit-exact): sub-FFTs whose first two stages
     // are radix 4 run them as one int8-MFMA radix-16 stage with `split` base-128 slices per operand.
-    void execute(const double* xx, double* out, hipStream_t s, PassTimes* times = nullptr, int split = 0);
+    // out_dd (exact-mode plans): each output sample as its dd value before the final rounding, a (hi, lo) pair;
+    // out holds ch x 2 nout doubles.
+    void execute(const double* xx, double* out, hipStream_t s, PassTimes* times = nullptr, int split = 0,
+                 bool out_dd = false);
     // The plain execute of tasks [t0, t1) only (task t: channel t / nblk, block t % nblk), which write the output
     // samples of their blocks; the other tasks' output is left as it is.
     void execute_tasks(const double* xx, double* out, hipStream_t s, long long t0, long long t1);
@@ -225,7 +228,7 @@ private:
         int tb;         // tasks per launch
     };
     void run(const double* xx, double* out, hipStream_t s, PassTimes* times, int split, const Work& w,
-             long long t0 = 0, long long t1 = -1);
+             long long t0 = 0, long long t1 = -1, bool out_dd = false);
     RunCtx context(const double* xx, double* out, hipStream_t s, int split, const Work& w);
     Plan p_;
     bool dd_;

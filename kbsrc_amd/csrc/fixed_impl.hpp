@@ -348,6 +348,7 @@ struct PassView {
     double* out;          // ST_OUT: sample x of this block's kept range [xlo, xhi) is out[x + xoff]
     int lde, ldg, ste, stg;
     int margP, xlo, xhi, xoff;
+    int pairs;            // ST_OUT: (hi, lo) pairs (PassArgs::out_dd), out[x + xoff] being the pair's index
     int zkeep, zh;        // zkeep >= 0: elements zkeep < i < zh - zkeep are not stored (z_unused)
 };
 template <int LD>
@@ -363,7 +364,8 @@ __device__ __forceinline__ PassView pass_view(const PassArgs& a, const TaskIO& i
     const long long shift = io.blk * a.hopP - a.o;
     const long long lo = std::max(0LL, std::min(a.hopP, -shift)), hi = std::max(0LL, std::min(a.hopP, io.nout - shift));
     const long long base = std::max(0LL, std::min(io.nout, shift));
-    v.out = a.out ? a.out + io.out_off + base : nullptr;
+    v.pairs = a.out_dd;
+    v.out = a.out ? a.out + (a.out_dd ? 2 : 1) * (io.out_off + base) : nullptr;
     v.margP = (int)a.margP; v.xlo = (int)lo; v.xhi = (int)hi; v.xoff = (int)(shift - base);
     return v;
 }
@@ -393,20 +395,28 @@ __device__ __forceinline__ void pass_store(const PassArgs& a, const PassView& w,
     if constexpr (ST == ST_CPLX) {
         gst32<A>(w.dst, (unsigned)i, v);
     } else if constexpr (ST == ST_OUT) {
+        const unsigned esz = w.pairs ? 16u : 8u;
         auto at = [&](int x) -> double* {
-            return reinterpret_cast<double*>(reinterpret_cast<char*>(w.out) + (size_t)((unsigned)(x + w.xoff) * 8u));
+            return reinterpret_cast<double*>(reinterpret_cast<char*>(w.out) + (size_t)((unsigned)(x + w.xoff) * esz));
+        };
+        // the sample's dd value rounded to double, or (out_dd) the pair itself
+        auto put = [&](int x, Val<A> y) {
+            if constexpr (A::kDD) {
+                if (w.pairs) {
+                    double* p = at(x);
+                    p[0] = y.hi;
+                    p[1] = y.lo;
+                } else {
+                    *at(x) = y.hi + y.lo;
+                }
+            } else {
+                *at(x) = y;
+            }
         };
         const int x0 = 2 * i - w.margP;
-        if (x0 >= w.xlo && x0 < w.xhi) {
-            if constexpr (A::kDD) *at(x0) = v.re.hi + v.re.lo;
-            else *at(x0) = v.re;
-        }
+        if (x0 >= w.xlo && x0 < w.xhi) put(x0, v.re);
         const int x1 = x0 + 1;
-        if (x1 >= w.xlo && x1 < w.xhi) {
-            const Val<A> ni = A::neg(v.im);
-            if constexpr (A::kDD) *at(x1) = ni.hi + ni.lo;
-            else *at(x1) = ni;
-        }
+        if (x1 >= w.xlo && x1 < w.xhi) put(x1, A::neg(v.im));
     }
 }
 

@@ -90,6 +90,18 @@ template <class A> __device__ __forceinline__ double to_f64(Val<A> v) {
     if constexpr (A::kDD) return v.hi + v.lo;
     else return v;
 }
+// output sample k of a buffer: the value rounded to double, or with pairs (PassArgs::out_dd, dd policies) the dd
+// value as the (hi, lo) pair at 2 k
+template <class A> __device__ __forceinline__ void put_sample(double* out, int pairs, long long k, Val<A> v) {
+    if constexpr (A::kDD) {
+        if (pairs) {
+            out[2 * k] = v.hi;
+            out[2 * k + 1] = v.lo;
+            return;
+        }
+    }
+    out[k] = to_f64<A>(v);
+}
 
 // Odd-radix butterfly by symmetric sums (src/dsp/fft.hpp:62-84)
 template <class A, int R>
@@ -269,16 +281,15 @@ __device__ __forceinline__ void subfft_body(const PassArgs& a, int bid) {
                 gstore<A>(a.dst + task * a.dst_tstride, i, v);
             } else {
                 // r[2i] = re, r[2i+1] = -im (src/dsp/fft.hpp:460); keep the block's hop (src/resampler.cpp:80-84)
-                double* o = a.out + io.out_off;
                 const long long x0 = 2 * i - a.margP;
                 if (x0 >= 0 && x0 < a.hopP) {
                     const long long d = io.blk * a.hopP + x0 - a.o;
-                    if (d >= 0 && d < io.nout) o[d] = to_f64<A>(v.re);
+                    if (d >= 0 && d < io.nout) put_sample<A>(a.out, a.out_dd, io.out_off + d, v.re);
                 }
                 const long long x1 = x0 + 1;
                 if (x1 >= 0 && x1 < a.hopP) {
                     const long long d = io.blk * a.hopP + x1 - a.o;
-                    if (d >= 0 && d < io.nout) o[d] = to_f64<A>(A::neg(v.im));
+                    if (d >= 0 && d < io.nout) put_sample<A>(a.out, a.out_dd, io.out_off + d, A::neg(v.im));
                 }
             }
         }
@@ -527,15 +538,14 @@ template <class A, int ST> __global__ __launch_bounds__(256) void k_blue_post(Bl
         gstore<A>(a.dst + task * a.dst_tstride, k, y);
     } else {
         const TaskIO io = task_io(a.tdesc, a.task0 + task, a.nblk, a.in_cstride, a.in_bstride, a.nout);
-        double* o = a.out + io.out_off;
         const long long x0 = 2LL * k - a.margP, x1 = x0 + 1;
         if (x0 >= 0 && x0 < a.hopP) {
             const long long d = io.blk * a.hopP + x0 - a.o;
-            if (d >= 0 && d < io.nout) o[d] = to_f64<A>(y.re);
+            if (d >= 0 && d < io.nout) put_sample<A>(a.out, a.out_dd, io.out_off + d, y.re);
         }
         if (x1 >= 0 && x1 < a.hopP) {
             const long long d = io.blk * a.hopP + x1 - a.o;
-            if (d >= 0 && d < io.nout) o[d] = to_f64<A>(A::neg(y.im));
+            if (d >= 0 && d < io.nout) put_sample<A>(a.out, a.out_dd, io.out_off + d, A::neg(y.im));
         }
     }
 }

@@ -74,6 +74,7 @@ struct PassArgs {
     // ST_OUT: unpack r[2i] = re, r[2i+1] = -im, keep [margP, margP + hopP), write out[c*nout + b*hopP + x - o]
     double* out = nullptr;
     long long nout = 0, o = 0, margP = 0, hopP = 0;
+    int out_dd = 0;                 // ST_OUT, dd policies: each sample as its (hi, lo) pair, at 2 (c*nout + d)
     FuseArgs f;
     double* gscr0 = nullptr;        // global scratch for sub-FFTs too long for LDS: ntask * nsub * n elements each
     double* gscr1 = nullptr;
@@ -132,6 +133,7 @@ struct BlueArgs {
     const double* vf = nullptr;     // inner FFT of the chirp sequence / m, m elements
     double* out = nullptr;          // ST_OUT, as PassArgs
     long long nout = 0, o = 0, margP = 0, hopP = 0;
+    int out_dd = 0;
 };
 enum BluePhase { BLUE_PRE = 0, BLUE_MID = 1, BLUE_POST = 2 };
 

@@ -82,22 +82,27 @@ template <class A, int ST> __global__ __launch_bounds__(256) void k_scatter(Pass
         gst<A>(a.dst + task * a.dst_tstride, i, v);
     } else {
         const TaskIO io = task_io(a.tdesc, a.task0 + task, a.nblk, a.in_cstride, a.in_bstride, a.nout);
-        double* o = a.out + io.out_off;
+        // the value rounded to double, or (out_dd) the dd value as its (hi, lo) pair
+        auto put = [&](long long k, Val<A> y) {
+            if constexpr (A::kDD) {
+                if (a.out_dd) {
+                    a.out[2 * k] = y.hi;
+                    a.out[2 * k + 1] = y.lo;
+                } else {
+                    a.out[k] = y.hi + y.lo;
+                }
+            } else {
+                a.out[k] = y;
+            }
+        };
         const long long x0 = 2 * i - a.margP, x1 = x0 + 1;
         if (x0 >= 0 && x0 < a.hopP) {
             const long long d = io.blk * a.hopP + x0 - a.o;
-            if (d >= 0 && d < io.nout) {
-                if constexpr (A::kDD) o[d] = v.re.hi + v.re.lo;
-                else o[d] = v.re;
-            }
+            if (d >= 0 && d < io.nout) put(io.out_off + d, v.re);
         }
         if (x1 >= 0 && x1 < a.hopP) {
             const long long d = io.blk * a.hopP + x1 - a.o;
-            if (d >= 0 && d < io.nout) {
-                const Val<A> ni = A::neg(v.im);
-                if constexpr (A::kDD) o[d] = ni.hi + ni.lo;
-                else o[d] = ni;
-            }
+            if (d >= 0 && d < io.nout) put(io.out_off + d, A::neg(v.im));
         }
     }
 }
