@@ -801,6 +801,38 @@ def bench_split(args):
     return 0
 
 
+def internal_errors(plan, xx, info, stream, slices, rate_in):
+    """The internal dd error, before the final rounding, of the plain form and the split form at `slices`, over one
+    middle block of channel 0 against the 2^-256 block restatement (tests/hp_block.py), each over the block's peak
+    (log2): what "at 2^-100" refers to. kbsrc_plan_execute_dd writes the samples as (hi, lo) pairs."""
+    import numpy as np
+    import torch
+
+    import kbsrc_amd
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import hp_block as hb
+    b = info["nblk"] // 2
+    P, Q, hop = info["P"], info["Q"], info["hop"]
+    blk = xx[b * hop * Q: b * hop * Q + info["Nb"]].cpu().numpy()
+    gc = math.gcd(info["c2"], rate_in)
+    taps = kbsrc_amd.kernel_taps(info["Lk"], info["c2"] // gc, rate_in // gc, 40.0)
+    spec = kbsrc_amd.kernel_spectrum(taps, info["Nb"], info["Nb"])
+    ref = hb.block_output(blk, spec, P, Q, info["nb"], info["marg"], False)
+    peak = float(np.abs(hb.to_double(ref)).max())
+    d0 = b * hop * P - info["o"]
+    pairs = torch.empty(2 * CH * info["nout"], dtype=torch.float64, device=xx.device)
+    out = {}
+    for name, s in (("plain", 0), (f"split_{slices}", slices)):
+        plan.execute_dd(xx.data_ptr(), pairs.data_ptr(), s, stream)
+        torch.cuda.synchronize()
+        p = pairs[2 * d0: 2 * (d0 + len(ref))].cpu().numpy().reshape(-1, 2)
+        diff = hb.fix(p[:, 0]) + hb.fix(p[:, 1]) - ref
+        err = max(math.ldexp(float(abs(int(v))), -hb.F) for v in diff) / peak
+        out[name] = round(math.log2(err), 1) if err > 0 else None
+    del pairs
+    return out
+
+
 def bench_split_conversion(args, slices):
     """The stereo 96 kHz -> 44.1 kHz 180 s exact conversion, plain and in the split form (its 320-point forward
     row sub-FFTs run their first two radix-4 stages as the int8-MFMA radix-16 stage): time per conversion, the
@@ -1042,6 +1074,11 @@ def main():
         print(f"split form, {SPLIT_SLICES_2M100} slices: {split_ms:.3f} ms/step (x{ms / split_ms:.3f} of plain), "
               f"max |diff| vs plain {diff:.3g}", file=sys.stderr, flush=True)
         del alt
+        t0 = time.perf_counter()
+        split["internal_error_over_peak_log2"] = internal_errors(plan, xx, info, stream, SPLIT_SLICES_2M100, RATE_IN)
+        print(f"internal dd error before the final rounding, one block against 2^-256 (log2 of peak): "
+              f"{split['internal_error_over_peak_log2']} ({time.perf_counter() - t0:.1f} s)", file=sys.stderr,
+              flush=True)
 
     graph_ms, graph_same = graph_timing(args, plan, xx, out)
     print(f"captured graph replay: {graph_ms:.3f} ms/step; same output: {graph_same}", file=sys.stderr, flush=True)
