@@ -801,104 +801,126 @@ def bench_split(args):
     return 0
 
 
-def internal_errors(plan, xx, info, stream, slices, rate_in):
-    """The internal dd error, before the final rounding, of the plain form and the split form at `slices`, over one
-    middle block of channel 0 against the 2^-256 block restatement (tests/hp_block.py), each over the block's peak
-    (log2): what "at 2^-100" refers to. kbsrc_plan_execute_dd writes the samples as (hi, lo) pairs."""
+def _block_reference(xx, info, rate_in, rate_out, channel=0):
+    """A middle block of one channel and its 2^-256 restatement (tests/hp_block.py): (block index, first output
+    sample of the block's kept range, fixed-point reference samples, the block's peak as a double)."""
     import numpy as np
-    import torch
 
     import kbsrc_amd
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import hp_block as hb
+    up = bool(info["up"])
     b = info["nblk"] // 2
     P, Q, hop = info["P"], info["Q"], info["hop"]
-    blk = xx[b * hop * Q: b * hop * Q + info["Nb"]].cpu().numpy()
-    gc = math.gcd(info["c2"], rate_in)
-    taps = kbsrc_amd.kernel_taps(info["Lk"], info["c2"] // gc, rate_in // gc, 40.0)
-    spec = kbsrc_amd.kernel_spectrum(taps, info["Nb"], info["Nb"])
-    ref = hb.block_output(blk, spec, P, Q, info["nb"], info["marg"], False)
-    peak = float(np.abs(hb.to_double(ref)).max())
-    d0 = b * hop * P - info["o"]
+    base = channel * info["xstride"]
+    blk = xx[base + b * hop * Q: base + b * hop * Q + info["Nb"]].cpu().numpy()
+    fk = rate_out if up else rate_in
+    gc = math.gcd(info["c2"], fk)
+    taps = kbsrc_amd.kernel_taps(info["Lk"], info["c2"] // gc, fk // gc, 40.0)
+    spec = kbsrc_amd.kernel_spectrum(taps, info["Mb"] if up else info["Nb"], info["Nb"])
+    ref = hb.block_output(blk, spec, P, Q, info["nb"], info["marg"], up)
+    return b, b * hop * P - info["o"], ref, float(np.abs(hb.to_double(ref)).max())
+
+
+def _pair_error(pairs, channel, nout, d0, ref, peak):
+    """log2 of the largest |hi + lo - ref| over the peak, for output samples d0.. of one channel of a pair buffer."""
+    import hp_block as hb
+    p = pairs[2 * (channel * nout + d0): 2 * (channel * nout + d0 + len(ref))].cpu().numpy().reshape(-1, 2)
+    diff = hb.fix(p[:, 0]) + hb.fix(p[:, 1]) - ref
+    err = max(math.ldexp(float(abs(int(v))), -hb.F) for v in diff) / peak
+    return round(math.log2(err), 1) if err > 0 else None
+
+
+def internal_errors(plan, xx, info, stream, slices, rate_in, rate_out=RATE_OUT):
+    """The internal dd error, before the final rounding, of the plain form and the split form at `slices`, over one
+    middle block of channel 0 against the 2^-256 block restatement (tests/hp_block.py), each over the block's peak
+    (log2): what "at 2^-100" refers to. kbsrc_plan_execute_dd writes the samples as (hi, lo) pairs."""
+    import torch
+    _, d0, ref, peak = _block_reference(xx, info, rate_in, rate_out)
     pairs = torch.empty(2 * CH * info["nout"], dtype=torch.float64, device=xx.device)
     out = {}
     for name, s in (("plain", 0), (f"split_{slices}", slices)):
         plan.execute_dd(xx.data_ptr(), pairs.data_ptr(), s, stream)
         torch.cuda.synchronize()
-        p = pairs[2 * d0: 2 * (d0 + len(ref))].cpu().numpy().reshape(-1, 2)
-        diff = hb.fix(p[:, 0]) + hb.fix(p[:, 1]) - ref
-        err = max(math.ldexp(float(abs(int(v))), -hb.F) for v in diff) / peak
-        out[name] = round(math.log2(err), 1) if err > 0 else None
+        out[name] = _pair_error(pairs, 0, info["nout"], d0, ref, peak)
     del pairs
     return out
 
 
 def bench_split_conversion(args, slices):
-    """The stereo 96 kHz -> 44.1 kHz 180 s exact conversion, plain and in the split form (its 320-point forward
-    row sub-FFTs run their first two radix-4 stages as the int8-MFMA radix-16 stage): time per conversion, the
-    largest difference from the plain output over every sample, and one middle block's error against the 2^-256
-    block restatement (tests/hp_block.py)."""
+    """Config 3's conversions, stereo 180 s exact at the automatic block: 96 kHz -> 44.1 kHz (its 320-point forward
+    row sub-FFTs take the radix-16 matrix stage) and 44.1 kHz -> 48 kHz (its 224-point inverse column sub-FFTs), plain
+    and in the split form at each slice count: time per conversion, the largest difference from the plain output
+    over every sample, and on one middle block against the 2^-256 restatement (tests/hp_block.py) both the error of
+    the f64 output and the internal dd error before the final rounding (kbsrc_plan_execute_dd)."""
     import numpy as np
     import torch
 
     import kbsrc_amd
     import hp_block as hb
-    from tests.helpers import input_96k_180s
+    from tests.helpers import input_44k_180s, input_96k_180s
 
-    frames = RATE_IN * SECONDS
-    x = input_96k_180s(frames)
-    plan = kbsrc_amd.Plan(frames, CH, RATE_IN, RATE_OUT, "exact")
-    info = plan.info
-    dev = torch.device("cuda:0")
-    xx = torch.empty(CH * info["xstride"], dtype=torch.float64, device=dev)
-    out = torch.empty(CH * info["nout"], dtype=torch.float64, device=dev)
-    plain = torch.empty_like(out)
-    stream = torch.cuda.current_stream(dev).cuda_stream
-    plan.stage(x, xx.data_ptr(), stream)
-    del x
+    def one(x, rate_in, rate_out):
+        frames = x.shape[0]
+        plan = kbsrc_amd.Plan(frames, CH, rate_in, rate_out, "exact")
+        info = plan.info
+        dev = torch.device("cuda:0")
+        xx = torch.empty(CH * info["xstride"], dtype=torch.float64, device=dev)
+        out = torch.empty(CH * info["nout"], dtype=torch.float64, device=dev)
+        plain = torch.empty_like(out)
+        pairs = torch.empty(2 * CH * info["nout"], dtype=torch.float64, device=dev)
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        plan.stage(x, xx.data_ptr(), stream)
 
-    def timed(run):
-        for _ in range(max(1, args.warmup)):
-            run()
-        torch.cuda.synchronize()
+        def timed(run):
+            for _ in range(max(1, args.warmup)):
+                run()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(args.steps):
+                run()
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t0) * 1e3 / args.steps
+
+        plain_ms = timed(lambda: plan.execute(xx.data_ptr(), plain.data_ptr(), stream))
+        peak_all = float(plain.abs().max().item())
         t0 = time.perf_counter()
-        for _ in range(args.steps):
-            run()
-        torch.cuda.synchronize()
-        return (time.perf_counter() - t0) * 1e3 / args.steps
+        b, d0, ref, peak = _block_reference(xx, info, rate_in, rate_out)
 
-    plain_ms = timed(lambda: plan.execute(xx.data_ptr(), plain.data_ptr(), stream))
-    peak = float(plain.abs().max().item())
-    b = info["nblk"] // 2
-    P, Q, hop = info["P"], info["Q"], info["hop"]
-    blk = xx[b * hop * Q: b * hop * Q + info["Nb"]].cpu().numpy()
-    gc = math.gcd(info["c2"], RATE_IN)
-    taps = kbsrc_amd.kernel_taps(info["Lk"], info["c2"] // gc, RATE_IN // gc, 40.0)
-    spec = kbsrc_amd.kernel_spectrum(taps, info["Nb"], info["Nb"])
-    t0 = time.perf_counter()
-    ref = hb.block_output(blk, spec, P, Q, info["nb"], info["marg"], False)
-    d0 = b * hop * P - info["o"]
+        def f64_err(y):
+            return round(math.log2(float(hb.errors(y[d0:d0 + len(ref)].cpu().numpy(), ref).max()) / peak), 1)
 
-    def block_err(y):
-        return float(hb.errors(y[d0:d0 + len(ref)].cpu().numpy(), ref).max() / peak)
+        def dd_err(s):
+            plan.execute_dd(xx.data_ptr(), pairs.data_ptr(), s, stream)
+            torch.cuda.synchronize()
+            return _pair_error(pairs, 0, info["nout"], d0, ref, peak)
 
-    e_plain = block_err(plain)
-    print(f"conversion plain: {plain_ms:.3f} ms; block {b} error 2^{math.log2(e_plain + 1e-300):.1f} of peak "
-          f"(2^-256 block in {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
-    rows = {}
-    for sl in slices:
-        ms = timed(lambda: plan.execute_split(xx.data_ptr(), out.data_ptr(), sl, stream))
-        d = float((out - plain).abs().max().item()) / peak
-        e = block_err(out)
-        rows[sl] = {"ms": round(ms, 3), "speed_vs_plain": round(plain_ms / ms, 3),
-                    "max_diff_vs_plain_over_peak_log2": round(math.log2(d), 1) if d > 0 else None,
-                    "block_err_vs_2^-256_over_peak_log2": round(math.log2(e), 1) if e > 0 else None}
-        print(f"conversion split S={sl}: {ms:.3f} ms (x{plain_ms / ms:.3f}), vs plain 2^{math.log2(d + 1e-300):.1f}, "
-              f"block error 2^{math.log2(e + 1e-300):.1f} of peak", file=sys.stderr, flush=True)
-    plan.close()
-    return {"workload": "stereo 96000 -> 44100 Hz, 180 s, exact, block 630 (f64 output)",
-            "plain": {"ms": round(plain_ms, 3), "block_err_vs_2^-256_over_peak_log2": round(math.log2(e_plain), 1)},
-            "split": rows}
+        e_plain, i_plain = f64_err(plain), dd_err(0)
+        name = f"stereo {rate_in} -> {rate_out} Hz, 180 s, exact, block {info['nb']}"
+        print(f"{name}: plain {plain_ms:.3f} ms; block {b}: f64 output error 2^{e_plain} of peak, internal dd error "
+              f"2^{i_plain} (2^-256 block in {time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+        rows = {}
+        for sl in slices:
+            ms = timed(lambda: plan.execute_split(xx.data_ptr(), out.data_ptr(), sl, stream))
+            d = float((out - plain).abs().max().item()) / peak_all
+            rows[sl] = {"ms": round(ms, 3), "speed_vs_plain": round(plain_ms / ms, 3),
+                        "max_diff_vs_plain_over_peak_log2": round(math.log2(d), 1) if d > 0 else None,
+                        "block_err_vs_2^-256_over_peak_log2": f64_err(out),
+                        "internal_dd_err_vs_2^-256_over_peak_log2": dd_err(sl)}
+            print(f"  split S={sl}: {ms:.3f} ms (x{plain_ms / ms:.3f}), vs plain 2^{rows[sl]['max_diff_vs_plain_over_peak_log2']}, "
+                  f"f64 error 2^{rows[sl]['block_err_vs_2^-256_over_peak_log2']}, internal dd error "
+                  f"2^{rows[sl]['internal_dd_err_vs_2^-256_over_peak_log2']}", file=sys.stderr, flush=True)
+        plan.close()
+        del xx, out, plain, pairs
+        torch.cuda.empty_cache()
+        return {"workload": name,
+                "plain": {"ms": round(plain_ms, 3), "block_err_vs_2^-256_over_peak_log2": e_plain,
+                          "internal_dd_err_vs_2^-256_over_peak_log2": i_plain},
+                "split": rows}
+
+    res = [one(input_96k_180s(RATE_IN * SECONDS), RATE_IN, RATE_OUT)]
+    res.append(one(input_44k_180s(), 44100, 48000))
+    return res
 
 
 def main():
