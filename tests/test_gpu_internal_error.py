@@ -76,3 +76,38 @@ def test_dd_output_error_against_the_2e256_block(fi, fo):
     assert rel[0] <= 2.0 ** -100, "plain dd beyond 2^-100 of peak"
     assert rel[15] <= 2.0 ** -100, "split form at 15 slices beyond 2^-100 of peak"
     assert rel[8] > rel[15], "8 slices should be coarser than 15"
+
+
+@pytest.mark.parametrize("fi,fo,frames,env", [(96000, 32001, 9000, None), (44100, 44101, 6000, None),
+                                              (48000, 44100, 20000, "KBSRC_GENERIC_PASSES"),
+                                              (48000, 44100, 20000, "KBSRC_GLOBAL_PASSES"),
+                                              (22050, 48001, 5000, None)])
+def test_dd_pairs_round_to_the_output_on_every_route(fi, fo, frames, env, monkeypatch):
+    """Every kernel that writes output samples writes the pairs too: Bluestein post-multiplies (96k -> 32001,
+    22.05k -> 48001), the general fold's generic passes (44.1k -> 44.101k), every pass through the generic kernel and
+    through global scratch; hi + lo rounds to the ordinary execute's sample in each."""
+    import torch
+
+    import kbsrc_amd as k
+    for v in ("KBSRC_GENERIC_PASSES", "KBSRC_GLOBAL_PASSES"):
+        monkeypatch.delenv(v, raising=False)
+    if env:
+        monkeypatch.setenv(env, "1")
+    x = np.random.default_rng(fi + fo).uniform(-0.5, 0.5, (frames, 2))
+    plan = k.Plan(frames, 2, fi, fo, "exact", length=0.05)
+    info = plan.info
+    nout = info["nout"]
+    d_pad = torch.zeros(2 * info["xstride"], dtype=torch.float64, device="cuda")
+    d_out = torch.empty(2 * nout, dtype=torch.float64, device="cuda")
+    d_pairs = torch.full((4 * nout,), float("nan"), dtype=torch.float64, device="cuda")
+    plan.stage(x, d_pad.data_ptr())
+    plan.execute(d_pad.data_ptr(), d_out.data_ptr())
+    plan.execute_dd(d_pad.data_ptr(), d_pairs.data_ptr(), 0)
+    torch.cuda.synchronize()
+    plan.close()
+    say(f"{fi} -> {fo} {env or 'default route'}: {nout} samples per channel")
+    y = d_out.cpu().numpy().reshape(2, nout)
+    p = d_pairs.cpu().numpy().reshape(2, nout, 2)
+    assert np.isfinite(p).all(), "a pair left unwritten"
+    assert np.array_equal(p[..., 0] + p[..., 1], y)
+    assert (np.abs(p[..., 1]) <= np.abs(np.spacing(p[..., 0])) / 2).all(), "pairs are normalised"
