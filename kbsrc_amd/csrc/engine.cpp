@@ -716,7 +716,8 @@ void run_cfft(const CFFTPlan& f, const RunCtx& c, int ld, const double* src, int
         b.src = scr[0]; b.src_tstride = ts; b.lde = 1; b.ldg = f.N2;
         b.dst = dst; b.dst_tstride = ts; b.ste = f.N1; b.stg = 1; b.renorm = lane && c.dd;
         b.f.M = f.N1;
-        do_pass(c, t_row, pol_four, LD_CPLX, st, b, "row pass");
+        if (c.rows && !c.collect && st == ST_CPLX) (*c.rows)(b);
+        else do_pass(c, t_row, pol_four, LD_CPLX, st, b, "row pass");
     } else {
         PassArgs a = c.proto;
         fill_sub(a, f.sd, *c.k, c.dd, pol_one, 1);
@@ -969,12 +970,6 @@ void Engine::run(const double* xx, double* out, hipStream_t s, PassTimes* times,
             run_cfft(iv_.c, c, LD_FOLD, W0, ST_OUT, nullptr, sf, &local.inv_col, &local.inv_row);
             continue;
         }
-        // forward real FFT: padded input -> z (W0); the last pass stores only the bins the bin map reads
-        double* const sf[3] = {W1, W2, W3};
-        a.zkeep = zkeep_;
-        a.zh = fw_.h;
-        run_cfft(fw_.c, c, LD_REAL_F64, xx, ST_CPLX, W0, sf, &local.fwd_col, &local.fwd_row);
-        a.zkeep = -1;
         // split, x H, bin map, inverse pre-split: z (W0) -> z' (W1)
         MidArgs m;
         m.z = W0; m.z_tstride = ts_; m.zi = W1; m.zi_tstride = ts_;
@@ -983,10 +978,36 @@ void Engine::run(const double* xx, double* out, hipStream_t s, PassTimes* times,
         m.jmax = std::min(p_.Nb / 2, p_.Mb - 1); m.up = p_.up; m.ntask = nt; m.Jg = Jg_;
         m.kchain = kchain_;
         m.tgroup = tgroup_;
-        timed(c, &local.mid, [&] { hip_check(launch_mid(dd_ ? POL_DDA : POL_F64, m, s), "mid kernel"); });
-        // inverse: z' (W1) -> output samples
-        double* const si[3] = {W0, W2, W3};
-        run_cfft(iv_.c, c, LD_CPLX, W1, ST_OUT, nullptr, si, &local.inv_col, &local.inv_row);
+        // forward real FFT: padded input -> z (W0); the last pass stores only the bins the bin map reads. Where the
+        // plan allows, the row pass and the split/fold step run as one launch: column output (W1) -> z' (W0)
+        double* zi = W1;
+        const std::function<void(PassArgs&)> rows = [&](PassArgs& b) {
+            if (!pass_offsets_fit32(b)) {   // the row pass and the split/fold kernel, as without the hook
+                do_pass(c, &local.fwd_row, pol_four, LD_CPLX, ST_CPLX, b, "row pass");
+                timed(c, &local.mid, [&] { hip_check(launch_mid(dd_ ? POL_DDA : POL_F64, m, s), "mid kernel"); });
+                return;
+            }
+            set_prune_window(b, ST_CPLX);
+            MidArgs mr = m;
+            mr.z = nullptr;
+            mr.zi = W0;
+            zi = W0;
+            timed(c, &local.fwd_row, [&] { hip_check(launch_rowmid(pol_four, b, mr, s), "row pass with the split/fold"); });
+        };
+        const bool rowmid = !split && use_fixed() && !global_forced() && !Jg_ && !p_.up && fw_.c.four && !fw_.c.blue &&
+                            rowmid_available(pol_four, fw_.c.N2, fw_.c.N1, iv_.h) && !env_off("KBSRC_ROWMID");
+        double* const sf[3] = {W1, W2, W3};
+        a.zkeep = zkeep_;
+        a.zh = fw_.h;
+        c.rows = rowmid ? &rows : nullptr;
+        run_cfft(fw_.c, c, LD_REAL_F64, xx, ST_CPLX, W0, sf, &local.fwd_col, &local.fwd_row);
+        c.rows = nullptr;
+        a.zkeep = -1;
+        if (!rowmid)
+            timed(c, &local.mid, [&] { hip_check(launch_mid(dd_ ? POL_DDA : POL_F64, m, s), "mid kernel"); });
+        // inverse: z' -> output samples
+        double* const si[3] = {zi == W1 ? W0 : W1, W2, W3};
+        run_cfft(iv_.c, c, LD_CPLX, zi, ST_OUT, nullptr, si, &local.inv_col, &local.inv_row);
     }
     if (times) {
         local.collect();

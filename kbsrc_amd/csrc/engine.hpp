@@ -1,6 +1,7 @@
 // GPU engine: FFT plans with device-resident twiddle tables, the kernel spectrum, and the overlap-save block
 // engine over all (channel, block) tasks. Host-side orchestration only; the kernels are in kernels*.hip.
 #pragma once
+#include <functional>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -151,6 +152,8 @@ struct RunCtx {
     PassTimes* times = nullptr;
     std::vector<PassStep>* collect = nullptr;   // set: passes are described here, not launched
     bool* collect_ok = nullptr;                 // cleared when a pass cannot run on the generic kernel
+    // set: a four-step transform's row pass is handed to it (its arguments complete) instead of launched
+    const std::function<void(PassArgs&)>* rows = nullptr;
 };
 // An engine's whole execute as generic-kernel steps: forward passes, the split/multiply/fold, inverse passes.
 struct EngineSteps {

@@ -9,6 +9,7 @@
 //     a 16-byte "lo" complex, two dwordx4 accesses per element; FP64 elements are the 16-byte hi half alone;
 //   - every (channel, block) task of a batch is one launch dimension, so a launch has thousands of workgroups.
 #include "kernels.hpp"
+#include "mid_impl.hpp"
 #include "split_core.hpp"
 
 #pragma clang fp contract(off)
@@ -312,94 +313,9 @@ __global__ __launch_bounds__(256) void k_subfft_multi(const PassArgs* __restrict
     subfft_body<A, LD, ST, 0>(d[s], bid - starts[s]);
 }
 
-// ---- forward split X[i] from the complex FFT output z (src/dsp/fft.hpp:365-407)
+// forward split X[i] of a global array of the complex FFT's bins (the spectrum design's k_spec_split)
 template <class A> __device__ __forceinline__ Cx<A> fwd_split(const double* z, const double* w, int h, int i) {
-    if (i == 0 || i == h) {
-        const Cx<A> z0 = gload<A>(z, 0);
-        return {i == 0 ? A::add(z0.re, z0.im) : A::sub(z0.re, z0.im), A::lift(0.0)};
-    }
-    const Val<A> half = A::lift(0.5);
-    if constexpr (!A::kDD) {   // FP64: every bin from its own pair (src/dsp/fft.hpp:366-373)
-        const Cx<A> Az = gload<A>(z, i), Bz = c_conj<A>(gload<A>(z, h - i));
-        const Cx<A> E = c_add<A>(Az, Bz), O = c_mul_mi<A>(c_sub<A>(Az, Bz));
-        const Cx<A> v = c_add<A>(E, c_mul<A>(O, gload<A>(w, i)));
-        return {A::mul(v.re, half), A::mul(v.im, half)};
-    } else {                   // dd: bins above h/2 from their conjugate partner's E and O (src/dsp/fft.hpp:397-407)
-        const bool lower = i <= h / 2;
-        const int k = lower ? i : h - i, j = h - k;
-        const Cx<A> Az = gload<A>(z, k), Bz = c_conj<A>(gload<A>(z, j));
-        const Cx<A> E = c_add<A>(Az, Bz), O = c_mul_mi<A>(c_sub<A>(Az, Bz));
-        const Cx<A> v = lower ? c_add<A>(E, c_mul<A>(O, gload<A>(w, k)))
-                              : c_add<A>(c_conj<A>(E), c_mul<A>(c_conj<A>(O), gload<A>(w, j)));
-        return {A::mul(v.re, half), A::mul(v.im, half)};
-    }
-}
-
-template <class A> __device__ __forceinline__ Val<A> load_H(const double* H, int k) { return gload_real<A>(H, k); }
-
-// spectrum bin idx in [0, hinv] after the kernel multiply and the bin map (src/resampler.cpp:64-78)
-// general fold (src/resampler.cpp:53-63): Y[idx] sums X~[j mod Nb] H[j] over every j <= Jg with j = +-idx mod Mb, in
-// ascending j, so each output bin accumulates in the reference's order without atomics
-template <class A> __device__ __forceinline__ Cx<A> gen_term(const MidArgs& a, const double* z, int j) {
-    const int jj = j % a.Nb;
-    const Cx<A> x = jj <= a.Nb / 2 ? fwd_split<A>(z, a.wf, a.h, jj) : c_conj<A>(fwd_split<A>(z, a.wf, a.h, a.Nb - jj));
-    const Val<A> g = load_H<A>(a.H, j);
-    return {A::mul(x.re, g), A::mul(x.im, g)};
-}
-template <class A> __device__ __forceinline__ Cx<A> gen_bin(const MidArgs& a, const double* z, int idx) {
-    Cx<A> y = {A::lift(0.0), A::lift(0.0)};
-    if (idx == 0 || 2 * idx == a.Mb) {   // k == 0 or k == Mb/2: real part doubled (j = 0 itself adds whole)
-        for (int j = idx; j <= a.Jg; j += a.Mb) {
-            const Cx<A> v = gen_term<A>(a, z, j);
-            if (j == 0) y = c_add<A>(y, v);
-            else y.re = A::add(y.re, A::mul(A::lift(2.0), v.re));
-        }
-        y.im = A::lift(0.0);
-        return y;
-    }
-    for (int t = 0;; ++t) {   // j = idx + t Mb adds v, j = Mb - idx + t Mb adds conj(v)
-        const int j1 = idx + t * a.Mb;
-        if (j1 > a.Jg) break;
-        y = c_add<A>(y, gen_term<A>(a, z, j1));
-        const int j2 = a.Mb - idx + t * a.Mb;
-        if (j2 > a.Jg) break;
-        y = c_add<A>(y, c_conj<A>(gen_term<A>(a, z, j2)));
-    }
-    return y;
-}
-
-template <class A> __device__ __forceinline__ Cx<A> mid_bin(const MidArgs& a, const double* z, int idx) {
-    if (a.Jg) return gen_bin<A>(a, z, idx);
-    if (!a.up) {
-        if (idx == a.hinv) {   // output Nyquist bin: twice the real part (src/resampler.cpp:71)
-            const Cx<A> x = fwd_split<A>(z, a.wf, a.h, idx);
-            return {A::mul(A::mul(A::lift(2.0), x.re), load_H<A>(a.H, idx)), A::lift(0.0)};
-        }
-        const Cx<A> x = fwd_split<A>(z, a.wf, a.h, idx);
-        const Val<A> hk = load_H<A>(a.H, idx);
-        Cx<A> y = {A::mul(x.re, hk), A::mul(x.im, hk)};
-        const int j = a.Mb - idx;   // bins above the output Nyquist fold onto Mb - j
-        if (j >= a.K && j <= a.jmax) {
-            const Cx<A> xj = fwd_split<A>(z, a.wf, a.h, j);
-            const Val<A> hj = load_H<A>(a.H, j);
-            const Cx<A> v = {A::mul(xj.re, hj), A::mul(xj.im, hj)};
-            y = c_add<A>(y, c_conj<A>(v));
-        }
-        return y;
-    }
-    const int kk = idx % a.Nb;   // upsampling: periodic extension of the input spectrum
-    const Cx<A> v = kk <= a.Nb / 2 ? fwd_split<A>(z, a.wf, a.h, kk) : c_conj<A>(fwd_split<A>(z, a.wf, a.h, a.Nb - kk));
-    const Val<A> hk = load_H<A>(a.H, idx);
-    Cx<A> y = {A::mul(v.re, hk), A::mul(v.im, hk)};
-    if (idx == a.hinv) y.im = A::lift(0.0);
-    return y;
-}
-
-// inverse pre-split (src/dsp/fft.hpp:412-458): v-form for bin k with partner j = hinv - k
-template <class A> __device__ __forceinline__ Cx<A> pre_v(Cx<A> Y, Cx<A> Yp, Cx<A> w) {
-    const Cx<A> B = c_conj<A>(Yp);
-    const Cx<A> E = c_add<A>(Y, B), D = c_mul<A>(c_sub<A>(Y, B), c_conj<A>(w));
-    return c_conj<A>(Cx<A>{A::sub(E.re, D.im), A::add(E.im, D.re)});
+    return fwd_split_at<A>([z](int k) { return gload<A>(z, k); }, w, h, i);
 }
 
 // Bin block of the q-th block in the launch order. With c = kchain > 0 the nk blocks run as the columns of a
@@ -445,20 +361,7 @@ template <class A> __device__ __forceinline__ void mid_body(const MidArgs& a, un
     if (k > a.hinv / 2) return;
     const double* z = a.z + task * a.z_tstride;
     double* zi = a.zi + task * a.zi_tstride;
-    const int j = a.hinv - k;
-    const Cx<A> Yk = mid_bin<A>(a, z, k), Yj = mid_bin<A>(a, z, j);
-    gstore_stream<A>(zi, k, pre_v<A>(Yk, Yj, gload<A>(a.wi, k)));
-    if (k && j != k) {
-        if constexpr (!A::kDD) {
-            gstore_stream<A>(zi, j, pre_v<A>(Yj, Yk, gload<A>(a.wi, j)));
-        } else {   // dd: the partner from the same E and split (src/dsp/fft.hpp:452-456)
-            const Cx<A> B = c_conj<A>(Yj);
-            const Cx<A> E = c_add<A>(Yk, B), sp = c_sub<A>(Yk, B);
-            const Cx<A> Ej = c_conj<A>(E);
-            const Cx<A> Dj = c_mul<A>(Cx<A>{A::neg(sp.re), sp.im}, c_conj<A>(gload<A>(a.wi, j)));
-            gstore_stream<A>(zi, j, c_conj<A>(Cx<A>{A::sub(Ej.re, Dj.im), A::add(Ej.im, Dj.re)}));
-        }
-    }
+    mid_pair<A>(a, [z](int i) { return gload<A>(z, i); }, k, [zi](int i, Cx<A> v) { gstore_stream<A>(zi, i, v); });
 }
 
 // Workgroups are dealt to the 8 XCDs round-robin; with tgroup > 0 each XCD takes a contiguous range of the launch

@@ -231,6 +231,11 @@ hipError_t launch_long_two_part(int policy, int ld, int st, const PassArgs& a, d
 // Sub-FFT pass through global memory, one launch per stage (lengths whose tile does not fit LDS).
 hipError_t launch_global(int policy, int ld, int st, const PassArgs& a, hipStream_t s);
 hipError_t launch_mid(int policy, const MidArgs& a, hipStream_t s);
+// The forward row pass and the split/multiply/fold step as one launch (kernels_rowmid.hip), for a row pass over n1
+// rows of n2 points of a downsampling plan: a registered shape exists for (policy, n2, n1, gcd(n1, hinv)). a: the
+// row pass (its load side and stage tables; the prune window set), m: the split/fold step (zi, its destination).
+bool rowmid_available(int policy, int n2, int n1, int hinv);
+hipError_t launch_rowmid(int policy, const PassArgs& a, const MidArgs& m, hipStream_t s);
 // One launch over several plan shapes' passes (device arrays: descriptors d[0..n), workgroup starts[0..n) with shape s
 // taking [starts[s], starts[s + 1]) of total): the generic sub-FFT pass (each d[s] with its G, ngrp set for 256
 // threads; lds = the largest tile) and the split/multiply/fold kernel (mid_workgroups each).

@@ -1,0 +1,177 @@
+// The forward row pass and the split/multiply/fold step as one kernel, for downsampling plans whose four-step
+// layout lets a workgroup own every forward bin that its inverse-input bins need.
+//
+// The forward complex FFT of h = N1 N2 points leaves bin i = k1 + N1 k2 in row k1 (the row pass's sub-FFT k1,
+// element k2). The split/fold step of the inverse-input pair (k, hinv - k) reads forward bins i and h - i (the real
+// split), Mb - i (the fold) and hinv - i (the pre-split's partner): with t = gcd(N1, hinv), and h and Mb = 2 hinv
+// multiples of t, every one of them lies in a row congruent to +-k mod t. The rows r + t m and (t - r) + t m
+// (m < N1 / t) form a unit. A workgroup runs the row sub-FFTs of U units in LDS (the row pass's operations: first
+// stage from the column pass's output, the rest in place, renormalised as the row pass stores them), then, reading
+// those bins from LDS, the split/fold step of every pair (k, hinv - k) with k = +-r mod t, k <= hinv / 2, writing
+// the inverse FFT's input. Same operations on the same values as the row pass followed by the split/fold kernel
+// (mid_impl.hpp), so the same bits; the forward spectrum never goes through HBM.
+//
+// Stereo 96 kHz -> 44.1 kHz (N1 = 315, N2 = 320, hinv = 46305): t = 315, each unit is the row pair (r, 315 - r).
+#include <cstdlib>
+
+#include "fixed_impl.hpp"
+#include "mid_impl.hpp"
+
+namespace kbsrc::gpu {
+
+namespace {
+
+template <int N1, int TT, int U> struct RowUnits {
+    static constexpr int NQ = N1 / TT;   // rows per residue
+    static constexpr int Q = 2 * NQ;     // rows per unit
+    static constexpr int G = U * Q;      // rows per workgroup
+    static constexpr int nunit = TT / 2 + 1;
+    // row of local slot g in unit group grp, or -1
+    static __device__ __forceinline__ int row(int g, int grp) {
+        const int ul = g / Q, q = g - ul * Q, side = q / NQ, mm = q - side * NQ;
+        const int u = grp * U + ul;
+        if (2 * u > TT) return -1;
+        if (side && (u == 0 || 2 * u == TT)) return -1;
+        return (side ? TT - u : u) + TT * mm;
+    }
+    // local slot of row j (a row of unit group grp)
+    static __device__ __forceinline__ int slot(int j, int grp) {
+        const int mm = j / TT, v = j - mm * TT;
+        const int u = v <= TT - v ? v : TT - v, side = v == u ? 0 : 1;
+        return (u - grp * U) * Q + side * NQ + mm;
+    }
+};
+
+template <class A, class RL, int N1, int TT, int U, int T, bool NTS>
+__global__ __launch_bounds__(T, 4) void k_rowmid(PassArgs a, MidArgs m, int ngrp) {
+    using RU = RowUnits<N1, TT, U>;
+    using S = SplitPol<A>;
+    static_assert(std::is_same_v<Val<S>, Val<A>>);
+    constexpr int N = RL::N, G = RU::G, CAP = N * G;
+    extern __shared__ double lds[];
+    const Lds<A, CAP> L{lds};
+    const int tid = threadIdx.x, wg = xcd_wg();
+    const int task = wg / ngrp, grp = wg - task * ngrp;
+    const TaskIO io = task_io(a.tdesc, a.task0 + task, a.nblk, a.in_cstride, a.in_bstride, a.nout);
+    const PassView view = pass_view<LD_CPLX>(a, io, task);
+    auto jof = [&](int g) -> int { return RU::row(g, grp); };
+    auto load_at = [&](int e, int j) -> Cx<A> { return pass_load<A, LD_CPLX>(view, e, j); };
+    const OwnTw tw{a.tw};
+    const Prune pr{a.wstart, a.wlen};
+    first_stage<A, RL, G, T>(L, tid, tw, a.oddc, jof, load_at);
+    inner_stages<A, RL, G, T>(L, tid, tw, a.oddc, pr, std::make_integer_sequence<int, RL::nst - 1>{});
+
+    // forward bin i = row j, element e, as the row pass would have stored it
+    auto zat = [&](int i) -> Cx<S> {
+        const int e = i / N1, j = i - e * N1;
+        const Cx<A> v = L.get(e * G + RU::slot(j, grp));
+        return a.renorm ? renorm_dd<A>(v) : v;
+    };
+    double* zi = m.zi + task * m.zi_tstride;
+    auto put = [&](int i, Cx<S> v) {
+        if constexpr (NTS) {
+            typedef double d2 __attribute__((ext_vector_type(2)));
+            d2* p = reinterpret_cast<d2*>(const_cast<double2*>(at32<S>(zi, (unsigned)i)));
+            if constexpr (S::kDD) {
+                __builtin_nontemporal_store(d2{v.re.hi, v.im.hi}, p);
+                __builtin_nontemporal_store(d2{v.re.lo, v.im.lo}, p + 1);
+            } else {
+                __builtin_nontemporal_store(d2{v.re, v.im}, p);
+            }
+        } else {
+            gst32<S>(zi, (unsigned)i, v);
+        }
+    };
+    // pair (k, hinv - k) for k = +-u mod TT: unit fastest (neighbouring lanes write neighbouring bins), then side
+    const int half = m.hinv / 2, mc = half / TT + 1, items = U * 2 * mc;
+    for (int w = tid; w < items; w += T) {
+        const int ul = w % U, rest = w / U, side = rest & 1, mm = rest >> 1;
+        const int u = grp * U + ul;
+        if (2 * u > TT || (side && (u == 0 || 2 * u == TT))) continue;
+        const int k = (side ? TT - u : u) + TT * mm;
+        if (k > half) continue;
+        mid_pair<S>(m, zat, k, put);
+    }
+}
+
+template <class K> hipError_t lds_attr(K kernel, size_t bytes) {
+    if (bytes <= 64 * 1024) return hipSuccess;
+    return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)bytes);
+}
+
+int gcd_int(int a, int b) {
+    while (b) {
+        const int t = a % b;
+        a = b;
+        b = t;
+    }
+    return a;
+}
+
+template <class A, class RL, int N1, int TT, int U, int T, bool NTS>
+hipError_t launch_rowmid_one(const PassArgs& a, const MidArgs& m, hipStream_t s) {
+    using RU = RowUnits<N1, TT, U>;
+    static_assert(N1 % TT == 0, "a unit is whole residue classes of the rows");
+    static_assert(RL::hold(RU::G, T) <= (A::kDD ? 8 : 16), "too many values held per thread");
+    constexpr size_t bytes = (size_t)kPlanes<A> * RL::N * RU::G * sizeof(double);
+    static_assert(bytes <= 160 * 1024, "LDS tile too large");
+    // the pass and the split/fold step must be the ones this kernel was built for
+    if (a.sd.n != RL::N || a.sd.nst != RL::nst || a.nsub != N1 || a.twmode != 0 || m.up || m.Jg || m.Mb != 2 * m.hinv ||
+        m.h != N1 * RL::N || gcd_int(N1, m.hinv) != TT || a.ntask < 1 || m.ntask != a.ntask)
+        return hipErrorInvalidValue;
+    for (int i = 0; i < RL::nst; ++i)
+        if (a.sd.st[i].r != RL::r[i] || a.sd.st[i].s != RL::stride(i)) return hipErrorInvalidValue;
+    static const hipError_t attr = lds_attr(&k_rowmid<A, RL, N1, TT, U, T, NTS>, bytes);
+    if (attr != hipSuccess) return attr;
+    const int ngrp = (RU::nunit + U - 1) / U;
+    const long long nwg = (long long)a.ntask * ngrp;
+    if (nwg > 0x7fffffffLL) return hipErrorInvalidValue;
+    hipLaunchKernelGGL((k_rowmid<A, RL, N1, TT, U, T, NTS>), dim3((unsigned)nwg), dim3(T), bytes, s, a, m, ngrp);
+    return hipGetLastError();
+}
+
+struct RowMidEntry {
+    int policy, n2, n1, tt;
+    hipError_t (*launch)(const PassArgs&, const MidArgs&, hipStream_t);
+};
+
+#define KB_ROWMID(POL, N1, TT, U, T, NTS, ...)                                                                     \
+    RowMidEntry{POL, Radices<__VA_ARGS__>::N, N1, TT,                                                              \
+                &launch_rowmid_one<PolicyOf<POL>::type, Radices<__VA_ARGS__>, N1, TT, U, T, NTS>}
+
+// Several entries for one shape: the first is the default, KBSRC_ROWMID_VARIANT=k takes the k-th.
+const RowMidEntry kRowMid[] = {
+    // stereo 96 kHz -> 44.1 kHz exact: 320-point rows, row pairs (r, 315 - r)
+    KB_ROWMID(POL_DDS, 315, 315, 2, 256, false, 4, 4, 4, 5),
+    KB_ROWMID(POL_DDS, 315, 315, 3, 512, false, 4, 4, 4, 5),
+    KB_ROWMID(POL_DDS, 315, 315, 3, 384, false, 4, 4, 4, 5),
+    KB_ROWMID(POL_DDS, 315, 315, 2, 256, true, 4, 4, 4, 5),
+    KB_ROWMID(POL_DDS, 315, 315, 3, 512, true, 4, 4, 4, 5),
+    KB_ROWMID(POL_DDS, 315, 315, 4, 512, false, 4, 4, 4, 5),
+    // stereo 96 kHz -> 44.1 kHz fast (FP64): 200-point rows, units of 16 rows (r + 25 m, 25 - r + 25 m)
+    KB_ROWMID(POL_F64, 200, 25, 1, 256, false, 4, 2, 5, 5),
+    KB_ROWMID(POL_F64, 200, 25, 1, 512, false, 4, 2, 5, 5),
+    KB_ROWMID(POL_F64, 200, 25, 1, 640, false, 4, 2, 5, 5),
+    KB_ROWMID(POL_F64, 200, 25, 1, 768, false, 4, 2, 5, 5),
+    KB_ROWMID(POL_F64, 200, 25, 1, 512, true, 4, 2, 5, 5),
+};
+
+}  // namespace
+
+bool rowmid_available(int policy, int n2, int n1, int hinv) {
+    for (const RowMidEntry& e : kRowMid)
+        if (e.policy == policy && e.n2 == n2 && e.n1 == n1 && gcd_int(n1, hinv) == e.tt) return true;
+    return false;
+}
+
+hipError_t launch_rowmid(int policy, const PassArgs& a, const MidArgs& m, hipStream_t s) {
+    int skip = 0;
+    if (const char* v = std::getenv("KBSRC_ROWMID_VARIANT")) skip = std::max(0, std::atoi(v));
+    for (const RowMidEntry& e : kRowMid)
+        if (e.policy == policy && e.n2 == a.sd.n && e.n1 == a.nsub && gcd_int(e.n1, m.hinv) == e.tt && skip-- <= 0)
+            return e.launch(a, m, s);
+    return hipErrorInvalidValue;
+}
+
+}  // namespace kbsrc::gpu

@@ -1,4 +1,5 @@
-"""The bench's readers of tool output, on recorded text (no GPU): amd-smi's power and clock report."""
+"""The bench's readers of tool output, on recorded text (no GPU): amd-smi's power and clock report, the pass labels of
+profiled dispatches and the fused pass's flop and byte model."""
 import bench
 
 # `amd-smi metric -c -p` on an MI355X under load (GFX_2 .. GFX_6 elided), as the bench reads it
@@ -46,3 +47,29 @@ def test_amd_smi_power_and_gfx_clocks():
 def test_amd_smi_unusable_text():
     assert bench.parse_amd_smi_metric("") is None
     assert bench.parse_amd_smi_metric("GPU: 0\n    POWER:\n        SOCKET_POWER: N/A\n") is None
+
+
+def test_dispatch_labels_of_both_engine_forms():
+    """Profiled dispatches are labelled by pass: five per execute with the split/fold kernel, four where the forward
+    row pass and the split/fold run as one kernel; a split-form execute (another row kernel) stays unlabelled."""
+    col = "void kbsrc::gpu::(anonymous namespace)::k_fixed<kbsrc::gpu::DDS, R<3,3,5,7>, 4, 256, 0, 0, true>(PassArgs)"
+    row = "void kbsrc::gpu::(anonymous namespace)::k_fixed<kbsrc::gpu::DDS, R<4,4,4,5>, 4, 256, 1, 0, true>(PassArgs)"
+    mid = "void kbsrc::gpu::(anonymous namespace)::k_mid<kbsrc::gpu::DDA>(MidArgs)"
+    rowmid = "void kbsrc::gpu::(anonymous namespace)::k_rowmid<kbsrc::gpu::DDS, R<4,4,4,5>, 315, 315, 2, 256, false>(...)"
+    inv_c = "void kbsrc::gpu::(anonymous namespace)::k_fixed<kbsrc::gpu::DDS, R<3,3,3,7>, 8, 256, 1, 0, true>(PassArgs)"
+    inv_r = "void kbsrc::gpu::(anonymous namespace)::k_fixed<kbsrc::gpu::DDS, R<5,7,7>, 5, 256, 1, 2, true>(PassArgs)"
+    split = "void kbsrc::gpu::(anonymous namespace)::k_fixed_split<kbsrc::gpu::DDS, R<4,4,4,5>, 4, 256, 15>(PassArgs)"
+    names = [col, row, mid, inv_c, inv_r, col, rowmid, inv_c, inv_r, "copyBuffer", col, split, mid, inv_c, inv_r]
+    assert bench._pass_of_dispatches(names) == [
+        "fwd_col", "fwd_row", "mid", "inv_col", "inv_row", "fwd_col", "row_mid", "inv_col", "inv_row", None,
+        None, None, None, None, None]
+
+
+def test_fused_pass_model():
+    """The fused pass's flops are the row pass's and the split/fold's; its bytes are the column output read and the
+    inverse input written (the forward spectrum never leaves LDS)."""
+    fl = bench.block_flops(201600, 92610, True, False)
+    by = bench.block_bytes(201600, 92610, True, 74382)
+    assert fl["row_mid"] == fl["fwd_row"] + fl["mid"]
+    assert by["row_mid"] == 32 * (100800 + 46305)
+    assert by["row_mid"] < by["fwd_row"] + by["mid"]

@@ -1,0 +1,127 @@
+"""The fused forward row pass + split/multiply/fold kernel (kbsrc_amd/csrc/kernels_rowmid.hip) on the CPU: its
+workgroup owns the rows r + t m and (t - r) + t m of the four-step layout (t = gcd(N1, hinv)) and runs the
+split/fold step of every inverse-input pair (k, hinv - k) with k = +-r mod t. Checked here:
+- every forward bin that step reads for such a pair lies in those rows, by the reference's access pattern (the real
+  split pairs bin i with h - i, src/dsp/fft.hpp:365-407; the bin map folds Mb - j onto j, src/resampler.cpp:64-78;
+  the inverse pre-split pairs k with hinv - k, src/dsp/fft.hpp:412-458), over downsampling plans of many rate pairs;
+- the kernel's slot and item maps (RowUnits::row / slot, the item loop) cover every row once and every pair once.
+No GPU."""
+from math import gcd
+
+import pytest
+
+import helpers as H
+
+
+def four_step_n1(n):
+    """The reference's column length of an n-point complex FFT: the largest divisor <= sqrt(n) (src/dsp/fft.hpp:243),
+    four-step only above 2048 points with N1 >= 8."""
+    if n <= 2048:
+        return None
+    n1 = max(d for d in range(1, int(n ** 0.5) + 1) if n % d == 0)
+    return n1 if n1 >= 8 else None
+
+
+def bins_read(k, h, Mb, hinv, K, jmax):
+    """Forward complex-FFT bins the split/fold step reads for the pair (k, hinv - k) (downsampling)."""
+    acc = set()
+
+    def x(i):   # X[i] from z[i'] and z[h - i'], i' = min(i, h - i); X[0] and X[h] from z[0]
+        if i == 0 or i == h:
+            acc.add(0)
+            return
+        lo = i if 2 * i <= h else h - i
+        acc.update((lo, h - lo))
+
+    def y(idx):
+        x(idx)
+        if idx != hinv and K <= Mb - idx <= jmax:
+            x(Mb - idx)
+
+    y(k)
+    y(hinv - k)
+    return acc
+
+
+def plans():
+    out = []
+    for rin, rout in ((96000, 44100), (48000, 44100), (96000, 48000), (192000, 44100), (88200, 48000),
+                      (44100, 32000), (48000, 22050), (176400, 96000), (32000, 22050), (192000, 88200)):
+        for mode in ("exact", "fast"):
+            p = H.oracle_plan(2_000_000, 1, rin, rout, mode)
+            if p["up"] or p["gen"]:
+                continue
+            n1 = four_step_n1(p["Nb"] // 2)
+            if n1:
+                out.append((rin, rout, mode, p["Nb"], p["Mb"], n1))
+    return out
+
+
+@pytest.mark.parametrize("rin,rout,mode,Nb,Mb,N1", plans())
+def test_split_fold_reads_only_its_units_rows(rin, rout, mode, Nb, Mb, N1):
+    h, hinv = Nb // 2, Mb // 2
+    K, jmax = Mb // 2 + 1, min(Nb // 2, Mb - 1)
+    t = gcd(N1, hinv)
+    assert h % t == 0 and Mb % t == 0
+    step = max(1, (hinv // 2) // 4000)   # every pair on small plans, a sample of ~4000 on large ones
+    for k in list(range(0, hinv // 2 + 1, step)) + [hinv // 2]:
+        rows = {(b % N1) % t for b in bins_read(k, h, Mb, hinv, K, jmax)}
+        assert rows <= {k % t, (-k) % t}, (k, rows)
+
+
+def test_registered_shapes_are_benchmark_plans():
+    """The shapes the kernel table holds: stereo 96 kHz -> 44.1 kHz exact (315 x 320, t = 315: row pairs) and fast
+    (200 x 200, t = 25: units of 16 rows)."""
+    for mode, n1, n2, t in (("exact", 315, 320, 315), ("fast", 200, 200, 25)):
+        p = H.oracle_plan(17_280_000, 2, 96000, 44100, mode)
+        assert four_step_n1(p["Nb"] // 2) == n1 and p["Nb"] // 2 == n1 * n2
+        assert gcd(n1, p["Mb"] // 2) == t
+
+
+def unit_maps(N1, TT, U, hinv):
+    """RowUnits::row over every workgroup's slots, and the rows and pairs each workgroup's item loop handles."""
+    NQ, nunit = N1 // TT, TT // 2 + 1
+    Q = 2 * NQ
+    G, ngrp, half = U * Q, (nunit + U - 1) // U, hinv // 2
+    mc = half // TT + 1
+    groups = []
+    for grp in range(ngrp):
+        rows = {}
+        for g in range(G):
+            ul, q = divmod(g, Q)
+            side, mm = divmod(q, NQ)
+            u = grp * U + ul
+            if 2 * u > TT or (side and (u == 0 or 2 * u == TT)):
+                continue
+            r = (TT - u if side else u) + TT * mm
+            m2, v = divmod(r, TT)   # RowUnits::slot
+            uu = min(v, TT - v)
+            assert (uu - grp * U) * Q + (0 if v == uu else 1) * NQ + m2 == g
+            rows[g] = r
+        ks = []
+        for w in range(U * 2 * mc):
+            ul, rest = w % U, w // U
+            side, mm = rest & 1, rest >> 1
+            u = grp * U + ul
+            if 2 * u > TT or (side and (u == 0 or 2 * u == TT)):
+                continue
+            k = (TT - u if side else u) + TT * mm
+            if k <= half:
+                ks.append(k)
+        groups.append((rows, ks))
+    return groups
+
+
+@pytest.mark.parametrize("N1,TT,U,hinv", [(315, 315, 1, 46305), (315, 315, 2, 46305), (315, 315, 3, 46305),
+                                          (315, 315, 4, 46305), (200, 25, 1, 18375), (200, 25, 2, 18375),
+                                          (24, 8, 1, 1000), (24, 12, 2, 1200), (40, 10, 3, 990)])
+def test_workgroups_cover_rows_and_pairs_once(N1, TT, U, hinv):
+    groups = unit_maps(N1, TT, U, hinv)
+    rows = sorted(r for g, _ in groups for r in g.values())
+    assert rows == list(range(N1))
+    pairs = sorted(k for _, ks in groups for k in ks)
+    assert pairs == list(range(hinv // 2 + 1))
+    for g, ks in groups:   # each pair's rows (k and hinv - k, mod t) are the workgroup's own
+        own = {r % TT for r in g.values()}
+        for k in ks:
+            assert k % TT in own and (hinv - k) % TT in own
