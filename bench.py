@@ -231,6 +231,8 @@ def summarize_split(d):
             m = re.search(r"k_fixed_split<.*, (\d+)>\(", name)
             conv.append((int(m.group(1)) if m else 0,
                          f"split forward row pass, S = {m.group(1)}" if m else "plain forward row pass (320 points)", v))
+        elif "Radices<4, 4, 4, 5>" in name and "k_rowmid<" in name:
+            conv.append((-1, "plain forward row pass + split/fold (one kernel)", v))
     out = ["# Split-precision radix-16 DFT (int8 MFMA slices), the plain dd radix-16 pass and the plain pass with its "
            "exchange inside the wave:",
            "# 4,194,304 vectors of 16 complex dd values (67,108,864 points, 32 B in + 32 B out per point = 4.295 GB "
@@ -245,7 +247,8 @@ def summarize_split(d):
     for _, _, label, v in sorted(stage):
         avg = sum(v) / len(v)
         out.append(f"  {label:44s} {len(v):8d} {avg:9.1f} {min(v):9.1f} {max(v):9.1f} {gb / (avg / 1e6):18.0f}")
-    out.append("# conversion row passes (per launch; 214 tasks, sub-FFTs of 320 points):")
+    out.append("# conversion row passes (per launch; 214 tasks, sub-FFTs of 320 points; the plain form runs its forward rows "
+               "and the split/fold step as one kernel, the split form its split row pass and then the split/fold kernel):")
     for _, label, v in sorted(conv):
         out.append(f"  {label:44s} {len(v):8d} {sum(v) / len(v):9.1f} {min(v):9.1f} {max(v):9.1f}")
     print("\n".join(out))

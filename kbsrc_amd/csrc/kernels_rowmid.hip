@@ -36,22 +36,37 @@ template <int N1, int TT, int U> struct RowUnits {
     }
     // local slot of row j (a row of unit group grp)
     static __device__ __forceinline__ int slot(int j, int grp) {
-        const int mm = j / TT, v = j - mm * TT;
+        int mm = 0, v = j;
+        if constexpr (NQ > 1) {
+            mm = j / TT;
+            v = j - mm * TT;
+        }
         const int u = v <= TT - v ? v : TT - v, side = v == u ? 0 : 1;
         return (u - grp * U) * Q + side * NQ + mm;
     }
 };
 
+// Workgroup wg's task and unit group: tasks in groups of tgroup, each group's unit groups in turn with its tasks
+// fastest, so that the tasks of one unit group, which read the same entries of the kernel spectrum and the split
+// twiddles (the same residues mod t), run one after another on one XCD (xcd_wg) and find them in its L2.
+__device__ __forceinline__ void rowmid_order(int wg, int ntask, int ngrp, int tgroup, int& task, int& grp) {
+    const int per = tgroup * ngrp, blk = wg / per, t0 = blk * tgroup, gs = min(tgroup, ntask - t0);
+    const int r = wg - blk * per;
+    grp = r / gs;
+    task = t0 + (r - grp * gs);
+}
+
 template <class A, class RL, int N1, int TT, int U, int T, bool NTS>
-__global__ __launch_bounds__(T, 4) void k_rowmid(PassArgs a, MidArgs m, int ngrp) {
+__global__ __launch_bounds__(T, 4) void k_rowmid(PassArgs a, MidArgs m, int ngrp, int tgroup) {
     using RU = RowUnits<N1, TT, U>;
     using S = SplitPol<A>;
     static_assert(std::is_same_v<Val<S>, Val<A>>);
     constexpr int N = RL::N, G = RU::G, CAP = N * G;
     extern __shared__ double lds[];
     const Lds<A, CAP> L{lds};
-    const int tid = threadIdx.x, wg = xcd_wg();
-    const int task = wg / ngrp, grp = wg - task * ngrp;
+    const int tid = threadIdx.x;
+    int task, grp;
+    rowmid_order(xcd_wg(), a.ntask, ngrp, tgroup, task, grp);
     const TaskIO io = task_io(a.tdesc, a.task0 + task, a.nblk, a.in_cstride, a.in_bstride, a.nout);
     const PassView view = pass_view<LD_CPLX>(a, io, task);
     auto jof = [&](int g) -> int { return RU::row(g, grp); };
@@ -60,12 +75,16 @@ __global__ __launch_bounds__(T, 4) void k_rowmid(PassArgs a, MidArgs m, int ngrp
     const Prune pr{a.wstart, a.wlen};
     first_stage<A, RL, G, T>(L, tid, tw, a.oddc, jof, load_at);
     inner_stages<A, RL, G, T>(L, tid, tw, a.oddc, pr, std::make_integer_sequence<int, RL::nst - 1>{});
+    if (a.renorm) {   // the row pass's stores renormalise each element: once here, in place
+        for (int w = tid; w < CAP; w += T)
+            if (jof(w % G) >= 0) L.put(w, renorm_dd<A>(L.get(w)));
+        __syncthreads();
+    }
 
     // forward bin i = row j, element e, as the row pass would have stored it
     auto zat = [&](int i) -> Cx<S> {
         const int e = i / N1, j = i - e * N1;
-        const Cx<A> v = L.get(e * G + RU::slot(j, grp));
-        return a.renorm ? renorm_dd<A>(v) : v;
+        return L.get(e * G + RU::slot(j, grp));
     };
     double* zi = m.zi + task * m.zi_tstride;
     auto put = [&](int i, Cx<S> v) {
@@ -109,7 +128,7 @@ int gcd_int(int a, int b) {
     return a;
 }
 
-template <class A, class RL, int N1, int TT, int U, int T, bool NTS>
+template <class A, class RL, int N1, int TT, int U, int T, bool NTS, int TG>
 hipError_t launch_rowmid_one(const PassArgs& a, const MidArgs& m, hipStream_t s) {
     using RU = RowUnits<N1, TT, U>;
     static_assert(N1 % TT == 0, "a unit is whole residue classes of the rows");
@@ -127,7 +146,10 @@ hipError_t launch_rowmid_one(const PassArgs& a, const MidArgs& m, hipStream_t s)
     const int ngrp = (RU::nunit + U - 1) / U;
     const long long nwg = (long long)a.ntask * ngrp;
     if (nwg > 0x7fffffffLL) return hipErrorInvalidValue;
-    hipLaunchKernelGGL((k_rowmid<A, RL, N1, TT, U, T, NTS>), dim3((unsigned)nwg), dim3(T), bytes, s, a, m, ngrp);
+    int tgroup = TG;
+    if (const char* v = std::getenv("KBSRC_ROWMID_TGROUP")) tgroup = std::max(1, std::atoi(v));   // tuning knob
+    hipLaunchKernelGGL((k_rowmid<A, RL, N1, TT, U, T, NTS>), dim3((unsigned)nwg), dim3(T), bytes, s, a, m, ngrp,
+                       std::min(tgroup, a.ntask));
     return hipGetLastError();
 }
 
@@ -136,25 +158,31 @@ struct RowMidEntry {
     hipError_t (*launch)(const PassArgs&, const MidArgs&, hipStream_t);
 };
 
-#define KB_ROWMID(POL, N1, TT, U, T, NTS, ...)                                                                     \
+#define KB_ROWMID(POL, N1, TT, U, T, NTS, TG, ...)                                                                 \
     RowMidEntry{POL, Radices<__VA_ARGS__>::N, N1, TT,                                                              \
-                &launch_rowmid_one<PolicyOf<POL>::type, Radices<__VA_ARGS__>, N1, TT, U, T, NTS>}
+                &launch_rowmid_one<PolicyOf<POL>::type, Radices<__VA_ARGS__>, N1, TT, U, T, NTS, TG>}
 
-// Several entries for one shape: the first is the default, KBSRC_ROWMID_VARIANT=k takes the k-th.
+// Several entries for one shape: the first is the default, KBSRC_ROWMID_VARIANT=k takes the k-th. TG: tasks per
+// group of the workgroup order (rowmid_order), measured: exact 1.617, 1.601, 1.595 and 1.589 ms per conversion at 1,
+// 2, 4 and 8; fast 0.423, 0.432, 0.449 and 0.486 ms per launch.
 const RowMidEntry kRowMid[] = {
-    // stereo 96 kHz -> 44.1 kHz exact: 320-point rows, row pairs (r, 315 - r)
-    KB_ROWMID(POL_DDS, 315, 315, 2, 256, false, 4, 4, 4, 5),
-    KB_ROWMID(POL_DDS, 315, 315, 3, 512, false, 4, 4, 4, 5),
-    KB_ROWMID(POL_DDS, 315, 315, 3, 384, false, 4, 4, 4, 5),
-    KB_ROWMID(POL_DDS, 315, 315, 2, 256, true, 4, 4, 4, 5),
-    KB_ROWMID(POL_DDS, 315, 315, 3, 512, true, 4, 4, 4, 5),
-    KB_ROWMID(POL_DDS, 315, 315, 4, 512, false, 4, 4, 4, 5),
-    // stereo 96 kHz -> 44.1 kHz fast (FP64): 200-point rows, units of 16 rows (r + 25 m, 25 - r + 25 m)
-    KB_ROWMID(POL_F64, 200, 25, 1, 256, false, 4, 2, 5, 5),
-    KB_ROWMID(POL_F64, 200, 25, 1, 512, false, 4, 2, 5, 5),
-    KB_ROWMID(POL_F64, 200, 25, 1, 640, false, 4, 2, 5, 5),
-    KB_ROWMID(POL_F64, 200, 25, 1, 768, false, 4, 2, 5, 5),
-    KB_ROWMID(POL_F64, 200, 25, 1, 512, true, 4, 2, 5, 5),
+    // stereo 96 kHz -> 44.1 kHz exact: 320-point rows, row pairs (r, 315 - r); per launch (214 tasks) 0.62-0.635 ms
+    // for 2 pairs on 256 threads (0.654-0.670 before the in-place renormalisation and the task groups; then 3 pairs
+    // on 512 threads the same, 0.67 with non-temporal stores, 0.688 for 4 on 512, 0.742 for 1 on 128, 0.80-0.81 for 2
+    // on 320, three workgroups per CU, and 3 on 384); the two kernels it replaces: 0.665-0.675
+    KB_ROWMID(POL_DDS, 315, 315, 2, 256, false, 8, 4, 4, 4, 5),
+    KB_ROWMID(POL_DDS, 315, 315, 3, 512, false, 8, 4, 4, 4, 5),
+    KB_ROWMID(POL_DDS, 315, 315, 3, 384, false, 8, 4, 4, 4, 5),
+    KB_ROWMID(POL_DDS, 315, 315, 2, 256, true, 8, 4, 4, 4, 5),
+    KB_ROWMID(POL_DDS, 315, 315, 3, 512, true, 8, 4, 4, 4, 5),
+    KB_ROWMID(POL_DDS, 315, 315, 4, 512, false, 8, 4, 4, 4, 5),
+    // stereo 96 kHz -> 44.1 kHz fast (FP64): 200-point rows, units of 16 rows (r + 25 m, 25 - r + 25 m); per launch
+    // (860 tasks) 0.423 ms at 512 threads, 0.430 at 768, 0.451 at 256, 0.459 at 640, 0.687 with non-temporal stores
+    KB_ROWMID(POL_F64, 200, 25, 1, 512, false, 1, 4, 2, 5, 5),
+    KB_ROWMID(POL_F64, 200, 25, 1, 256, false, 1, 4, 2, 5, 5),
+    KB_ROWMID(POL_F64, 200, 25, 1, 640, false, 1, 4, 2, 5, 5),
+    KB_ROWMID(POL_F64, 200, 25, 1, 768, false, 1, 4, 2, 5, 5),
+    KB_ROWMID(POL_F64, 200, 25, 1, 512, true, 1, 4, 2, 5, 5),
 };
 
 }  // namespace

@@ -125,3 +125,21 @@ def test_workgroups_cover_rows_and_pairs_once(N1, TT, U, hinv):
         own = {r % TT for r in g.values()}
         for k in ks:
             assert k % TT in own and (hinv - k) % TT in own
+
+
+def rowmid_order(wg, ntask, ngrp, tgroup):
+    """The kernel's workgroup order: tasks in groups of tgroup, each group's unit groups in turn, tasks fastest."""
+    per = tgroup * ngrp
+    blk = wg // per
+    t0 = blk * tgroup
+    gs = min(tgroup, ntask - t0)
+    r = wg - blk * per
+    grp = r // gs
+    return t0 + (r - grp * gs), grp
+
+
+@pytest.mark.parametrize("ntask,ngrp,tgroup", [(214, 79, 4), (860, 13, 4), (5, 79, 4), (1, 13, 4), (7, 53, 3),
+                                               (214, 79, 1), (6, 5, 6)])
+def test_workgroup_order_is_a_bijection(ntask, ngrp, tgroup):
+    seen = [rowmid_order(w, ntask, ngrp, min(tgroup, ntask)) for w in range(ntask * ngrp)]
+    assert sorted(seen) == [(t, g) for t in range(ntask) for g in range(ngrp)]
