@@ -91,7 +91,12 @@ def block_flops(Nb, Mb, dd, up):
     K = Mb // 2 + 1
     fold = 0 if up else max(0, min(Nb // 2, Mb - 1) - K + 1)
     mid = (Nb // 4) * (12 * add + 12 * mul) + (Mb // 4) * (12 * add + 8 * mul) + K * 2 * mul + fold * (2 * mul + 2 * add)
-    return {"fwd_col": fc, "fwd_row": fr, "mid": mid, "row_mid": fr + mid, "inv_col": ic, "inv_row": ir + Mb}
+    return {"fwd_col": fc, "fwd_row": fr, "mid": mid, "inv_col": ic, "inv_row": ir + Mb}
+
+
+def pass_flops(fl, p):
+    """Flops of pass p from block_flops' five; row_mid, the forward row pass fused with the split/fold, does both."""
+    return fl["fwd_row"] + fl["mid"] if p == "row_mid" else fl[p]
 
 
 def block_bytes(Nb, Mb, dd, hop_out):
@@ -1055,8 +1060,8 @@ def main():
     fl = block_flops(info["Nb"], info["Mb"], args.mode == "exact", bool(info["up"]))
     passes = {}
     for k in kt:
-        tf = fl[k] * tasks / (kt[k] / 1e3) / 1e12 if kt[k] > 0 else 0.0
-        passes[k] = {"ms": round(kt[k], 4), "gflop": round(fl[k] * tasks / 1e9, 3), "tflops": round(tf, 2)}
+        tf = pass_flops(fl, k) * tasks / (kt[k] / 1e3) / 1e12 if kt[k] > 0 else 0.0
+        passes[k] = {"ms": round(kt[k], 4), "gflop": round(pass_flops(fl, k) * tasks / 1e9, 3), "tflops": round(tf, 2)}
     dom = max(kt, key=kt.get)
     dd = args.mode == "exact"
     by = block_bytes(info["Nb"], info["Mb"], dd, info["hop"] * info["P"])
@@ -1068,11 +1073,11 @@ def main():
     if dd:
         # dd arithmetic: ~12 flops per compulsory byte in the dominant pass, above the FP64 ridge (78.6 TFLOP/s over
         # 8 TB/s = 9.8 flop/B): bound by FP64 instruction issue on the vector ALUs
-        achieved = fl[dom] * tasks / (kt[dom] / 1e3) / 1e12
+        achieved = pass_flops(fl, dom) * tasks / (kt[dom] / 1e3) / 1e12
         roof = {"bound": "fp64-valu", "kernel": dom, "achieved": round(achieved, 3), "peak": FP64_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 4)}
         if prof_us:
-            roof["frac_from_profile"] = round(fl[dom] * tasks / (prof_us / 1e6) / 1e12 / FP64_PEAK_TFLOPS, 4)
+            roof["frac_from_profile"] = round(pass_flops(fl, dom) * tasks / (prof_us / 1e6) / 1e12 / FP64_PEAK_TFLOPS, 4)
     else:
         # FP64 arithmetic: ~1.4 flops per compulsory byte, far below the ridge: HBM bound
         achieved = passes[dom]["gbs"]
@@ -1080,7 +1085,7 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4)}
     roof["avg_launch_us_events"] = round(kt[dom] * 1e3, 1)
     roof["avg_launch_us_profile"] = prof_us
-    roof["algorithmic_per_launch"] = (f"{fl[dom] * tasks / 1e9:.3f} GFLOP" if dd
+    roof["algorithmic_per_launch"] = (f"{pass_flops(fl, dom) * tasks / 1e9:.3f} GFLOP" if dd
                                       else f"{passes[dom]['algorithmic_gb']:.4f} GB")
     roof["traffic"] = committed_traffic(args.mode, dom)
     # the same pass's VALU issue rate (SQ_ACTIVE_INST_VALU against one instruction per 4 cycles per SIMD): dd adds

@@ -70,6 +70,7 @@ def test_fused_pass_model():
     inverse input written (the forward spectrum never leaves LDS)."""
     fl = bench.block_flops(201600, 92610, True, False)
     by = bench.block_bytes(201600, 92610, True, 74382)
-    assert fl["row_mid"] == fl["fwd_row"] + fl["mid"]
+    assert bench.pass_flops(fl, "row_mid") == fl["fwd_row"] + fl["mid"]
+    assert sorted(fl) == ["fwd_col", "fwd_row", "inv_col", "inv_row", "mid"]   # the step's total sums these once
     assert by["row_mid"] == 32 * (100800 + 46305)
     assert by["row_mid"] < by["fwd_row"] + by["mid"]
