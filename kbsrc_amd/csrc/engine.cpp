@@ -994,7 +994,7 @@ void Engine::run(const double* xx, double* out, hipStream_t s, PassTimes* times,
             zi = W0;
             timed(c, &local.fwd_row, [&] { hip_check(launch_rowmid(pol_four, b, mr, s), "row pass with the split/fold"); });
         };
-        const bool rowmid = !split && use_fixed() && !global_forced() && !Jg_ && !p_.up && fw_.c.four && !fw_.c.blue &&
+        const bool rowmid = !split && use_fixed() && !global_forced() && !Jg_ && fw_.c.four && !fw_.c.blue &&
                             rowmid_available(pol_four, fw_.c.N2, fw_.c.N1, iv_.h) && !env_off("KBSRC_ROWMID");
         double* const sf[3] = {W1, W2, W3};
         a.zkeep = zkeep_;

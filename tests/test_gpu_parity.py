@@ -192,10 +192,10 @@ def test_long_column_parts_give_the_one_pass_bits(mode, var, val, monkeypatch):
     assert y2.tobytes() == ref2.tobytes()
 
 
-def _rowmid_run(x, mode, block, lpc=True):
-    """One conversion 96 kHz -> 44.1 kHz through plan/stage/execute_timed: (output, pass times)."""
+def _rowmid_run(x, rin, rout, mode, block, lpc=True):
+    """One conversion through plan/stage/execute_timed: (output, pass times, plan)."""
     import torch
-    with gpu().Plan(len(x), x.shape[1], 96000, 44100, mode, block=block, lpc_edges=lpc) as plan:
+    with gpu().Plan(len(x), x.shape[1], rin, rout, mode, block=block, lpc_edges=lpc) as plan:
         info = plan.info
         xx = torch.empty(x.shape[1] * info["xstride"], dtype=torch.float64, device="cuda")
         out = torch.empty(x.shape[1] * info["nout"], dtype=torch.float64, device="cuda")
@@ -205,12 +205,12 @@ def _rowmid_run(x, mode, block, lpc=True):
         return out.cpu().numpy(), t, info
 
 
-@pytest.mark.parametrize("mode,block,nvar", [("exact", 630, 6), ("fast", 250, 5)])
-def test_row_pass_with_split_fold_gives_the_separate_bits(mode, block, nvar, monkeypatch):
+@pytest.mark.parametrize("rin,rout,mode,block,nvar", [(96000, 44100, "exact", 630, 6), (96000, 44100, "fast", 250, 5)])
+def test_row_pass_with_split_fold_gives_the_separate_bits(rin, rout, mode, block, nvar, monkeypatch):
     """The forward row pass fused with the split/multiply/fold step (one launch, the forward spectrum kept in LDS)
     against the row pass and the split/fold kernel it replaces (KBSRC_ROWMID=0), every registered workgroup shape, on
-    the benchmark's transform shapes (exact: 315 x 320 forward, row pairs; fast: 200 x 200, units of 16 rows) with 1,
-    2 and 3 channels of several blocks, LPC edges on and off."""
+    the registered transform shapes (96 kHz -> 44.1 kHz exact: 315 x 320 forward, row pairs; fast: 200 x 200, units of
+    16 rows) with 1, 2 and 3 channels of several blocks, LPC edges on and off."""
     monkeypatch.delenv("KBSRC_ROWMID", raising=False)
     monkeypatch.delenv("KBSRC_ROWMID_VARIANT", raising=False)
     rng = np.random.default_rng(23)
@@ -218,13 +218,13 @@ def test_row_pass_with_split_fold_gives_the_separate_bits(mode, block, nvar, mon
                                                                       (400000, 3, False))]
     for x, lpc in cases:
         monkeypatch.setenv("KBSRC_ROWMID", "0")
-        ref, t0, info = _rowmid_run(x, mode, block, lpc)
+        ref, t0, info = _rowmid_run(x, rin, rout, mode, block, lpc)
         assert t0["mid"] > 0
         monkeypatch.delenv("KBSRC_ROWMID")
         for v in range(nvar):
             monkeypatch.setenv("KBSRC_ROWMID_VARIANT", str(v))
-            say(f"96k->44.1k fused row/split-fold {mode} {x.shape} lpc={lpc} variant {v} (Nb {info['Nb']})")
-            y, t, _ = _rowmid_run(x, mode, block, lpc)
+            say(f"{rin}->{rout} fused row/split-fold {mode} {x.shape} lpc={lpc} variant {v} (Nb {info['Nb']})")
+            y, t, _ = _rowmid_run(x, rin, rout, mode, block, lpc)
             assert t["mid"] == 0 and t["fwd_row"] > 0   # the fused launch ran
             assert y.tobytes() == ref.tobytes()
         monkeypatch.delenv("KBSRC_ROWMID_VARIANT")
