@@ -232,7 +232,7 @@ hipError_t launch_long_two_part(int policy, int ld, int st, const PassArgs& a, d
 hipError_t launch_global(int policy, int ld, int st, const PassArgs& a, hipStream_t s);
 hipError_t launch_mid(int policy, const MidArgs& a, hipStream_t s);
 // The forward row pass and the split/multiply/fold step as one launch (kernels_rowmid.hip), for a row pass over n1
-// rows of n2 points (no general fold): a registered shape exists for (policy, n2, n1, gcd(n1, hinv)). a: the
+// rows of n2 points (no general fold): a registered shape exists for (policy, n2, n1, hinv). a: the
 // row pass (its load side and stage tables; the prune window set), m: the split/fold step (zi, its destination).
 bool rowmid_available(int policy, int n2, int n1, int hinv);
 hipError_t launch_rowmid(int policy, const PassArgs& a, const MidArgs& m, hipStream_t s);

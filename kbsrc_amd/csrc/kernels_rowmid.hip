@@ -14,6 +14,7 @@
 //
 // Stereo 96 kHz -> 44.1 kHz (N1 = 315, N2 = 320, hinv = 46305): t = 315, each unit is the row pair (r, 315 - r).
 #include <cstdlib>
+#include <numeric>
 
 #include "fixed_impl.hpp"
 #include "mid_impl.hpp"
@@ -155,13 +156,15 @@ hipError_t launch_rowmid_one(const PassArgs& a, const MidArgs& m, hipStream_t s)
 }
 
 struct RowMidEntry {
-    int policy, n2, n1, tt;
+    int policy, n2, n1, hinv;
     hipError_t (*launch)(const PassArgs&, const MidArgs&, hipStream_t);
 };
 
-#define KB_ROWMID(POL, N1, TT, U, T, NTS, TG, ...)                                                                 \
-    RowMidEntry{POL, Radices<__VA_ARGS__>::N, N1, TT,                                                              \
-                &launch_rowmid_one<PolicyOf<POL>::type, Radices<__VA_ARGS__>, N1, TT, U, T, NTS, TG>}
+// An entry serves one plan shape: the row pass (policy, N2-point rows, N1 of them) and the inverse input's hinv, for
+// which it was measured faster than the two kernels (the split/fold step's share of the work grows with hinv).
+#define KB_ROWMID(POL, N1, HINV, U, T, NTS, TG, ...)                                                               \
+    RowMidEntry{POL, Radices<__VA_ARGS__>::N, N1, HINV,                                                            \
+                &launch_rowmid_one<PolicyOf<POL>::type, Radices<__VA_ARGS__>, N1, std::gcd(N1, HINV), U, T, NTS, TG>}
 
 // Several entries for one shape: the first is the default, KBSRC_ROWMID_VARIANT=k takes the k-th. TG: tasks per
 // group of the workgroup order (rowmid_order), measured: exact 1.617, 1.601, 1.595 and 1.589 ms per conversion at 1,
@@ -171,29 +174,40 @@ const RowMidEntry kRowMid[] = {
     // for 2 pairs on 256 threads (0.654-0.670 before the in-place renormalisation and the task groups; then 3 pairs
     // on 512 threads the same, 0.67 with non-temporal stores, 0.688 for 4 on 512, 0.742 for 1 on 128, 0.80-0.81 for 2
     // on 320, three workgroups per CU, and 3 on 384); the two kernels it replaces: 0.665-0.675
-    KB_ROWMID(POL_DDS, 315, 315, 2, 256, false, 8, 4, 4, 4, 5),
-    KB_ROWMID(POL_DDS, 315, 315, 3, 512, false, 8, 4, 4, 4, 5),
-    KB_ROWMID(POL_DDS, 315, 315, 3, 384, false, 8, 4, 4, 4, 5),
-    KB_ROWMID(POL_DDS, 315, 315, 2, 256, true, 8, 4, 4, 4, 5),
-    KB_ROWMID(POL_DDS, 315, 315, 3, 512, true, 8, 4, 4, 4, 5),
-    KB_ROWMID(POL_DDS, 315, 315, 4, 512, false, 8, 4, 4, 4, 5),
+    KB_ROWMID(POL_DDS, 315, 46305, 2, 256, false, 8, 4, 4, 4, 5),
+    KB_ROWMID(POL_DDS, 315, 46305, 3, 512, false, 8, 4, 4, 4, 5),
+    KB_ROWMID(POL_DDS, 315, 46305, 3, 384, false, 8, 4, 4, 4, 5),
+    KB_ROWMID(POL_DDS, 315, 46305, 2, 256, true, 8, 4, 4, 4, 5),
+    KB_ROWMID(POL_DDS, 315, 46305, 3, 512, true, 8, 4, 4, 4, 5),
+    KB_ROWMID(POL_DDS, 315, 46305, 4, 512, false, 8, 4, 4, 4, 5),
     // stereo 96 kHz -> 44.1 kHz fast (FP64): 200-point rows, units of 16 rows (r + 25 m, 25 - r + 25 m); per launch
     // (860 tasks) 0.423 ms at 512 threads, 0.430 at 768, 0.451 at 256, 0.459 at 640, 0.687 with non-temporal stores
-    KB_ROWMID(POL_F64, 200, 25, 1, 512, false, 1, 4, 2, 5, 5),
-    KB_ROWMID(POL_F64, 200, 25, 1, 256, false, 1, 4, 2, 5, 5),
-    KB_ROWMID(POL_F64, 200, 25, 1, 640, false, 1, 4, 2, 5, 5),
-    KB_ROWMID(POL_F64, 200, 25, 1, 768, false, 1, 4, 2, 5, 5),
-    KB_ROWMID(POL_F64, 200, 25, 1, 512, true, 1, 4, 2, 5, 5),
-    // not registered: stereo 44.1 kHz -> 48 kHz exact (upsampling; 245-point rows, units of 6 rows r + 63 m and
-    // 63 - r + 63 m; same bits): 0.469-0.565 ms per launch on 320-512 threads against 0.434 for the two kernels, its
-    // radix-5 and -7 stages filling 41-57% of the lanes of six rows
+    KB_ROWMID(POL_F64, 200, 18375, 1, 512, false, 1, 4, 2, 5, 5),
+    KB_ROWMID(POL_F64, 200, 18375, 1, 256, false, 1, 4, 2, 5, 5),
+    KB_ROWMID(POL_F64, 200, 18375, 1, 640, false, 1, 4, 2, 5, 5),
+    KB_ROWMID(POL_F64, 200, 18375, 1, 768, false, 1, 4, 2, 5, 5),
+    KB_ROWMID(POL_F64, 200, 18375, 1, 512, true, 1, 4, 2, 5, 5),
+    // other downsampling plans of long inputs between 44.1 and 192 kHz, exact (60 s stereo conversions, fused
+    // against two kernels): 88.2 kHz -> 44.1 kHz 0.566 against 0.588 ms, 176.4 -> 44.1 0.866 / 0.911, 192 -> 96
+    // 1.309 / 1.337, 192 -> 48 1.075 / 1.101, 176.4 -> 48 0.975 / 0.995
+    KB_ROWMID(POL_DDS, 280, 44800, 2, 256, false, 8, 4, 4, 4, 5),   // 88.2 kHz -> 44.1 kHz: row pairs
+    KB_ROWMID(POL_DDS, 400, 44800, 2, 256, false, 8, 4, 4, 4, 7),   // 176.4 kHz -> 44.1 kHz: row pairs
+    KB_ROWMID(POL_DDS, 392, 48020, 1, 512, false, 8, 2, 5, 7, 7),   // 192 kHz -> 48 kHz: units of 4 rows
+    KB_ROWMID(POL_DDS, 392, 96040, 2, 512, false, 8, 2, 5, 7, 7),   // 192 kHz -> 96 kHz: row pairs
+    KB_ROWMID(POL_DDS, 405, 48600, 2, 384, false, 8, 3, 3, 7, 7),   // 176.4 kHz -> 48 kHz: row pairs
+    // not registered (same bits, slower than the two kernels, where the split/fold step's items take two or three
+    // rounds of the workgroup): 176.4 kHz -> 96 kHz (405 x 441, hinv 97200) 1.26 against 1.228 ms, 96 -> 48 kHz (280
+    // x 343, units of 4 rows) 0.72 / 0.685, 96 -> 88.2 kHz (250 x 400, units of 4 rows) 0.90 / 0.827; stereo 44.1 kHz
+    // -> 48 kHz exact (upsampling; 245-point rows, units of 6 rows r + 63 m and 63 - r + 63 m): 0.469-0.565 ms per
+    // launch on 320-512 threads against 0.434 for the two kernels, its radix-5 and -7 stages filling 41-57% of the
+    // lanes of six rows
 };
 
 }  // namespace
 
 bool rowmid_available(int policy, int n2, int n1, int hinv) {
     for (const RowMidEntry& e : kRowMid)
-        if (e.policy == policy && e.n2 == n2 && e.n1 == n1 && gcd_int(n1, hinv) == e.tt) return true;
+        if (e.policy == policy && e.n2 == n2 && e.n1 == n1 && e.hinv == hinv) return true;
     return false;
 }
 
@@ -201,7 +215,7 @@ hipError_t launch_rowmid(int policy, const PassArgs& a, const MidArgs& m, hipStr
     int skip = 0;
     if (const char* v = std::getenv("KBSRC_ROWMID_VARIANT")) skip = std::max(0, std::atoi(v));
     for (const RowMidEntry& e : kRowMid)
-        if (e.policy == policy && e.n2 == a.sd.n && e.n1 == a.nsub && gcd_int(e.n1, m.hinv) == e.tt && skip-- <= 0)
+        if (e.policy == policy && e.n2 == a.sd.n && e.n1 == a.nsub && e.hinv == m.hinv && skip-- <= 0)
             return e.launch(a, m, s);
     return hipErrorInvalidValue;
 }

@@ -205,12 +205,16 @@ def _rowmid_run(x, rin, rout, mode, block, lpc=True):
         return out.cpu().numpy(), t, info
 
 
-@pytest.mark.parametrize("rin,rout,mode,block,nvar", [(96000, 44100, "exact", 630, 6), (96000, 44100, "fast", 250, 5)])
+@pytest.mark.parametrize("rin,rout,mode,block,nvar", [(96000, 44100, "exact", 630, 6), (96000, 44100, "fast", 250, 5),
+                                                      (88200, 44100, "exact", 89600, 1), (176400, 44100, "exact", 89600, 1),
+                                                      (192000, 48000, "exact", 96040, 1), (192000, 96000, "exact", 192080, 1),
+                                                      (176400, 48000, "exact", 2430, 1)])
 def test_row_pass_with_split_fold_gives_the_separate_bits(rin, rout, mode, block, nvar, monkeypatch):
     """The forward row pass fused with the split/multiply/fold step (one launch, the forward spectrum kept in LDS)
     against the row pass and the split/fold kernel it replaces (KBSRC_ROWMID=0), every registered workgroup shape, on
     the registered transform shapes (96 kHz -> 44.1 kHz exact: 315 x 320 forward, row pairs; fast: 200 x 200, units of
-    16 rows) with 1, 2 and 3 channels of several blocks, LPC edges on and off."""
+    16 rows; the other downsampling pairs' shapes of long inputs, at their block sizes) with 1, 2 and 3 channels of
+    several blocks, LPC edges on and off."""
     monkeypatch.delenv("KBSRC_ROWMID", raising=False)
     monkeypatch.delenv("KBSRC_ROWMID_VARIANT", raising=False)
     rng = np.random.default_rng(23)

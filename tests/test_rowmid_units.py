@@ -123,7 +123,10 @@ def unit_maps(N1, TT, U, hinv):
 
 @pytest.mark.parametrize("N1,TT,U,hinv", [(315, 315, 1, 46305), (315, 315, 2, 46305), (315, 315, 3, 46305),
                                           (315, 315, 4, 46305), (200, 25, 1, 18375), (200, 25, 2, 18375),
-                                          (189, 63, 1, 50400), (24, 8, 1, 1000), (24, 12, 2, 1200), (40, 10, 3, 990)])
+                                          (189, 63, 1, 50400), (280, 280, 2, 44800), (400, 400, 2, 44800),
+                                          (280, 140, 1, 48020), (392, 392, 2, 96040), (392, 196, 1, 48020),
+                                          (405, 405, 2, 48600), (405, 405, 2, 97200), (250, 125, 1, 91875),
+                                          (24, 8, 1, 1000), (24, 12, 2, 1200), (40, 10, 3, 990)])
 def test_workgroups_cover_rows_and_pairs_once(N1, TT, U, hinv):
     groups = unit_maps(N1, TT, U, hinv)
     rows = sorted(r for g, _ in groups for r in g.values())
