@@ -111,7 +111,15 @@ __global__ __launch_bounds__(T, 4) void k_rowmid(PassArgs a, MidArgs m, int ngrp
         if (2 * u > TT || (side && (u == 0 || 2 * u == TT))) continue;
         const int k = (side ? TT - u : u) + TT * mm;
         if (k > half) continue;
-        mid_pair<S>(m, zat, k, put);
+        if constexpr (RU::NQ == 1) {   // row pairs: row u in slot 2 ul, row N1 - u in slot 2 ul + 1 (RowUnits::slot)
+            auto zpair = [&](int i) -> Cx<S> {
+                const int e = i / N1, j = i - e * N1;
+                return L.get(e * G + 2 * ul + (j != u ? 1 : 0));
+            };
+            mid_pair<S>(m, zpair, k, put);
+        } else {
+            mid_pair<S>(m, zat, k, put);
+        }
     }
 }
 

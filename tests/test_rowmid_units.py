@@ -106,6 +106,8 @@ def unit_maps(N1, TT, U, hinv):
             m2, v = divmod(r, TT)   # RowUnits::slot
             uu = min(v, TT - v)
             assert (uu - grp * U) * Q + (0 if v == uu else 1) * NQ + m2 == g
+            if NQ == 1:   # the row-pair shortcut of the item loop: row u in slot 2 ul, row N1 - u in 2 ul + 1
+                assert 2 * ul + (1 if r != u else 0) == g
             rows[g] = r
         ks = []
         for w in range(U * 2 * mc):
