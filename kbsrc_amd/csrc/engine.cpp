@@ -992,10 +992,17 @@ void Engine::run(const double* xx, double* out, hipStream_t s, PassTimes* times,
             mr.z = nullptr;
             mr.zi = W0;
             zi = W0;
-            timed(c, &local.fwd_row, [&] { hip_check(launch_rowmid(pol_four, b, mr, s), "row pass with the split/fold"); });
+            timed(c, &local.fwd_row, [&] {
+                hip_check(b.split ? launch_rowmid_split(b, mr, s) : launch_rowmid(pol_four, b, mr, s),
+                          "row pass with the split/fold");
+            });
         };
-        const bool rowmid = !split && use_fixed() && !global_forced() && !Jg_ && fw_.c.four && !fw_.c.blue &&
-                            rowmid_available(pol_four, fw_.c.N2, fw_.c.N1, iv_.h) && !env_off("KBSRC_ROWMID");
+        const char* sg = std::getenv("KBSRC_SPLIT_GLOBAL");   // a split route forced: the separate kernels
+        const bool rowmid = use_fixed() && !global_forced() && !Jg_ && fw_.c.four && !fw_.c.blue &&
+                            !env_off("KBSRC_ROWMID") &&
+                            (split ? dd_ && !(sg && (sg[0] == '1' || sg[0] == '2')) &&
+                                         rowmid_split_available(fw_.c.N2, fw_.c.N1, iv_.h, split)
+                                   : rowmid_available(pol_four, fw_.c.N2, fw_.c.N1, iv_.h));
         double* const sf[3] = {W1, W2, W3};
         a.zkeep = zkeep_;
         a.zh = fw_.h;

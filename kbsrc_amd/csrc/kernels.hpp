@@ -236,6 +236,10 @@ hipError_t launch_mid(int policy, const MidArgs& a, hipStream_t s);
 // row pass (its load side and stage tables; the prune window set), m: the split/fold step (zi, its destination).
 bool rowmid_available(int policy, int n2, int n1, int hinv);
 hipError_t launch_rowmid(int policy, const PassArgs& a, const MidArgs& m, hipStream_t s);
+// Its split-precision form (kernels_rowmid_split.hip): a row pass of the split form (a.split slices, a.adig,
+// a.tw16 set), the rows' first two radix-4 stages as the int8-MFMA radix-16 stage.
+bool rowmid_split_available(int n2, int n1, int hinv, int slices);
+hipError_t launch_rowmid_split(const PassArgs& a, const MidArgs& m, hipStream_t s);
 // One launch over several plan shapes' passes (device arrays: descriptors d[0..n), workgroup starts[0..n) with shape s
 // taking [starts[s], starts[s + 1]) of total): the generic sub-FFT pass (each d[s] with its G, ngrp set for 256
 // threads; lds = the largest tile) and the split/multiply/fold kernel (mid_workgroups each).

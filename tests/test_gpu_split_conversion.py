@@ -107,9 +107,10 @@ def test_split_form_rejections():
 
 @pytest.mark.parametrize("fi,fo,block", [(96000, 44100, 630), (44100, 48000, 630), (384000, 44100, 1536)])
 def test_split_routes_give_the_same_bits(fi, fo, block, monkeypatch):
-    """The split form's three routes compute the same operations and give the same bits: the specialised split
-    passes (320- and 224-point sub-FFTs), the generic LDS pass with the matrix stage (KBSRC_SPLIT_GLOBAL=2), and
-    the global-memory route (gather, stage, tail pass, scatter; KBSRC_SPLIT_GLOBAL=1)."""
+    """The split form's routes compute the same operations and give the same bits: the specialised split passes
+    (320- and 224-point sub-FFTs; on 96 kHz -> 44.1 kHz the forward row pass fused with the split/fold step, and the
+    two as separate kernels with KBSRC_ROWMID=0), the generic LDS pass with the matrix stage (KBSRC_SPLIT_GLOBAL=2),
+    and the global-memory route (gather, stage, tail pass, scatter; KBSRC_SPLIT_GLOBAL=1)."""
     import torch
     k = gpu()
     x = np.random.default_rng(fi + fo).uniform(-0.5, 0.5, (2 * fi, 2))
@@ -117,19 +118,21 @@ def test_split_routes_give_the_same_bits(fi, fo, block, monkeypatch):
     d_pad = torch.zeros(2 * plan.info["xstride"], dtype=torch.float64, device="cuda")
     d_out = torch.empty(2 * plan.info["nout"], dtype=torch.float64, device="cuda")
     plan.stage(x, d_pad.data_ptr())
-    for s in (6, 15):
+    for s in (6, 7, 8, 15) if fi == 96000 else (6, 15):
         outs = []
-        for route in (None, "2", "1"):
+        for route in (None, "rowmid0", "2", "1"):
             say(f"{fi} -> {fo}: split S={s}, route {route or 'specialised'}")
-            if route:
+            monkeypatch.delenv("KBSRC_SPLIT_GLOBAL", raising=False)
+            monkeypatch.delenv("KBSRC_ROWMID", raising=False)
+            if route == "rowmid0":
+                monkeypatch.setenv("KBSRC_ROWMID", "0")
+            elif route:
                 monkeypatch.setenv("KBSRC_SPLIT_GLOBAL", route)
-            else:
-                monkeypatch.delenv("KBSRC_SPLIT_GLOBAL", raising=False)
             d_out.zero_()
             plan.execute_split(d_pad.data_ptr(), d_out.data_ptr(), s)
             torch.cuda.synchronize()
             outs.append(d_out.cpu().numpy().copy())
-        assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2]), s
+        assert all(np.array_equal(outs[0], o) for o in outs[1:]), s
     plan.close()
 
 
