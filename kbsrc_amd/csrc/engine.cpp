@@ -699,6 +699,7 @@ void run_cfft(const CFFTPlan& f, const RunCtx& c, int ld, const double* src, int
         a.tw4 = f.tw4.d(); a.twmode = lane ? 1 : 2; a.renorm = lane && c.dd;
         a.f.M = f.N2;
         a.zkeep = -1;   // the column pass's outputs all feed the rows
+        if (const char* v = std::getenv("KBSRC_COL_TGROUP")) a.tgroup = std::max(0, std::atoi(v));   // tuning knob
         // a long column sub-FFT as two specialised passes through the row pass's destination, free until the rows
         // (KBSRC_LONG_PARTS=0: always the one-pass kernel)
         const bool split_stage = a.split > 0 && a.sd.nst >= 2 && a.sd.st[0].r == 4 && a.sd.st[1].r == 4;
@@ -716,6 +717,7 @@ void run_cfft(const CFFTPlan& f, const RunCtx& c, int ld, const double* src, int
         b.src = scr[0]; b.src_tstride = ts; b.lde = 1; b.ldg = f.N2;
         b.dst = dst; b.dst_tstride = ts; b.ste = f.N1; b.stg = 1; b.renorm = lane && c.dd;
         b.f.M = f.N1;
+        if (const char* v = std::getenv("KBSRC_ROW_TGROUP")) b.tgroup = std::max(0, std::atoi(v));   // tuning knob
         if (c.rows && !c.collect && st == ST_CPLX) (*c.rows)(b);
         else do_pass(c, t_row, pol_four, LD_CPLX, st, b, "row pass");
     } else {

@@ -244,6 +244,20 @@ __device__ __forceinline__ void last_stage(const Lds<A, RL::N * G>& L, int tid, 
     }
 }
 
+// Workgroup wg's task and sub-FFT group in a launch of ntask x ngrp: tasks slowest (tgroup <= 0), or in groups of
+// tgroup tasks, each group's sub-FFT groups in turn with its tasks fastest.
+__device__ __forceinline__ void task_group_order(int wg, int ntask, int ngrp, int tgroup, int& task, int& grp) {
+    if (tgroup <= 0) {
+        task = wg / ngrp;
+        grp = wg - task * ngrp;
+        return;
+    }
+    const int per = tgroup * ngrp, blk = wg / per, t0 = blk * tgroup, gs = min(tgroup, ntask - t0);
+    const int r = wg - blk * per;
+    grp = r / gs;
+    task = t0 + (r - grp * gs);
+}
+
 // Workgroup index remapped so that each of the 8 XCDs (which take workgroups round-robin) works on a contiguous
 // range of the grid: neighbouring sub-FFT groups share cache lines in that XCD's L2.
 __device__ __forceinline__ int xcd_wg() {
@@ -435,7 +449,8 @@ __global__ __launch_bounds__(T, 4) void k_fixed(PassArgs a) {
     const Lds<A, CAP> L{lds};
     const int tid = threadIdx.x;
     const int wg = xcd_wg();
-    const int task = wg / a.ngrp, grp = wg - task * a.ngrp;
+    int task, grp;
+    task_group_order(wg, a.ntask, a.ngrp, a.tgroup, task, grp);
     const int j0 = grp * G;
     const TaskIO io = task_io(a.tdesc, a.task0 + task, a.nblk, a.in_cstride, a.in_bstride, a.nout);
     // sub-FFT index of local g, or -1

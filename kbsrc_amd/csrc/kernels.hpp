@@ -96,6 +96,10 @@ struct PassArgs {
     // [wstart, wstart + wlen) mod sd.n, wlen < 0 for all. The specialised passes run only the butterflies whose
     // outputs reach it.
     int wstart = 0, wlen = -1;
+    // > 0: workgroups take the tasks in groups of tgroup, each group's sub-FFT groups in turn with its tasks fastest
+    // (task_group_order), so that the tasks of one sub-FFT group, which read the same twiddles (and, for the
+    // forward columns, overlapping input), run back to back on one XCD; 0: tasks slowest
+    int tgroup = 0;
 };
 // true when a pass with zkeep set may skip storing output element i
 __host__ __device__ inline bool z_unused(const PassArgs& a, long long i) { return a.zkeep >= 0 && i > a.zkeep && i < a.zh - a.zkeep; }

@@ -34,13 +34,10 @@ template <int N1, int TT, int U> struct RowUnits {
 };
 
 // Workgroup wg's task and unit group: tasks in groups of tgroup, each group's unit groups in turn with its tasks
-// fastest, so that the tasks of one unit group, which read the same entries of the kernel spectrum and the split
-// twiddles (the same residues mod t), run one after another on one XCD (xcd_wg) and find them in its L2.
+// fastest (task_group_order), so that the tasks of one unit group, which read the same entries of the kernel spectrum
+// and the split twiddles (the same residues mod t), run one after another on one XCD (xcd_wg) and find them in its L2.
 __device__ __forceinline__ void rowmid_order(int wg, int ntask, int ngrp, int tgroup, int& task, int& grp) {
-    const int per = tgroup * ngrp, blk = wg / per, t0 = blk * tgroup, gs = min(tgroup, ntask - t0);
-    const int r = wg - blk * per;
-    grp = r / gs;
-    task = t0 + (r - grp * gs);
+    task_group_order(wg, ntask, ngrp, tgroup, task, grp);
 }
 
 // After the row sub-FFTs of unit group grp (local sub-FFT g = RowUnits::row(g, grp)) are in the LDS tile L, in
