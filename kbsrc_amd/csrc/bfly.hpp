@@ -96,12 +96,15 @@ __device__ __forceinline__ void butterfly(const Cx<A> (&x)[R], Cx<A> (&out)[R], 
         out[0] = sum;
 #pragma unroll
         for (int u = 1; u <= H; ++u) {
-            Cx<A> ac = x[0], bc = {A::lift(0.0), A::lift(0.0)};
+            Cx<A> ac = x[0], bc;
 #pragma unroll
             for (int t = 1; t <= H; ++t) {
                 const Cx<A> cs = gld<A>(oddc, ri * 7 + (t * u) % R);
                 ac = {A::add(ac.re, A::mul(S[t - 1].re, cs.re)), A::add(ac.im, A::mul(S[t - 1].im, cs.re))};
-                bc = {A::add(bc.re, A::mul(D[t - 1].re, cs.im)), A::add(bc.im, A::mul(D[t - 1].im, cs.im))};
+                if (t == 1)   // the sum starts at zero: zero plus the first product (A::add0)
+                    bc = {A::add0(A::mul(D[0].re, cs.im)), A::add0(A::mul(D[0].im, cs.im))};
+                else
+                    bc = {A::add(bc.re, A::mul(D[t - 1].re, cs.im)), A::add(bc.im, A::mul(D[t - 1].im, cs.im))};
             }
             Cx<A> xu = {A::add(ac.re, bc.im), A::sub(ac.im, bc.re)};
             Cx<A> xr = {A::sub(ac.re, bc.im), A::add(ac.im, bc.re)};

@@ -59,6 +59,11 @@ KB_HD dd ddmul1(dd a, double b) {   // dd x double (src/dsp/double_double.hpp:77
     return quick2(p, e);
 }
 KB_HD dd sub_acc(dd a, dd b) { return add_acc(a, ddneg(b)); }
+// add_slp({0, 0}, b) and add_acc({0, 0}, b) for b a ddmul result, in two operations: ddmul's result is normalised
+// (fl(b.hi + b.lo) == b.hi), so both adds return b, except that a negative zero component comes back positive, which
+// adding 0.0 to each component reproduces (checked over subnormal, zero and cancelling operands by
+// tests/dd_zero_add.cpp).
+KB_HD dd zero_add_product(dd b) { return dd{b.hi + 0.0, b.lo + 0.0}; }
 KB_HD dd dddiv1(dd a, double b) {   // dd / double, three quotient digits (src/dsp/double_double.hpp:80-84)
     double q1 = a.hi / b;
     dd r = sub_acc(a, ddmul1(mkdd(b), q1));
@@ -90,6 +95,7 @@ struct F64 {   // Fast mode: plain FP64
     KB_HD static T neg(T a) { return -a; }
     KB_HD static T lift(double v) { return v; }
     KB_HD static T from(dd v) { return v.hi; }   // from_dd<double> (src/dsp/fft.hpp:16)
+    KB_HD static T add0(T b) { return 0.0 + b; }   // add(lift(0), b) for b a mul result
 };
 struct DDA {   // dd with the accurate add
     using T = dd;
@@ -100,6 +106,7 @@ struct DDA {   // dd with the accurate add
     KB_HD static T neg(T a) { return ddneg(a); }
     KB_HD static T lift(double v) { return mkdd(v); }
     KB_HD static T from(dd v) { return v; }
+    KB_HD static T add0(T b) { return zero_add_product(b); }   // add(lift(0), b) for b a mul result
 };
 struct DDS {   // dd with the one-TwoSum add of the reference's 4-lane sub-FFTs
     using T = dd;
@@ -110,6 +117,7 @@ struct DDS {   // dd with the one-TwoSum add of the reference's 4-lane sub-FFTs
     KB_HD static T neg(T a) { return ddneg(a); }
     KB_HD static T lift(double v) { return mkdd(v); }
     KB_HD static T from(dd v) { return v; }
+    KB_HD static T add0(T b) { return zero_add_product(b); }   // add(lift(0), b) for b a mul result
 };
 
 template <class A> KB_HD cx<typename A::T> c_add(cx<typename A::T> a, cx<typename A::T> b) {
