@@ -2,6 +2,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <numeric>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -882,6 +883,20 @@ Engine::Engine(const Plan& p, const double* H, long long nH, int Jg, hipStream_t
         g_[0].alloc(per_task * tb_);
         g_[1].alloc(per_task * tb_);
     }
+    // the fused row pass's tables in residue-major order, where its kernel reads them so (made once per plan, so
+    // that executes allocate nothing)
+    if (fw_.c.four && !fw_.c.blue && !Jg_ && rowmid_residue_tables(dd_ ? POL_DDS : POL_F64, fw_.c.N2, fw_.c.N1, iv_.h)) {
+        const int t = std::gcd(fw_.c.N1, iv_.h);
+        const long long n = std::max({nH, (long long)fw_.h + 1, (long long)iv_.h + 1});
+        Lr_ = (int)((n + t - 1) / t);
+        const int er = dd_ ? 2 : 1, ec = dd_ ? 4 : 2;   // doubles per real and per complex entry
+        Hr_.alloc((size_t)t * Lr_ * er * sizeof(double));
+        wfr_.alloc((size_t)t * Lr_ * ec * sizeof(double));
+        wir_.alloc((size_t)t * Lr_ * ec * sizeof(double));
+        hip_check(launch_residue_major(H_.d(), nH, er, t, Lr_, Hr_.d(), s), "residue-major H");
+        hip_check(launch_residue_major(fw_.w.d(), fw_.h + 1, ec, t, Lr_, wfr_.d(), s), "residue-major wf");
+        hip_check(launch_residue_major(iv_.w.d(), iv_.h + 1, ec, t, Lr_, wir_.d(), s), "residue-major wi");
+    }
     hip_check(hipStreamSynchronize(s), "engine set-up");   // H and the tables are in place; the caller may free H
 }
 
@@ -980,6 +995,7 @@ void Engine::run(const double* xx, double* out, hipStream_t s, PassTimes* times,
         m.jmax = std::min(p_.Nb / 2, p_.Mb - 1); m.up = p_.up; m.ntask = nt; m.Jg = Jg_;
         m.kchain = kchain_;
         m.tgroup = tgroup_;
+        m.Hr = Hr_.d(); m.wfr = wfr_.d(); m.wir = wir_.d(); m.Lr = Lr_;
         // forward real FFT: padded input -> z (W0); the last pass stores only the bins the bin map reads. Where the
         // plan allows, the row pass and the split/fold step run as one launch: column output (W1) -> z' (W0)
         double* zi = W1;

@@ -245,6 +245,8 @@ private:
     DeviceConsts k_;
     RFFTPlan fw_, iv_;
     DevBuf H_, w_[4], g_[2];
+    DevBuf Hr_, wfr_, wir_;   // residue-major H, wf and wi for the fused row pass (MidArgs::Hr, wfr, wir)
+    int Lr_ = 0;
     std::map<int, DevBuf> adig_;   // split form: the DFT matrix's slices per slice count
 };
 

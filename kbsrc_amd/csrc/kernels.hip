@@ -318,7 +318,7 @@ __global__ __launch_bounds__(256) void k_subfft_multi(const PassArgs* __restrict
 
 // forward split X[i] of a global array of the complex FFT's bins (the spectrum design's k_spec_split)
 template <class A> __device__ __forceinline__ Cx<A> fwd_split(const double* z, const double* w, int h, int i) {
-    return fwd_split_at<A>([z](int k) { return gload<A>(z, k); }, w, h, i);
+    return fwd_split_at<A>([z](int k) { return gload<A>(z, k); }, MidTabs{nullptr, w, nullptr}, h, i);
 }
 
 // Bin block of the q-th block in the launch order. With c = kchain > 0 the nk blocks run as the columns of a
@@ -364,7 +364,8 @@ template <class A> __device__ __forceinline__ void mid_body(const MidArgs& a, un
     if (k > a.hinv / 2) return;
     const double* z = a.z + task * a.z_tstride;
     double* zi = a.zi + task * a.zi_tstride;
-    mid_pair<A>(a, [z](int i) { return gload<A>(z, i); }, k, [zi](int i, Cx<A> v) { gstore_stream<A>(zi, i, v); });
+    mid_pair<A>(a, [z](int i) { return gload<A>(z, i); }, MidTabs{a.H, a.wf, a.wi}, k,
+                [zi](int i, Cx<A> v) { gstore_stream<A>(zi, i, v); });
 }
 
 // Workgroups are dealt to the 8 XCDs round-robin; with tgroup > 0 each XCD takes a contiguous range of the launch

@@ -118,6 +118,11 @@ struct MidArgs {
     int Jg = 0;                     // > 0: general fold of H[0..Jg] on the finer kernel grid (near-unity ratios)
     int kchain = 0;                 // > 0: bin blocks kchain apart run one after another (mid_block)
     int tgroup = 0;                 // > 0: tasks in groups of tgroup, each group's bin blocks in turn (mid_order)
+    // H, wf and wi residue-major for the fused row pass (entry i at (i mod t) Lr + i / t, t = gcd(N1, hinv)), or null
+    const double* Hr = nullptr;
+    const double* wfr = nullptr;
+    const double* wir = nullptr;
+    int Lr = 0;
 };
 
 // Bluestein (chirp-z) steps of a complex FFT whose length n has a prime factor above 7 (src/dsp/fft.hpp:228-278),
@@ -243,6 +248,10 @@ hipError_t launch_rowmid(int policy, const PassArgs& a, const MidArgs& m, hipStr
 // Its split-precision form (kernels_rowmid_split.hip): a row pass of the split form (a.split slices, a.adig,
 // a.tw16 set), the rows' first two radix-4 stages as the int8-MFMA radix-16 stage.
 bool rowmid_split_available(int n2, int n1, int hinv, int slices);
+// The plan shapes above whose fused kernels read the residue-major tables (MidArgs::Hr, wfr, wir; t = gcd(n1, hinv)).
+bool rowmid_residue_tables(int policy, int n2, int n1, int hinv);
+// dst[((i mod t) L + i / t) e + c] = src[i e + c] for i < n, c < e (doubles per entry)
+hipError_t launch_residue_major(const double* src, long long n, int e, int t, int L, double* dst, hipStream_t s);
 hipError_t launch_rowmid_split(const PassArgs& a, const MidArgs& m, hipStream_t s);
 // One launch over several plan shapes' passes (device arrays: descriptors d[0..n), workgroup starts[0..n) with shape s
 // taking [starts[s], starts[s + 1]) of total): the generic sub-FFT pass (each d[s] with its G, ngrp set for 256

@@ -20,7 +20,7 @@ __device__ __forceinline__ void later_stages(const Lds<DDS, RL::N * G>& L, int t
 }
 
 // two waves per SIMD, as the split row pass: the matrix stage's slicing needs up to ~150 VGPRs at 15 slices
-template <class RL, int N1, int TT, int U, int T, int SPL>
+template <class RL, int N1, int TT, int U, int T, int SPL, bool RES>
 __global__ __launch_bounds__(T, 2) void k_rowmid_split(PassArgs a, MidArgs m, int ngrp, int tgroup) {
     using A = DDS;
     using RU = RowUnits<N1, TT, U>;
@@ -74,7 +74,7 @@ __global__ __launch_bounds__(T, 2) void k_rowmid_split(PassArgs a, MidArgs m, in
     const Prune pr{a.wstart, a.wlen};
     const OwnTw tw{a.tw};
     later_stages<RL, G, T>(L, tid, tw, a.oddc, pr, std::make_integer_sequence<int, RL::nst - 2>{});
-    rowmid_tail<A, RL, N1, TT, U, T, false>(L, a, m, task, grp, tid);
+    rowmid_tail<A, RL, N1, TT, U, T, false, RES>(L, a, m, task, grp, tid);
 }
 
 template <class K> hipError_t lds_attr(K kernel, size_t bytes) {
@@ -83,7 +83,7 @@ template <class K> hipError_t lds_attr(K kernel, size_t bytes) {
                                (int)bytes);
 }
 
-template <class RL, int N1, int TT, int U, int T, int SPL, int TG>
+template <class RL, int N1, int TT, int U, int T, int SPL, int TG, bool RES>
 hipError_t launch_rowmid_split_one(const PassArgs& a, const MidArgs& m, hipStream_t s) {
     using RU = RowUnits<N1, TT, U>;
     static_assert(N1 % TT == 0, "a unit is whole residue classes of the rows");
@@ -93,18 +93,18 @@ hipError_t launch_rowmid_split_one(const PassArgs& a, const MidArgs& m, hipStrea
     static_assert(bytes <= 160 * 1024, "LDS tile too large");
     if (a.sd.n != RL::N || a.sd.nst != RL::nst || a.nsub != N1 || a.twmode != 0 || a.split != SPL || !a.adig ||
         !a.tw16 || m.Jg || m.Mb != 2 * m.hinv || m.h != N1 * RL::N || std::gcd(N1, m.hinv) != TT || a.ntask < 1 ||
-        m.ntask != a.ntask)
+        m.ntask != a.ntask || (RES && (!m.Hr || !m.wfr || !m.wir || (long long)m.Lr * TT < (long long)m.h + 1)))
         return hipErrorInvalidValue;
     for (int i = 0; i < RL::nst; ++i)
         if (a.sd.st[i].r != RL::r[i] || a.sd.st[i].s != RL::stride(i)) return hipErrorInvalidValue;
-    static const hipError_t attr = lds_attr(&k_rowmid_split<RL, N1, TT, U, T, SPL>, bytes);
+    static const hipError_t attr = lds_attr(&k_rowmid_split<RL, N1, TT, U, T, SPL, RES>, bytes);
     if (attr != hipSuccess) return attr;
     const int ngrp = (RU::nunit + U - 1) / U;
     const long long nwg = (long long)a.ntask * ngrp;
     if (nwg > 0x7fffffffLL) return hipErrorInvalidValue;
     int tgroup = TG;
     if (const char* v = std::getenv("KBSRC_ROWMID_TGROUP")) tgroup = std::max(1, std::atoi(v));   // tuning knob
-    hipLaunchKernelGGL((k_rowmid_split<RL, N1, TT, U, T, SPL>), dim3((unsigned)nwg), dim3(T), bytes, s, a, m, ngrp,
+    hipLaunchKernelGGL((k_rowmid_split<RL, N1, TT, U, T, SPL, RES>), dim3((unsigned)nwg), dim3(T), bytes, s, a, m, ngrp,
                        std::min(tgroup, a.ntask));
     return hipGetLastError();
 }
@@ -114,17 +114,17 @@ struct RowMidSplitEntry {
     hipError_t (*launch)(const PassArgs&, const MidArgs&, hipStream_t);
 };
 
-#define KB_ROWMID_SPLIT(N1, HINV, U, T, SPL, TG, ...)                                                              \
+#define KB_ROWMID_SPLIT(N1, HINV, U, T, SPL, TG, RES, ...)                                                         \
     RowMidSplitEntry{Radices<__VA_ARGS__>::N, N1, HINV, SPL,                                                       \
-                     &launch_rowmid_split_one<Radices<__VA_ARGS__>, N1, std::gcd(N1, HINV), U, T, SPL, TG>}
+                     &launch_rowmid_split_one<Radices<__VA_ARGS__>, N1, std::gcd(N1, HINV), U, T, SPL, TG, RES>}
 
 // The plain form's benchmark shape (stereo 96 kHz -> 44.1 kHz exact: 320-point rows, row pairs), at the slice counts
 // of the split form's sweep; the workgroup shape of the split row pass (4 rows on 256 threads).
 const RowMidSplitEntry kRowMidSplit[] = {
-    KB_ROWMID_SPLIT(315, 46305, 2, 256, 6, 8, 4, 4, 4, 5),
-    KB_ROWMID_SPLIT(315, 46305, 2, 256, 7, 8, 4, 4, 4, 5),
-    KB_ROWMID_SPLIT(315, 46305, 2, 256, 8, 8, 4, 4, 4, 5),
-    KB_ROWMID_SPLIT(315, 46305, 2, 256, 15, 8, 4, 4, 4, 5),
+    KB_ROWMID_SPLIT(315, 46305, 2, 256, 6, 8, true, 4, 4, 4, 5),
+    KB_ROWMID_SPLIT(315, 46305, 2, 256, 7, 8, true, 4, 4, 4, 5),
+    KB_ROWMID_SPLIT(315, 46305, 2, 256, 8, 8, true, 4, 4, 4, 5),
+    KB_ROWMID_SPLIT(315, 46305, 2, 256, 15, 8, true, 4, 4, 4, 5),
 };
 
 }  // namespace

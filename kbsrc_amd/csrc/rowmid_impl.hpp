@@ -42,8 +42,9 @@ __device__ __forceinline__ void rowmid_order(int wg, int ntask, int ngrp, int tg
 
 // After the row sub-FFTs of unit group grp (local sub-FFT g = RowUnits::row(g, grp)) are in the LDS tile L, in
 // natural order: renormalise them once as the row pass's store does (a.renorm), then run the split/fold step of every
-// inverse-input pair (k, hinv - k) with k in the group's units, writing the inverse FFT's input (m.zi, task's part).
-template <class A, class RL, int N1, int TT, int U, int T, bool NTS>
+// inverse-input pair (k, hinv - k) with k in the group's units, writing the inverse FFT's input (m.zi, task's part);
+// RES: its tables from the residue-major copies (MidArgs::Hr, wfr, wir), whose entries for one residue are adjacent.
+template <class A, class RL, int N1, int TT, int U, int T, bool NTS, bool RES>
 __device__ __forceinline__ void rowmid_tail(const Lds<A, RL::N * RowUnits<N1, TT, U>::G>& L, const PassArgs& a,
                                             const MidArgs& m, int task, int grp, int tid) {
     using RU = RowUnits<N1, TT, U>;
@@ -90,9 +91,11 @@ __device__ __forceinline__ void rowmid_tail(const Lds<A, RL::N * RowUnits<N1, TT
                 const int e = i / N1, j = i - e * N1;
                 return L.get(e * G + 2 * ul + (j != u ? 1 : 0));
             };
-            mid_pair<S>(m, zpair, k, put);
+            if constexpr (RES) mid_pair<S>(m, zpair, MidTabsRes<TT>{m.Hr, m.wfr, m.wir, m.Lr}, k, put);
+            else mid_pair<S>(m, zpair, MidTabs{m.H, m.wf, m.wi}, k, put);
         } else {
-            mid_pair<S>(m, zat, k, put);
+            if constexpr (RES) mid_pair<S>(m, zat, MidTabsRes<TT>{m.Hr, m.wfr, m.wir, m.Lr}, k, put);
+            else mid_pair<S>(m, zat, MidTabs{m.H, m.wf, m.wi}, k, put);
         }
     }
 }
