@@ -109,7 +109,10 @@ struct RowMidEntry {
 // group of the workgroup order (rowmid_order), measured: exact 1.617, 1.601, 1.595 and 1.589 ms per conversion at 1,
 // 2, 4 and 8; fast 0.423, 0.432, 0.449 and 0.486 ms per launch. RES: the split/fold step reads the residue-major
 // tables (MidArgs::Hr, wfr, wir), whose entries for one residue are adjacent: the fast plan's fused launch 0.425 ->
-// 0.324 ms (its table loads were one cache line per lane), exact 0.625 -> 0.61-0.62 ms, the other shapes 0-2%.
+// 0.324 ms (its table loads were one cache line per lane), exact 0.625 -> 0.61-0.62 ms, the other shapes 0-2%. With
+// the residue-major tables, measured against the defaults below: fast 0.373 ms on 256 threads, 0.326 on 768, 0.350
+// in task groups of 4, 0.445 for two units on 1024 threads; exact 0.62-0.63 ms for 3 pairs on 512 threads, 0.612 in
+// task groups of 4, against 0.60-0.61.
 const RowMidEntry kRowMid[] = {
     // stereo 96 kHz -> 44.1 kHz exact: 320-point rows, row pairs (r, 315 - r); per launch (214 tasks) 0.62-0.635 ms
     // for 2 pairs on 256 threads (0.654-0.670 before the in-place renormalisation and the task groups; then 3 pairs
