@@ -232,6 +232,10 @@ def test_row_pass_with_split_fold_gives_the_separate_bits(rin, rout, mode, block
             assert t["mid"] == 0 and t["fwd_row"] > 0   # the fused launch ran
             assert y.tobytes() == ref.tobytes()
         monkeypatch.delenv("KBSRC_ROWMID_VARIANT")
+        monkeypatch.setenv("KBSRC_BATCH_TASKS", "3")   # several launches per execute (tasks from a later offset)
+        y, t, _ = _rowmid_run(x, rin, rout, mode, block, lpc)
+        assert t["mid"] == 0 and y.tobytes() == ref.tobytes()
+        monkeypatch.delenv("KBSRC_BATCH_TASKS")
 
 
 def test_extreme_ratio_full_length_pins():

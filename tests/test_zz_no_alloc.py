@@ -28,13 +28,13 @@ def lib():
     return L
 
 
-def run_both(L, frames, ch, rin, rout, precision, seed, length=0.2):
+def run_both(L, frames, ch, rin, rout, precision, seed, length=0.2, block=0):
     import torch
 
     import kbsrc_amd
     say(f"noalloc vs execute: {ch} ch {rin}->{rout} {frames} frames {precision}")
     x = np.random.default_rng(seed).uniform(-0.5, 0.5, (frames, ch))
-    plan = kbsrc_amd.Plan(frames, ch, rin, rout, precision, length=length)
+    plan = kbsrc_amd.Plan(frames, ch, rin, rout, precision, length=length, block=block)
     info = plan.info
     xx = torch.empty(ch * info["xstride"], dtype=torch.float64, device="cuda")
     plan.stage(x, xx.data_ptr())
@@ -64,6 +64,9 @@ def test_noalloc_equals_execute_twice_second_smaller():
 def test_noalloc_fast_bluestein_and_general_fold():
     L = lib()
     run_both(L, 96000 * 5, 2, 96000, 44100, "fast", 3)
+    # the benchmark's block: the forward rows and the split/fold as one kernel, 6.4 MB per task, several launches
+    run_both(L, 96000 * 20, 2, 96000, 44100, "exact", 6, block=630)
+    run_both(L, 96000 * 20, 2, 96000, 44100, "fast", 7, block=250)
     run_both(L, 20000, 1, 96000, 32001, "exact", 4, length=0.05)    # Bluestein lengths: four work buffers
     run_both(L, 20000, 2, 44100, 44101, "exact", 5, length=0.05)    # general fold
 
