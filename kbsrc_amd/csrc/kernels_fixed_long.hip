@@ -165,12 +165,15 @@ bool long_two_part(int policy, int n, int ld, int st) {
 
 hipError_t launch_long_two_part(int policy, int ld, int st, const PassArgs& a, double* scr, long long scr_ts,
                                 hipStream_t s) {
-    int skip = 0;
+    int skip = 0;   // KBSRC_LONG_VARIANT=k: the k-th entry of the shape, the first where it has fewer
     if (const char* v = std::getenv("KBSRC_LONG_VARIANT")) skip = std::max(0, std::atoi(v));
+    const LongEntry* first = nullptr;
     for (const LongEntry& e : kLong)
-        if (e.policy == policy && e.n == a.sd.n && e.ld == ld && e.st == st && skip-- <= 0)
-            return e.launch(a, scr, scr_ts, s);
-    return hipErrorInvalidValue;
+        if (e.policy == policy && e.n == a.sd.n && e.ld == ld && e.st == st) {
+            if (!first) first = &e;
+            if (skip-- <= 0) return e.launch(a, scr, scr_ts, s);
+        }
+    return first ? first->launch(a, scr, scr_ts, s) : hipErrorInvalidValue;
 }
 
 }  // namespace kbsrc::gpu

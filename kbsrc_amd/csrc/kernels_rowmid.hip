@@ -112,7 +112,8 @@ struct RowMidEntry {
 // 0.324 ms (its table loads were one cache line per lane), exact 0.625 -> 0.61-0.62 ms, the other shapes 0-2%. With
 // the residue-major tables, measured against the defaults below: fast 0.373 ms on 256 threads, 0.326 on 768, 0.350
 // in task groups of 4, 0.445 for two units on 1024 threads; exact 0.62-0.63 ms for 3 pairs on 512 threads, 0.612 in
-// task groups of 4, against 0.60-0.61.
+// task groups of 4, against 0.60-0.61; two pairs on 320 threads at 5 waves per SIMD (84 VGPRs, no spills) 0.70 ms,
+// at 4 waves 0.76, two pairs on 256 threads at 5 waves (a 52-byte spill) 0.65.
 const RowMidEntry kRowMid[] = {
     // stereo 96 kHz -> 44.1 kHz exact: 320-point rows, row pairs (r, 315 - r); per launch (214 tasks) 0.62-0.635 ms
     // for 2 pairs on 256 threads (0.654-0.670 before the in-place renormalisation and the task groups; then 3 pairs
@@ -171,12 +172,15 @@ bool rowmid_available(int policy, int n2, int n1, int hinv) {
 }
 
 hipError_t launch_rowmid(int policy, const PassArgs& a, const MidArgs& m, hipStream_t s) {
-    int skip = 0;
+    int skip = 0;   // KBSRC_ROWMID_VARIANT=k: the k-th entry of the shape, the first where it has fewer
     if (const char* v = std::getenv("KBSRC_ROWMID_VARIANT")) skip = std::max(0, std::atoi(v));
+    const RowMidEntry* first = nullptr;
     for (const RowMidEntry& e : kRowMid)
-        if (e.policy == policy && e.n2 == a.sd.n && e.n1 == a.nsub && e.hinv == m.hinv && skip-- <= 0)
-            return e.launch(a, m, s);
-    return hipErrorInvalidValue;
+        if (e.policy == policy && e.n2 == a.sd.n && e.n1 == a.nsub && e.hinv == m.hinv) {
+            if (!first) first = &e;
+            if (skip-- <= 0) return e.launch(a, m, s);
+        }
+    return first ? first->launch(a, m, s) : hipErrorInvalidValue;
 }
 
 }  // namespace kbsrc::gpu
