@@ -76,6 +76,23 @@ def test_split_fold_reads_only_its_units_rows(rin, rout, mode, Nb, Mb, N1, up):
         assert rows <= {k % t, (-k) % t}, (k, rows)
 
 
+@pytest.mark.parametrize("rin,rout,mode,n1,n2,hinv", [(96000, 44100, "exact", 315, 320, 46305),
+                                                      (96000, 44100, "fast", 200, 200, 18375),
+                                                      (88200, 44100, "exact", 280, 320, 44800),
+                                                      (176400, 44100, "exact", 400, 448, 44800),
+                                                      (192000, 48000, "exact", 392, 490, 48020),
+                                                      (192000, 96000, "exact", 392, 490, 96040),
+                                                      (176400, 48000, "exact", 405, 441, 48600)])
+def test_registered_fused_shapes_are_these_plans(rin, rout, mode, n1, n2, hinv):
+    """The fused kernel's registered shapes (kernels_rowmid.hip: row count, row length, hinv) are the plans of these
+    rate pairs for long inputs (60 s stereo), and their units fit the kernel: t = gcd(N1, hinv) gives row pairs or
+    units of 4 (exact) / 16 (fast) rows."""
+    p = H.oracle_plan(60 * rin, 2, rin, rout, mode)
+    assert four_step_n1(p["Nb"] // 2) == n1 and p["Nb"] // 2 == n1 * n2 and p["Mb"] // 2 == hinv
+    assert not p["up"] and not p["gen"]
+    assert 2 * n1 // gcd(n1, hinv) in ((2, 4) if mode == "exact" else (16,))
+
+
 def test_registered_shapes_are_benchmark_plans():
     """The shapes the kernel table holds: stereo 96 kHz -> 44.1 kHz exact (315 x 320, t = 315: row pairs) and fast
     (200 x 200, t = 25: units of 16 rows); and stereo 44.1 kHz -> 48 kHz exact (189 x 245, t = 63: units of 6 rows),
