@@ -212,6 +212,7 @@ const FixedEntry* fixed_table_auto0(int* count);
 const FixedEntry* fixed_table_auto1(int* count);
 const FixedEntry* fixed_table_auto2(int* count);
 const FixedEntry* fixed_table_auto3(int* count);
+const FixedEntry* fixed_table_auto4(int* count);
 const FixedEntry* fixed_table_extreme(int* count);
 // specialised split-form passes (kernels_fixed_split.hip): exact-mode (POL_DDS) sub-FFT passes with the first two
 // radix-4 stages as the int8-MFMA radix-16 stage; nullptr when none is built for (n, ld, st, slices)

@@ -94,7 +94,8 @@ struct Table {
 std::vector<Table> tables() {   // the measured table first: its entries are the defaults where both have a shape
     std::vector<Table> t;
     t.push_back({kEntries, (int)(sizeof(kEntries) / sizeof(kEntries[0]))});
-    for (auto f : {fixed_table_auto0, fixed_table_auto1, fixed_table_auto2, fixed_table_auto3, fixed_table_extreme}) {
+    for (auto f : {fixed_table_auto0, fixed_table_auto1, fixed_table_auto2, fixed_table_auto3, fixed_table_auto4,
+                   fixed_table_extreme}) {
         int n = 0;
         const FixedEntry* e = f(&n);
         t.push_back({e, n});
