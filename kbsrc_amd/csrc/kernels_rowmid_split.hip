@@ -93,7 +93,9 @@ hipError_t launch_rowmid_split_one(const PassArgs& a, const MidArgs& m, hipStrea
     static_assert(bytes <= 160 * 1024, "LDS tile too large");
     if (a.sd.n != RL::N || a.sd.nst != RL::nst || a.nsub != N1 || a.twmode != 0 || a.split != SPL || !a.adig ||
         !a.tw16 || m.Jg || m.Mb != 2 * m.hinv || m.h != N1 * RL::N || std::gcd(N1, m.hinv) != TT || a.ntask < 1 ||
-        m.ntask != a.ntask || (RES && (!m.Hr || !m.wfr || !m.wir || (long long)m.Lr * TT < (long long)m.h + 1)))
+        m.ntask != a.ntask || (RES && (!m.Hr || !m.wfr || !m.wir || m.Lr != RU::NQ * RL::N + 1 ||
+                (long long)m.Lr * TT * 32 > 0xFFFFFFFFLL)) ||
+        m.Mb > (int)kDivSmallMax || m.h > (int)kDivSmallMax)   // the bin indices of the tail (div_small)
         return hipErrorInvalidValue;
     for (int i = 0; i < RL::nst; ++i)
         if (a.sd.st[i].r != RL::r[i] || a.sd.st[i].s != RL::stride(i)) return hipErrorInvalidValue;

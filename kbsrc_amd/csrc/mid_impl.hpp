@@ -31,6 +31,27 @@ template <class A> __device__ __forceinline__ MVal<A> mid_ldr(const double* base
         return base[i];
     }
 }
+// the same with the byte offset formed in 32 bits (below 4 GiB)
+template <class A> __device__ __forceinline__ MCx<A> mid_ldc32(const double* base, unsigned i) {
+    const char* b = reinterpret_cast<const char*>(base);
+    if constexpr (A::kDD) {
+        const double2* p = reinterpret_cast<const double2*>(b + (size_t)(i * 32u));
+        const double2 h = p[0], l = p[1];
+        return {dd{h.x, l.x}, dd{h.y, l.y}};
+    } else {
+        const double2 v = *reinterpret_cast<const double2*>(b + (size_t)(i * 16u));
+        return {v.x, v.y};
+    }
+}
+template <class A> __device__ __forceinline__ MVal<A> mid_ldr32(const double* base, unsigned i) {
+    const char* b = reinterpret_cast<const char*>(base);
+    if constexpr (A::kDD) {
+        const double2 v = *reinterpret_cast<const double2*>(b + (size_t)(i * 16u));
+        return dd{v.x, v.y};
+    } else {
+        return *reinterpret_cast<const double*>(b + (size_t)(i * 8u));
+    }
+}
 
 // The step's tables: the kernel spectrum H, the forward split twiddles wf and the inverse pre-split twiddles wi,
 // entry i at i (natural order), or residue-major for a workgroup that reads only the residues +-r mod T (the fused
@@ -43,18 +64,31 @@ struct MidTabs {
     template <class A> __device__ __forceinline__ MCx<A> fwd(int i) const { return mid_ldc<A>(wf, i); }
     template <class A> __device__ __forceinline__ MCx<A> inv(int i) const { return mid_ldc<A>(wi, i); }
 };
-template <int T> struct MidTabsRes {
+// floor(u / D) for 0 <= u <= kDivSmallMax by one FP32 product and a truncation, no integer multiply: (u + 1/2) (1/D)
+// rounded twice (1/D, then the fma) is within (u + 1/2) 2^-23 / D of (u + 1/2) / D, less than the (1/2) / D between
+// that value and either integer around it while u + 1/2 < 2^22.
+constexpr unsigned kDivSmallMax = (1u << 22) - 2;
+template <unsigned D> __device__ __forceinline__ unsigned div_small(unsigned u) {
+    constexpr float inv = 1.0f / D;
+    return (unsigned)__builtin_fmaf((float)u, inv, 0.5f * inv);
+}
+
+// The residue-major form, L entries per residue (a compile-time count): positions and byte offsets in 32 bits, the
+// index arithmetic on 24-bit products (indices are non-negative and at most kDivSmallMax, each table below 4 GiB:
+// the launcher checks both), so that the table base stays in scalar registers.
+template <int T, int L> struct MidTabsRes {
     const double* H;
     const double* wf;
     const double* wi;
-    int L;
-    __device__ __forceinline__ long long pos(int i) const {
-        const int q = i / T;
-        return (long long)(i - q * T) * L + q;
+    __device__ __forceinline__ unsigned pos(int i) const {
+        const unsigned u = (unsigned)i, q = div_small<T>(u);
+        return __umul24(u - __umul24(q, T), L) + q;
     }
-    template <class A> __device__ __forceinline__ MVal<A> kern(int i) const { return mid_ldr<A>(H, pos(i)); }
-    template <class A> __device__ __forceinline__ MCx<A> fwd(int i) const { return mid_ldc<A>(wf, pos(i)); }
-    template <class A> __device__ __forceinline__ MCx<A> inv(int i) const { return mid_ldc<A>(wi, pos(i)); }
+    template <class A> __device__ __forceinline__ MVal<A> kern(int i) const {
+        return mid_ldr32<A>(H, pos(i));
+    }
+    template <class A> __device__ __forceinline__ MCx<A> fwd(int i) const { return mid_ldc32<A>(wf, pos(i)); }
+    template <class A> __device__ __forceinline__ MCx<A> inv(int i) const { return mid_ldc32<A>(wi, pos(i)); }
 };
 
 // ---- forward split X[i] from the complex FFT output, z(k) = bin k (src/dsp/fft.hpp:365-407)

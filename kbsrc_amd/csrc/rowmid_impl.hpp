@@ -25,7 +25,7 @@ template <int N1, int TT, int U> struct RowUnits {
     static __device__ __forceinline__ int slot(int j, int grp) {
         int mm = 0, v = j;
         if constexpr (NQ > 1) {
-            mm = j / TT;
+            mm = (int)div_small<TT>((unsigned)j);
             v = j - mm * TT;
         }
         const int u = v <= TT - v ? v : TT - v, side = v == u ? 0 : 1;
@@ -51,6 +51,7 @@ __device__ __forceinline__ void rowmid_tail(const Lds<A, RL::N * RowUnits<N1, TT
     using S = SplitPol<A>;
     static_assert(std::is_same_v<Val<S>, Val<A>>);
     constexpr int G = RU::G, CAP = RL::N * G;
+    constexpr int LR = RU::NQ * RL::N + 1;   // residue-major entries per residue: h / t + 1 (MidArgs::Lr)
     auto jof = [&](int g) -> int { return RU::row(g, grp); };
     if (a.renorm) {   // the row pass's stores renormalise each element: once here, in place
         for (int w = tid; w < CAP; w += T)
@@ -59,9 +60,9 @@ __device__ __forceinline__ void rowmid_tail(const Lds<A, RL::N * RowUnits<N1, TT
     }
 
     // forward bin i = row j, element e, as the row pass would have stored it
-    auto zat = [&](int i) -> Cx<S> {
-        const int e = i / N1, j = i - e * N1;
-        return L.get(e * G + RU::slot(j, grp));
+    auto zat = [&](int i) -> Cx<S> {   // bin i = row j, element e (i <= kDivSmallMax: the launcher checks)
+        const unsigned e = div_small<N1>((unsigned)i), j = (unsigned)i - __umul24(e, N1);
+        return L.get((int)e * G + RU::slot((int)j, grp));
     };
     double* zi = m.zi + task * m.zi_tstride;
     auto put = [&](int i, Cx<S> v) {
@@ -88,13 +89,13 @@ __device__ __forceinline__ void rowmid_tail(const Lds<A, RL::N * RowUnits<N1, TT
         if (k > half) continue;
         if constexpr (RU::NQ == 1) {   // row pairs: row u in slot 2 ul, row N1 - u in slot 2 ul + 1 (RowUnits::slot)
             auto zpair = [&](int i) -> Cx<S> {
-                const int e = i / N1, j = i - e * N1;
-                return L.get(e * G + 2 * ul + (j != u ? 1 : 0));
+                const unsigned e = div_small<N1>((unsigned)i), j = (unsigned)i - __umul24(e, N1);
+                return L.get((int)e * G + 2 * ul + (j != (unsigned)u ? 1 : 0));
             };
-            if constexpr (RES) mid_pair<S>(m, zpair, MidTabsRes<TT>{m.Hr, m.wfr, m.wir, m.Lr}, k, put);
+            if constexpr (RES) mid_pair<S>(m, zpair, MidTabsRes<TT, LR>{m.Hr, m.wfr, m.wir}, k, put);
             else mid_pair<S>(m, zpair, MidTabs{m.H, m.wf, m.wi}, k, put);
         } else {
-            if constexpr (RES) mid_pair<S>(m, zat, MidTabsRes<TT>{m.Hr, m.wfr, m.wir, m.Lr}, k, put);
+            if constexpr (RES) mid_pair<S>(m, zat, MidTabsRes<TT, LR>{m.Hr, m.wfr, m.wir}, k, put);
             else mid_pair<S>(m, zat, MidTabs{m.H, m.wf, m.wi}, k, put);
         }
     }

@@ -5,10 +5,12 @@ split/fold step of every inverse-input pair (k, hinv - k) with k = +-r mod t. Ch
   split pairs bin i with h - i, src/dsp/fft.hpp:365-407; the bin map folds Mb - j onto j, src/resampler.cpp:64-78;
   the inverse pre-split pairs k with hinv - k, src/dsp/fft.hpp:412-458; upsampling extends the spectrum periodically,
   src/resampler.cpp:72-77), over the plans of many rate pairs, down and up;
-- the kernel's slot and item maps (RowUnits::row / slot, the item loop) cover every row once and every pair once.
+- the kernel's slot and item maps (RowUnits::row / slot, the item loop) cover every row once and every pair once;
+- the FP32 quotient its index arithmetic uses (div_small) is exact for every index up to its bound.
 No GPU."""
 from math import gcd
 
+import numpy as np
 import pytest
 
 import helpers as H
@@ -174,3 +176,28 @@ def rowmid_order(wg, ntask, ngrp, tgroup):
 def test_workgroup_order_is_a_bijection(ntask, ngrp, tgroup):
     seen = [rowmid_order(w, ntask, ngrp, min(tgroup, ntask)) for w in range(ntask * ngrp)]
     assert sorted(seen) == [(t, g) for t in range(ntask) for g in range(ngrp)]
+
+
+def div_small(u, d):
+    """The kernel's quotient floor(u / d) (div_small in kbsrc_amd/csrc/mid_impl.hpp): fl32((u + 1/2) fl32(1 / d)),
+    one FP32 fma, truncated. The fma's single rounding is emulated exactly: u fl32(1/d) + fl32(1/d) / 2 needs at
+    most 47 significant bits for u < 2^22, so the float64 sum is exact and one float32 rounding follows."""
+    inv = np.float32(1.0) / np.float32(d)
+    y = (u.astype(np.float64) * np.float64(inv) + np.float64(np.float32(0.5) * inv)).astype(np.float32)
+    return y.astype(np.int64)
+
+
+K_DIV_SMALL_MAX = (1 << 22) - 2
+
+
+@pytest.mark.parametrize("d", [315, 200, 25, 280, 400, 392, 196, 405, 1, 3, 7, 4095])
+def test_float_quotient_is_exact_below_its_bound(d):
+    """Every u in [0, kDivSmallMax]: the row and residue divisors of the registered shapes, and a few others."""
+    u = np.arange(0, K_DIV_SMALL_MAX + 1, dtype=np.int64)
+    assert np.array_equal(div_small(u, d), u // d)
+
+
+def test_float_quotient_fails_above_its_bound():
+    """The bound is needed: above 2^22 the same product truncates to a wrong quotient for some u."""
+    u = np.arange(K_DIV_SMALL_MAX + 1, 1 << 25, dtype=np.int64)
+    assert np.count_nonzero(div_small(u, 315) != u // 315) > 0
